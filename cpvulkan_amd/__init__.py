@@ -1,0 +1,150 @@
+"""cpvulkan_amd -- MI355X-native draw path of the CPVulkan CPU Vulkan driver.
+
+The product is native: libcpv.so (HIP kernels + the C-ABI of include/cpv.h)
+and the Vulkan ICD built on it.  This package holds the build recipe, the
+ctypes binding of the C-ABI (used by the benchmark and the tests) and the
+synthetic workload generators of the benchmark configurations.
+
+Nothing here falls back to a CPU path: loading the library or creating a
+context on a machine without the built library or without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+from . import abi
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+LIBCPV = LIB_DIR / "libcpv.so"
+
+_lib = None
+
+
+def _declare(lib: C.CDLL) -> None:
+    P = C.c_void_p
+    lib.cpv_last_error.restype = C.c_char_p
+    lib.cpv_context_create.argtypes = [C.c_int, C.POINTER(P)]
+    lib.cpv_context_destroy.argtypes = [P]
+    lib.cpv_context_destroy.restype = None
+    lib.cpv_alloc.argtypes = [P, C.c_uint64, C.POINTER(C.c_uint64)]
+    lib.cpv_free.argtypes = [P, C.c_uint64]
+    lib.cpv_upload.argtypes = [P, C.c_uint64, P, C.c_uint64]
+    lib.cpv_download.argtypes = [P, P, C.c_uint64, C.c_uint64]
+    lib.cpv_synchronize.argtypes = [P]
+    lib.cpv_set_config.argtypes = [P, C.POINTER(abi.Config)]
+    lib.cpv_get_config.argtypes = [P, C.POINTER(abi.Config)]
+    lib.cpv_render_pass.argtypes = [P, C.POINTER(abi.PassDesc), C.POINTER(abi.PipelineDesc), C.POINTER(abi.DrawDesc),
+                                    C.POINTER(C.c_uint32), C.c_uint32]
+    lib.cpv_render_jobs.argtypes = [P, C.POINTER(abi.JobDesc), C.c_uint32]
+    lib.cpv_last_fragment_count.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.cpv_last_timing.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    lib.cpv_program_create_spirv.argtypes = [P, P, C.c_size_t, C.c_char_p, P, C.c_size_t, C.c_char_p, C.POINTER(P)]
+    lib.cpv_program_destroy.argtypes = [P, P]
+    lib.cpv_program_destroy.restype = None
+    lib.cpv_program_source.argtypes = [P]
+    lib.cpv_program_source.restype = C.c_char_p
+    lib.cpv_abi_sizes.argtypes = [C.POINTER(C.c_uint32), C.c_uint32]
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libcpv.so (built by cpvulkan_amd.build) and check its ABI."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIBCPV
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built; run python -m cpvulkan_amd.build")
+    lib = C.CDLL(str(p))
+    _declare(lib)
+    abi.check_layout(lib)
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class CpvError(RuntimeError):
+    pass
+
+
+def _check(lib: C.CDLL, rc: int, what: str) -> None:
+    if rc != 0:
+        raise CpvError(f"{what} failed ({rc}): {lib.cpv_last_error().decode()}")
+
+
+class Context:
+    """One GPU context of libcpv (a HIP stream and the pipeline's work buffers)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        self.handle = C.c_void_p()
+        _check(self.lib, self.lib.cpv_context_create(device, C.byref(self.handle)), "cpv_context_create")
+        self._allocs: list[int] = []
+
+    def close(self) -> None:
+        if self.handle:
+            for a in self._allocs:
+                self.lib.cpv_free(self.handle, a)
+            self._allocs.clear()
+            self.lib.cpv_context_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # memory
+    def alloc(self, nbytes: int) -> int:
+        addr = C.c_uint64()
+        _check(self.lib, self.lib.cpv_alloc(self.handle, max(int(nbytes), 1), C.byref(addr)), "cpv_alloc")
+        self._allocs.append(addr.value)
+        return addr.value
+
+    def free(self, addr: int) -> None:
+        self._allocs.remove(addr)
+        _check(self.lib, self.lib.cpv_free(self.handle, addr), "cpv_free")
+
+    def upload(self, addr: int, data) -> None:
+        import numpy as np
+        a = np.ascontiguousarray(data)
+        _check(self.lib, self.lib.cpv_upload(self.handle, addr, a.ctypes.data, a.nbytes), "cpv_upload")
+
+    def download(self, addr: int, nbytes: int):
+        import numpy as np
+        out = np.empty(int(nbytes), dtype=np.uint8)
+        _check(self.lib, self.lib.cpv_download(self.handle, out.ctypes.data, addr, out.nbytes), "cpv_download")
+        return out
+
+    def synchronize(self) -> None:
+        _check(self.lib, self.lib.cpv_synchronize(self.handle), "cpv_synchronize")
+
+    # configuration
+    def set_config(self, tile_size=16, waves_per_tile=4, scheduler=0, split_threshold=0, record_costs=0) -> None:
+        cfg = abi.Config(tile_size, waves_per_tile, scheduler, split_threshold, record_costs)
+        _check(self.lib, self.lib.cpv_set_config(self.handle, C.byref(cfg)), "cpv_set_config")
+
+    # rendering
+    def render_pass(self, pass_desc, pipelines, draws, draw_pipeline=None) -> None:
+        n = len(draws)
+        parr = (abi.PipelineDesc * max(len(pipelines), 1))(*pipelines)
+        darr = (abi.DrawDesc * max(n, 1))(*draws)
+        dp = (C.c_uint32 * max(n, 1))(*(draw_pipeline or [0] * n))
+        rc = self.lib.cpv_render_pass(self.handle, C.byref(pass_desc), parr, darr, dp, n)
+        _check(self.lib, rc, "cpv_render_pass")
+
+    def render_jobs(self, jobs) -> None:
+        arr = (abi.JobDesc * max(len(jobs), 1))(*jobs)
+        _check(self.lib, self.lib.cpv_render_jobs(self.handle, arr, len(jobs)), "cpv_render_jobs")
+
+    def fragment_count(self) -> int:
+        v = C.c_uint64()
+        _check(self.lib, self.lib.cpv_last_fragment_count(self.handle, C.byref(v)), "cpv_last_fragment_count")
+        return v.value
+
+    def timing(self) -> tuple[float, float]:
+        r, t = C.c_float(), C.c_float()
+        _check(self.lib, self.lib.cpv_last_timing(self.handle, C.byref(r), C.byref(t)), "cpv_last_timing")
+        return r.value, t.value

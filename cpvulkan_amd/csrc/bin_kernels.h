@@ -1,0 +1,326 @@
+// Shader-independent kernels: triangle setup, prefix scans and the
+// order-preserving tile binning (emit + stable LSD radix sort by tile).
+//
+// Binning must keep submission order inside every tile: blending and the
+// LESS depth test are order dependent, and the reference draws primitives
+// strictly in order (CPVulkan/CommandBuffer.Draw.cpp:1526-1593).  Pairs
+// (tile, primitive) are emitted in primitive order and then sorted by tile
+// with a stable radix sort whose per-wave ranks come from wavefront ballots.
+#pragma once
+#pragma clang fp contract(off)
+
+#include "raster_device.h"
+
+namespace cpv {
+
+// --------------------------------------------------------------- setup
+
+struct SetupArgs {
+    const DrawDesc* draws;
+    const uint32_t* block_draw;   // setup-grid block -> draw
+    const TargetParams* targets;
+    const float* vout;
+    TriSetup* recs;
+    uint32_t* counts;             // tiles overlapped per primitive (0 = dropped)
+    uint32_t stride;
+    int32_t tile_shift;
+};
+
+// EdgeFunction(a, b, c) with c the pixel centre, stored as a, (b.y - a.y), (b.x - a.x).
+__device__ __forceinline__ void put_edge(TriSetup& r, int k, const float* a, const float* b)
+{
+    r.ax[k] = a[0];
+    r.ay[k] = a[1];
+    r.dy[k] = b[1] - a[1];
+    r.dx[k] = b[0] - a[0];
+}
+
+// Per-primitive part of ProcessTriangles (CommandBuffer.Draw.cpp:1526-1569)
+// and the per-triangle constants of GetFragmentInput (:879-898, :900 cull).
+__global__ __launch_bounds__(256) void setup_kernel(SetupArgs a)
+{
+    const uint32_t di = a.block_draw[blockIdx.x];
+    const DrawDesc& d = a.draws[di];
+    const uint32_t p = (blockIdx.x - d.sblock_base) * 256u + threadIdx.x;
+    if (p >= d.prim_count) return;
+    const uint32_t gp = d.prim_base + p;
+
+    // CalculatePrimitives (:567-673)
+    uint32_t raw[3], prov;
+    if (d.topology == 3) { raw[0] = 3 * p; raw[1] = 3 * p + 1; raw[2] = 3 * p + 2; prov = 3 * p; }
+    else if (d.topology == 4) { raw[0] = p; raw[1] = p + 1; raw[2] = p + 2; prov = p; }
+    else { raw[0] = 0; raw[1] = p + 1; raw[2] = p + 2; prov = p + 1; }
+
+    uint32_t slot[4];
+    const uint32_t rs[4] = {raw[0], raw[1], raw[2], prov};
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int64_t s = rs[k];
+        if (d.vmode == kSlotVertexId) {
+            uint32_t idx;
+            if (d.index_size == 1) idx = ((const uint8_t*)d.index_addr)[d.first + rs[k]];
+            else if (d.index_size == 2) idx = ((const uint16_t*)d.index_addr)[d.first + rs[k]];
+            else idx = ((const uint32_t*)d.index_addr)[d.first + rs[k]];
+            s = (int64_t)d.vertex_offset + (int64_t)idx;
+        }
+        ok = ok && s >= 0 && s < (int64_t)d.vout_count;
+        slot[k] = d.vout_base + (uint32_t)s;
+    }
+    if (!ok) { a.counts[gp] = 0; return; }   // vertex outside the bound buffer
+
+    uint32_t i0 = slot[0], i1 = slot[1], i2 = slot[2];
+    if (d.front_face == 1) { const uint32_t t = i0; i0 = i2; i2 = t; }   // CLOCKWISE swaps p0, p2
+
+    float P[3][4];
+    const uint32_t ids[3] = {i0, i1, i2};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float4 q = *(const float4*)(a.vout + (uint64_t)ids[k] * a.stride);
+        P[k][0] = q.x / q.w;
+        P[k][1] = q.y / q.w;
+        P[k][2] = q.z / q.w;
+        P[k][3] = q.w;
+    }
+    int32_t sx[3], sy[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        sx[k] = trunc_to_i32((P[k][0] + 1.0f) * 0.5f * d.vp_w);
+        sy[k] = trunc_to_i32((P[k][1] + 1.0f) * 0.5f * d.vp_h);
+    }
+    const int32_t mnx = min(sx[0], min(sx[1], sx[2])), mxx = max(sx[0], max(sx[1], sx[2]));
+    const int32_t mny = min(sy[0], min(sy[1], sy[2])), mxy = max(sy[0], max(sy[1], sy[2]));
+    const int32_t x0 = max(0, mnx), y0 = max(0, mny);
+    const int32_t x1 = (int32_t)min((int64_t)d.limit_x, (int64_t)mxx + 1);
+    const int32_t y1 = (int32_t)min((int64_t)d.limit_y, (int64_t)mxy + 1);
+
+    TriSetup r;
+    float area = (P[2][0] - P[0][0]) * (P[1][1] - P[0][1]) - (P[2][1] - P[0][1]) * (P[1][0] - P[0][0]);
+    bool front;
+    if (area < 0) {
+        area = -area;
+        front = false;
+        put_edge(r, 0, P[2], P[1]);
+        put_edge(r, 1, P[0], P[2]);
+        put_edge(r, 2, P[1], P[0]);
+    } else {
+        front = true;
+        put_edge(r, 0, P[1], P[2]);
+        put_edge(r, 1, P[2], P[0]);
+        put_edge(r, 2, P[0], P[1]);
+    }
+    const bool culled = ((d.cull_mode & 2u) && !front) || ((d.cull_mode & 1u) && front);
+    if (culled || x0 >= x1 || y0 >= y1) { a.counts[gp] = 0; return; }
+
+    r.area = area;
+    r.z[0] = P[0][2]; r.z[1] = P[1][2]; r.z[2] = P[2][2];
+    r.w[0] = P[0][3]; r.w[1] = P[1][3]; r.w[2] = P[2][3];
+    r.x0 = x0; r.y0 = y0; r.x1 = x1; r.y1 = y1;
+    r.v[0] = i0; r.v[1] = i1; r.v[2] = i2;
+    r.flags = front ? 1u : 0u;
+    r.provoking = slot[3];
+    r.draw = di;
+    uint4* dst = (uint4*)&a.recs[gp];
+    const uint4* src = (const uint4*)&r;
+#pragma unroll
+    for (int q = 0; q < 8; q++) dst[q] = src[q];
+
+    const int sh = a.tile_shift;
+    const uint32_t tx0 = (uint32_t)x0 >> sh, tx1 = (uint32_t)(x1 - 1) >> sh;
+    const uint32_t ty0 = (uint32_t)y0 >> sh, ty1 = (uint32_t)(y1 - 1) >> sh;
+    a.counts[gp] = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+}
+
+// ---------------------------------------------------------------- scans
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t& total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    uint32_t wave_base = 0;
+    for (int w = 0; w < wave; w++) wave_base += s_tmp[w];
+    total = s_tmp[0] + s_tmp[1] + s_tmp[2] + s_tmp[3];
+    __syncthreads();
+    return wave_base + x - v;
+}
+
+// Sum of each kScanTile-element slab.
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* in, uint32_t n, uint32_t* sums)
+{
+    __shared__ uint32_t s_tmp[4];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) s += (base + i < n) ? in[base + i] : 0u;
+    uint32_t total;
+    block_exclusive_scan(s, s_tmp, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the slab sums in place by one workgroup.
+__global__ __launch_bounds__(kScanThreads) void scan_sums_kernel(uint32_t* sums, uint32_t n)
+{
+    __shared__ uint32_t s_tmp[4];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += kScanThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < n ? sums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, s_tmp, total);
+        if (i < n) sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+// Exclusive scan of every slab plus its slab offset; out[n] = grand total
+// when write_total is set.
+__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(const uint32_t* in, uint32_t n, const uint32_t* sums,
+                                                                 uint32_t* out, int write_total)
+{
+    __shared__ uint32_t s_tmp[4];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        v[i] = (base + i < n) ? in[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t total;
+    uint32_t run = sums[blockIdx.x] + block_exclusive_scan(s, s_tmp, total);
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+    if (write_total && base < n && base + kScanItems >= n) out[n] = run;
+    if (write_total && n == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+}
+
+// ---------------------------------------------------------------- emit
+
+struct EmitArgs {
+    const TriSetup* recs;
+    const uint32_t* counts;
+    const uint32_t* offsets;
+    const DrawDesc* draws;
+    const TargetParams* targets;
+    uint32_t* keys;               // global tile index per pair
+    uint32_t* vals;               // primitive index per pair
+    uint32_t* tile_counts;        // pairs per global tile
+    uint32_t prims;
+    int32_t tile_shift;
+};
+
+// Writes the (tile, primitive) pairs of every primitive at its scanned
+// offset, so the pair array is in primitive order.
+__global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
+{
+    for (uint32_t gp = blockIdx.x * 256u + threadIdx.x; gp < a.prims; gp += gridDim.x * 256u) {
+        const uint32_t c = a.counts[gp];
+        if (c == 0) continue;
+        const TriSetup& r = a.recs[gp];
+        const TargetParams& t = a.targets[a.draws[r.draw].target];
+        const int sh = a.tile_shift;
+        const uint32_t tx0 = (uint32_t)r.x0 >> sh, tx1 = (uint32_t)(r.x1 - 1) >> sh;
+        const uint32_t ty0 = (uint32_t)r.y0 >> sh, ty1 = (uint32_t)(r.y1 - 1) >> sh;
+        uint32_t off = a.offsets[gp];
+        for (uint32_t ty = ty0; ty <= ty1; ty++) {
+            for (uint32_t tx = tx0; tx <= tx1; tx++) {
+                const uint32_t key = t.tile_base + ty * t.tiles_x + tx;
+                a.keys[off] = key;
+                a.vals[off] = gp;
+                atomicAdd(&a.tile_counts[key], 1u);
+                off++;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------- radix sort
+
+// A pass sorts by an 8-bit digit.  The pair array is cut into kSortSegments
+// contiguous segments, one per wavefront; the histogram kernel counts each
+// segment's digits, a scan over (digit, segment) turns the counts into
+// output bases, and the scatter kernel walks each segment in order, ranking
+// equal digits inside every 64-item round with ballots so the pass is
+// stable.
+constexpr int kSortSegments = 4096;
+constexpr int kSortWaves = 4;
+
+__device__ __forceinline__ void segment_range(uint32_t n, uint32_t seg, uint32_t& b, uint32_t& e)
+{
+    const uint32_t per = ((n + kSortSegments - 1) / kSortSegments + 63u) & ~63u;
+    b = min(n, seg * per);
+    e = min(n, b + per);
+}
+
+__global__ __launch_bounds__(64 * kSortWaves) void radix_hist_kernel(const uint32_t* keys, const uint32_t* n_ptr, int shift,
+                                                                     uint32_t* hist)
+{
+    __shared__ uint32_t s_cnt[kSortWaves][256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = lane; i < 256; i += 64) s_cnt[wave][i] = 0;
+    __syncthreads();
+    const uint32_t seg = blockIdx.x * kSortWaves + wave;
+    uint32_t b, e;
+    segment_range(*n_ptr, seg, b, e);
+    for (uint32_t i = b + lane; i < e; i += 64) atomicAdd(&s_cnt[wave][(keys[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    for (int i = lane; i < 256; i += 64) hist[(uint32_t)i * kSortSegments + seg] = s_cnt[wave][i];
+}
+
+__global__ __launch_bounds__(64 * kSortWaves) void radix_scatter_kernel(const uint32_t* keys_in, const uint32_t* vals_in,
+                                                                        uint32_t* keys_out, uint32_t* vals_out,
+                                                                        const uint32_t* n_ptr, int shift,
+                                                                        const uint32_t* base)
+{
+    __shared__ uint32_t s_run[kSortWaves][256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t seg = blockIdx.x * kSortWaves + wave;
+    for (int i = lane; i < 256; i += 64) s_run[wave][i] = base[(uint32_t)i * kSortSegments + seg];
+    __syncthreads();
+    uint32_t b, e;
+    const uint32_t e_total = *n_ptr;
+    segment_range(e_total, seg, b, e);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t i0 = b; i0 < e; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const bool valid = i < e;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t val = valid ? vals_in[i] : 0u;
+        const uint32_t dg = (key >> shift) & 255u;
+        // lanes holding the same digit
+        uint64_t match = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const bool set = (dg >> bit) & 1u;
+            const uint64_t bal = __ballot(valid && set);
+            match &= set ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(match & lt);
+        const int leader = __ffsll((long long)match) - 1;
+        uint32_t run = 0;
+        if (valid) run = s_run[wave][dg];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && lane == leader) s_run[wave][dg] = run + (uint32_t)__popcll(match);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && run + rank < e_total) {
+            keys_out[run + rank] = key;
+            vals_out[run + rank] = val;
+        }
+    }
+}
+
+}  // namespace cpv

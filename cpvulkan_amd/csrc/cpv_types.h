@@ -1,0 +1,128 @@
+// Host/device shared layouts of the raster pipeline.
+//
+// Everything here is plain data that the host fills and the kernels read:
+// the per-format descriptor, the per-triangle setup record written by the
+// setup kernel and read by the fine raster kernel, and the kernel parameter
+// blocks.  Included by the AOT kernels, the hiprtc-compiled program modules
+// and the host runtime.
+#pragma once
+
+#ifndef __HIPCC_RTC__
+#include <stdint.h>
+#endif
+
+namespace cpv {
+
+// Channel base types of FormatInformation (CPVulkanBase/Formats.h:30-103).
+enum BaseType : int32_t {
+    kBaseNone = 0,
+    kBaseUNorm = 1,
+    kBaseSNorm = 2,
+    kBaseUScaled = 3,
+    kBaseSScaled = 4,
+    kBaseUInt = 5,
+    kBaseSInt = 6,
+    kBaseSFloat = 7,
+};
+
+// Pixel format as the kernels see it: byte offsets of R,G,B,A (or -1),
+// element size, total pixel size; depth/stencil byte offsets for
+// depth/stencil formats.  Mirrors the normal and depth/stencil rows of
+// the reference's format table (CPVulkanBase/Formats.cpp:209-444).
+struct FormatDesc {
+    uint32_t vk_format;
+    int32_t base;
+    int32_t element;
+    int32_t total;
+    int32_t off[4];
+    int32_t depth_off;
+    int32_t stencil_off;
+    int32_t depth_kind;   // 0 none, 16 = D16, 24 = D24, 32 = D32
+    int32_t pad;
+};
+
+// Triangle setup record: 32 dwords (128 B, one L2 line).  Edge k is
+// E_k(p) = (p.x - ax[k]) * dy[k] - (p.y - ay[k]) * dx[k], the reference's
+// EdgeFunction (CPVulkan/CommandBuffer.Draw.cpp:410-418) with the
+// per-triangle differences (b.y - a.y), (b.x - a.x) hoisted; the
+// weight order and orientation already follow GetFragmentInput's sign
+// branch (:879-898), so the raster kernel never re-derives them.
+struct TriSetup {
+    float ax[3], ay[3];
+    float dy[3], dx[3];
+    float area;          // |EdgeFunction(p0, p1, p2)|
+    float z[3];          // p_k.z / p_k.w
+    float w[3];          // clip-space w of p_k
+    int32_t x0, y0, x1, y1;   // pixel bounding box, [x0, x1) x [y0, y1)
+    uint32_t v[3];       // vertex-output indices of p0, p1, p2 (after the CW swap)
+    uint32_t flags;      // bit 0: front facing
+    uint32_t provoking;  // vertex-output index of the provoking vertex
+    uint32_t draw;       // draw index within the launch
+    uint32_t pad[3];
+};
+static_assert(sizeof(TriSetup) == 128, "TriSetup must be 128 bytes");
+
+enum { kMaxVaryings = 16 };
+
+// Fixed-function state of one pipeline as the fragment stage needs it.
+struct FragmentState {
+    FormatDesc color;
+    FormatDesc ds;
+    int32_t has_color, has_ds;
+    int32_t depth_test, depth_write, depth_compare, depth_bounds_test, stencil_test;
+    int32_t blend_enable;
+    int32_t src_color, dst_color, color_op, src_alpha, dst_alpha, alpha_op;
+    float min_depth_bounds, max_depth_bounds;
+    float blend_constants[4];
+    // front / back stencil: fail, pass, depth_fail, compare, compare_mask, write_mask, reference
+    int32_t st_front[7];
+    int32_t st_back[7];
+    int32_t origin_upper_left;
+    int32_t pad;
+};
+
+// Per-render-target parameters (one per job in a batch).
+struct TargetParams {
+    uint64_t color_addr;
+    uint64_t ds_addr;
+    uint32_t width, height;        // attachment extent
+    uint32_t tiles_x, tiles_y;
+    uint32_t tile_base;            // first global tile index of this target
+    int32_t color_load, ds_load;   // CPV_LOAD_OP_*
+    int32_t has_color, has_ds;
+    int32_t color_bytes, ds_bytes; // bytes per pixel
+    uint32_t clear_color[4];       // VkClearColorValue bits (float32 / uint32 / int32 by format)
+    uint32_t clear_ds[2];          // VkClearDepthStencilValue: depth bits, stencil
+    FormatDesc color_fmt, ds_fmt;
+};
+
+// Per-draw parameters.  Vertex-output slots and primitive indices of all
+// draws of one launch live in shared arrays; a draw owns the ranges
+// [vout_base, vout_base + vout_count) and [prim_base, prim_base + prim_count).
+struct DrawDesc {
+    uint64_t env;              // device address of the draw's DrawEnv
+    uint64_t index_addr;       // index buffer (indexed draws)
+    uint32_t target;           // TargetParams index
+    uint32_t fstate;           // FragmentState index
+    uint32_t vout_base, vout_count;
+    uint32_t prim_base, prim_count;
+    uint32_t vblock_base, sblock_base;   // first block of this draw in the vertex / setup grids
+    uint32_t topology, front_face, cull_mode;
+    uint32_t vmode;            // vertex slot mode, see VertexSlotMode
+    uint32_t first;            // firstVertex or firstIndex
+    int32_t vertex_offset;
+    uint32_t index_size;
+    uint32_t instance_id;
+    float vp_w, vp_h, vp_min_depth, vp_max_depth;
+    float half_x, half_y;      // (1 / width) * 0.5, (1 / height) * 0.5 (ProcessTriangles :1524)
+    int32_t limit_x, limit_y;  // min(viewport extent, attachment extent)
+};
+
+// How the slot of an assembled vertex is found.
+enum VertexSlotMode : uint32_t {
+    kSlotSequential = 0,   // non-indexed: slot i shades vertex firstVertex + i
+    kSlotVertexId = 1,     // indexed: slot = vertexOffset + index (every buffer vertex shaded once)
+    kSlotPerIndex = 2,     // indexed: slot i shades index i (the reference's per-index shading)
+};
+
+}  // namespace cpv

@@ -1,0 +1,269 @@
+/*
+ * cpv.h -- C-ABI of the MI355X draw path (libcpv.so).
+ *
+ * This is the thin layer between the host C++ Vulkan front end (the ICD,
+ * libcpvulkan_amd_icd.so) and the HIP kernels that replace the reference's
+ * CPU draw loop.  Every entry point takes plain pointers, sizes and POD
+ * structs; no HIP or torch types appear here.
+ *
+ * What each entry point replaces in the reference (paths relative to the
+ * reference repository):
+ *   cpv_render_pass      -> BeginRenderPassCommand::Process clears
+ *                           (CPVulkan/CommandBuffer.cpp:591-640) followed by
+ *                           DrawCommand::Process / DrawIndexedCommand::Process
+ *                           (CPVulkan/CommandBuffer.Draw.cpp:1777-1804, 1840-1864)
+ *                           for every draw recorded inside the render pass.
+ *   cpv_render_jobs      -> many independent render passes replayed by
+ *                           Queue::Submit (CPVulkan/Queue.cpp:11-77), batched
+ *                           into one persistent launch (no reference counterpart
+ *                           for the batching; each job's result equals
+ *                           cpv_render_pass on that job).
+ *   cpv_program_create_spirv -> CompileVertexPipeline / CompileFragmentPipeline
+ *                           (LLVMRuntime/PipelineCompiler.cpp:1801-1832), called
+ *                           from GraphicsPipeline::Create (CPVulkan/Pipeline.cpp:599-714).
+ *   cpv_alloc/cpv_free/cpv_upload/cpv_download -> DeviceMemory allocation and
+ *                           vkMapMemory access (CPVulkan/Device.cpp:97-200).
+ *
+ * Addresses in descriptor structs are device addresses obtained from
+ * cpv_alloc (or any HIP device allocation).  The CPU oracle under oracle/
+ * consumes the same structs with host addresses.
+ *
+ * Every function returns 0 on success or a negative CPV_ERROR_* code; the
+ * message for the last failure on the calling thread is cpv_last_error().
+ */
+#ifndef CPV_H
+#define CPV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cpv_vk_enums.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPV_MAX_VERTEX_BINDINGS 16
+#define CPV_MAX_VERTEX_ATTRIBUTES 16
+#define CPV_MAX_COLOR_ATTACHMENTS 1
+#define CPV_MAX_UNIFORMS 4
+#define CPV_MAX_TEXTURES 4
+#define CPV_MAX_MIP_LEVELS 16
+
+enum {
+    CPV_OK = 0,
+    CPV_ERROR_INVALID = -1,      /* malformed descriptor or argument */
+    CPV_ERROR_UNSUPPORTED = -2,  /* state the reference aborts on, or not built here */
+    CPV_ERROR_DEVICE = -3,       /* HIP runtime failure (VK_ERROR_DEVICE_LOST) */
+    CPV_ERROR_OUT_OF_MEMORY = -4,
+    CPV_ERROR_COMPILE = -5       /* SPIR-V translation or hiprtc failure */
+};
+
+/* Built-in shader programs.  Each mirrors a GLSL shader of the benchmark
+ * workloads; the SPIR-V path (cpv_program_create_spirv) compiles arbitrary
+ * shaders of the supported subset into the same kernel templates. */
+enum {
+    CPV_VS_PASSTHROUGH = 1,   /* in vec4 pos @0, in vec4 color @1; gl_Position = pos; out vec4 = color */
+    CPV_VS_MVP_COLOR = 2,     /* uniform(binding 0) mat4 mvp; gl_Position = mvp * pos; out vec4 = color
+                                 (Samples/15-draw_cube/15-draw_cube.cpp:38-51) */
+    CPV_VS_MVP_LAMBERT = 3    /* in vec3 pos @0, vec3 normal @1, vec2 uv @2; uniform(binding 0) mat4 mvp;
+                                 gl_Position = mvp * vec4(pos, 1); out vec3 normal; out vec2 uv */
+};
+enum {
+    CPV_FS_COLOR = 1,         /* in vec4 color; out vec4 = color (15-draw_cube.cpp:53-61) */
+    CPV_FS_TEX_LAMBERT = 2    /* in vec3 n, in vec2 uv; sampler2D(binding 1); uniform(binding 2) vec4 L;
+                                 out = texture(tex, uv) * max(dot(normalize(n), L.xyz), 0) */
+};
+
+/* Scheduling policies of the tile work queue (cpv_config.scheduler). */
+enum {
+    CPV_SCHED_GRID = 0,       /* one workgroup per tile, hardware dispatch order ("plain" path) */
+    CPV_SCHED_COUNTER = 1,    /* persistent kernel, tiles taken in index order from an atomic counter */
+    CPV_SCHED_PREDICTED = 2   /* persistent kernel, tiles ordered by the learned cost predictor */
+};
+
+typedef struct cpv_context cpv_context;
+typedef struct cpv_program cpv_program;
+
+typedef struct cpv_vertex_binding_desc {
+    uint32_t binding;
+    uint32_t stride;
+    uint32_t input_rate;      /* CPV_INPUT_RATE_* */
+    uint32_t reserved;
+} cpv_vertex_binding_desc;
+
+typedef struct cpv_vertex_attribute_desc {
+    uint32_t location;
+    uint32_t binding;
+    uint32_t format;          /* VkFormat */
+    uint32_t offset;
+} cpv_vertex_attribute_desc;
+
+typedef struct cpv_stencil_state {
+    uint32_t fail_op, pass_op, depth_fail_op, compare_op;
+    uint32_t compare_mask, write_mask, reference, reserved;
+} cpv_stencil_state;
+
+typedef struct cpv_blend_attachment {
+    uint32_t blend_enable;
+    uint32_t src_color_factor, dst_color_factor, color_op;
+    uint32_t src_alpha_factor, dst_alpha_factor, alpha_op;
+    uint32_t color_write_mask;
+} cpv_blend_attachment;
+
+/* Everything GraphicsPipeline::Create freezes (CPVulkan/Pipeline.cpp:599-714,
+ * CPVulkanBase/PipelineState.h:8-163), flattened. */
+typedef struct cpv_pipeline_desc {
+    uint32_t vs_kind;               /* CPV_VS_*, or 0 when program != NULL */
+    uint32_t fs_kind;               /* CPV_FS_*, or 0 when program != NULL */
+    cpv_program* program;           /* SPIR-V program from cpv_program_create_spirv */
+    uint32_t topology;              /* CPV_TOPOLOGY_* */
+    uint32_t primitive_restart;
+    uint32_t binding_count;
+    uint32_t attribute_count;
+    cpv_vertex_binding_desc bindings[CPV_MAX_VERTEX_BINDINGS];
+    cpv_vertex_attribute_desc attributes[CPV_MAX_VERTEX_ATTRIBUTES];
+    uint32_t cull_mode, front_face, polygon_mode, rasterizer_discard;
+    uint32_t depth_clamp, depth_test, depth_write, depth_compare;
+    uint32_t depth_bounds_test, stencil_test, logic_op_enable, origin_upper_left;
+    float min_depth_bounds, max_depth_bounds;
+    cpv_stencil_state front, back;
+    float blend_constants[4];
+    uint32_t color_count;
+    uint32_t color_formats[CPV_MAX_COLOR_ATTACHMENTS];
+    uint32_t depth_stencil_format;  /* CPV_FORMAT_UNDEFINED when absent */
+    cpv_blend_attachment blend[CPV_MAX_COLOR_ATTACHMENTS];
+} cpv_pipeline_desc;
+
+typedef struct cpv_sampler_desc {
+    uint32_t mag_filter, min_filter, mipmap_mode;
+    uint32_t address_u, address_v, address_w;
+    float mip_lod_bias, min_lod, max_lod;
+    uint32_t border_color;
+    uint32_t unnormalized;
+    uint32_t reserved;
+} cpv_sampler_desc;
+
+/* A combined image sampler: image base address (layer 0, level 0), the
+ * linear tightly packed mip chain layout of GetNormalImageSize
+ * (CPVulkanBase/Formats.cpp:455-482), and the sampler. */
+typedef struct cpv_texture_desc {
+    uint64_t address;
+    uint32_t format, width, height, levels;
+    uint32_t base_level, level_count;
+    cpv_sampler_desc sampler;
+} cpv_texture_desc;
+
+typedef struct cpv_draw_desc {
+    uint32_t indexed;
+    uint32_t count;                 /* vertexCount or indexCount */
+    uint32_t instance_count;
+    uint32_t first;                 /* firstVertex or firstIndex */
+    int32_t vertex_offset;
+    uint32_t first_instance;
+    uint32_t index_size;            /* 1, 2 or 4 bytes */
+    uint32_t reserved;
+    uint64_t index_address;         /* already advanced by the bound offset */
+    uint64_t vertex_address[CPV_MAX_VERTEX_BINDINGS];   /* already advanced by bound offsets */
+    uint64_t vertex_bytes[CPV_MAX_VERTEX_BINDINGS];     /* bytes readable from vertex_address */
+    float viewport[6];              /* x, y, width, height, minDepth, maxDepth */
+    uint32_t reserved2[2];
+    uint64_t uniform_address[CPV_MAX_UNIFORMS];         /* uniform buffers of set 0 by binding */
+    cpv_texture_desc textures[CPV_MAX_TEXTURES];        /* combined image samplers of set 0 by binding */
+} cpv_draw_desc;
+
+typedef struct cpv_attachment_desc {
+    uint64_t address;               /* the image view's base level/layer */
+    uint32_t format;
+    uint32_t load_op;               /* CPV_LOAD_OP_* */
+    union {
+        float f32[4];
+        uint32_t u32[4];
+        int32_t i32[4];
+        struct { float depth; uint32_t stencil; } ds;
+    } clear;
+} cpv_attachment_desc;
+
+typedef struct cpv_pass_desc {
+    uint32_t width, height;         /* attachment extent (all attachments share it) */
+    uint32_t color_count;
+    uint32_t has_depth_stencil;
+    cpv_attachment_desc color[CPV_MAX_COLOR_ATTACHMENTS];
+    cpv_attachment_desc depth_stencil;
+} cpv_pass_desc;
+
+/* One independent render job of a batch: a render pass with one draw. */
+typedef struct cpv_job_desc {
+    cpv_pass_desc pass;
+    cpv_pipeline_desc pipeline;
+    cpv_draw_desc draw;
+} cpv_job_desc;
+
+typedef struct cpv_config {
+    uint32_t tile_size;             /* 8, 16 or 32 pixels */
+    uint32_t waves_per_tile;        /* 1, 2 or 4 */
+    uint32_t scheduler;             /* CPV_SCHED_* */
+    uint32_t split_threshold;       /* tile list length above which a tile is split into chunks; 0 = never */
+    uint32_t record_costs;          /* record per-tile cost traces (cpv_cost_trace) */
+    uint32_t reserved[3];
+} cpv_config;
+
+/* Per-tile record of one rendered tile, written when cpv_config.record_costs
+ * is set: the binning statistics that feed the cost predictor and the
+ * measured cost (shader clock cycles spent on the tile). */
+typedef struct cpv_tile_trace {
+    float features[12];
+    float cycles;
+    uint32_t job, tile, reserved;
+} cpv_tile_trace;
+
+/* ---- context and memory ---- */
+const char* cpv_last_error(void);
+int cpv_context_create(int device_ordinal, cpv_context** out);
+void cpv_context_destroy(cpv_context* ctx);
+int cpv_alloc(cpv_context* ctx, uint64_t bytes, uint64_t* address);
+int cpv_free(cpv_context* ctx, uint64_t address);
+int cpv_upload(cpv_context* ctx, uint64_t dst, const void* src, uint64_t bytes);
+int cpv_download(cpv_context* ctx, void* dst, uint64_t src, uint64_t bytes);
+int cpv_synchronize(cpv_context* ctx);
+int cpv_set_config(cpv_context* ctx, const cpv_config* config);
+int cpv_get_config(cpv_context* ctx, cpv_config* config);
+
+/* ---- rendering (asynchronous on the context's stream) ---- */
+int cpv_render_pass(cpv_context* ctx, const cpv_pass_desc* pass,
+                    const cpv_pipeline_desc* pipelines, const cpv_draw_desc* draws,
+                    const uint32_t* draw_pipeline, uint32_t draw_count);
+int cpv_render_jobs(cpv_context* ctx, const cpv_job_desc* jobs, uint32_t job_count);
+
+/* Number of (primitive, pixel) pairs that passed coverage and culling in the
+ * last cpv_render_pass / cpv_render_jobs call (synchronises). */
+int cpv_last_fragment_count(cpv_context* ctx, uint64_t* fragments);
+
+/* Device-side duration in milliseconds of the last render call's raster
+ * kernel and of the whole call, measured with HIP events on the context's
+ * stream (synchronises). */
+int cpv_last_timing(cpv_context* ctx, float* raster_ms, float* total_ms);
+
+/* Copies up to capacity tile traces of the last render call with
+ * record_costs set; *count receives the number available. */
+int cpv_get_tile_traces(cpv_context* ctx, cpv_tile_trace* traces, uint32_t capacity, uint32_t* count);
+
+/* ---- shader programs ---- */
+int cpv_program_create_spirv(cpv_context* ctx,
+                             const uint32_t* vs_words, size_t vs_word_count, const char* vs_entry,
+                             const uint32_t* fs_words, size_t fs_word_count, const char* fs_entry,
+                             cpv_program** out);
+void cpv_program_destroy(cpv_context* ctx, cpv_program* program);
+/* Generated HIP source of a program (for inspection); valid until destroy. */
+const char* cpv_program_source(cpv_program* program);
+
+/* sizeof() of the descriptor structs, in the order: pipeline, draw, pass,
+ * attachment, texture, sampler, job, config, tile trace.  Lets bindings in
+ * other languages check their struct layouts.  Returns the count. */
+int cpv_abi_sizes(uint32_t* out, uint32_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

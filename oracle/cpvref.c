@@ -1,0 +1,1177 @@
+/*
+ * cpvref.c -- CPU oracle for the CPVulkan draw path (test infrastructure).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this; the product never does.  It restates, single-threaded and in the
+ * reference's loop order, what the reference executes for vkCmdDraw /
+ * vkCmdDrawIndexed replayed by vkQueueSubmit:
+ *
+ *   clears            CPVulkan/CommandBuffer.cpp:591-640,
+ *                     CPVulkan/CommandBuffer.Draw.cpp:117-149, 2398-2419
+ *   input assembly    CommandBuffer.Draw.cpp:567-760
+ *   vertex stage      CommandBuffer.Draw.cpp:776-814,
+ *                     LLVMRuntime/PipelineCompiler.cpp:513-582, 821-981
+ *   triangle raster   CommandBuffer.Draw.cpp:410-418, 816-954, 1300-1313, 1510-1697
+ *   fragment ops      LLVMRuntime/PipelineCompiler.cpp:1020-1799
+ *   blend             CommandBuffer.Draw.cpp:956-1262 (ApplyBlend; dead code in
+ *                     the reference, whose JIT aborts on blendEnable at
+ *                     PipelineCompiler.cpp:1674-1684 -- it is the only blend
+ *                     specification the reference has)
+ *   pixel formats     LLVMRuntime/ImageCompiler.cpp:14-100, 161-230, 303-382,
+ *                     541-606, 869-947, 1000-1240; CPVulkanBase/Formats.cpp:209-587
+ *   sampling          CPVulkan/ImageSampler.cpp:12-55, 105-145, 363-410, 461-673;
+ *                     CPVulkan/GlslFunctions.cpp:598-671
+ *   shader math       LLVMRuntime/SpirvFunctions.cpp:5-47 (glm mat4*vec4,
+ *                     dot), CPVulkan/GlslFunctions.cpp:122-126, 179-183
+ *
+ * Built with -O2 -ffp-contract=off and no fast-math on x86-64 (SSE scalar
+ * float), so every float operation below rounds exactly once, as the
+ * reference's host code and its un-flagged LLVM JIT do.
+ */
+#include "cpvref.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* Format information (CPVulkanBase/Formats.cpp:209-444)                */
+/* ------------------------------------------------------------------ */
+
+enum { BT_NONE, BT_UNORM, BT_SNORM, BT_USCALED, BT_SSCALED, BT_UINT, BT_SINT, BT_SFLOAT };
+enum { FT_NORMAL, FT_DEPTH_STENCIL };
+
+typedef struct {
+    int type;
+    int base;
+    uint32_t total;     /* bytes per pixel */
+    uint32_t element;   /* bytes per channel */
+    int off[4];         /* byte offset of R,G,B,A or -1 */
+    int depth_off, stencil_off;
+} fmt_t;
+
+static int fmt_get(uint32_t f, fmt_t* o)
+{
+    static const int RGBA[4] = {0, 1, 2, 3}, BGRA[4] = {2, 1, 0, 3};
+    memset(o, 0, sizeof(*o));
+    o->off[0] = o->off[1] = o->off[2] = o->off[3] = -1;
+    o->depth_off = o->stencil_off = -1;
+    o->type = FT_NORMAL;
+    int ch = 0, el = 1, base = BT_NONE;
+    const int* order = RGBA;
+    switch (f) {
+    case CPV_FORMAT_R8_UNORM: ch = 1; base = BT_UNORM; break;
+    case CPV_FORMAT_R8_SNORM: ch = 1; base = BT_SNORM; break;
+    case CPV_FORMAT_R8_USCALED: ch = 1; base = BT_USCALED; break;
+    case CPV_FORMAT_R8_SSCALED: ch = 1; base = BT_SSCALED; break;
+    case CPV_FORMAT_R8_UINT: ch = 1; base = BT_UINT; break;
+    case CPV_FORMAT_R8_SINT: ch = 1; base = BT_SINT; break;
+    case CPV_FORMAT_R8G8_UNORM: ch = 2; base = BT_UNORM; break;
+    case CPV_FORMAT_R8G8_SNORM: ch = 2; base = BT_SNORM; break;
+    case CPV_FORMAT_R8G8_UINT: ch = 2; base = BT_UINT; break;
+    case CPV_FORMAT_R8G8_SINT: ch = 2; base = BT_SINT; break;
+    case CPV_FORMAT_R8G8B8_UNORM: ch = 3; base = BT_UNORM; break;
+    case CPV_FORMAT_R8G8B8_SNORM: ch = 3; base = BT_SNORM; break;
+    case CPV_FORMAT_R8G8B8_UINT: ch = 3; base = BT_UINT; break;
+    case CPV_FORMAT_R8G8B8_SINT: ch = 3; base = BT_SINT; break;
+    case CPV_FORMAT_B8G8R8_UNORM: ch = 3; base = BT_UNORM; order = BGRA; break;
+    case CPV_FORMAT_B8G8R8_SNORM: ch = 3; base = BT_SNORM; order = BGRA; break;
+    case CPV_FORMAT_B8G8R8_UINT: ch = 3; base = BT_UINT; order = BGRA; break;
+    case CPV_FORMAT_B8G8R8_SINT: ch = 3; base = BT_SINT; order = BGRA; break;
+    case CPV_FORMAT_R8G8B8A8_UNORM: ch = 4; base = BT_UNORM; break;
+    case CPV_FORMAT_R8G8B8A8_SNORM: ch = 4; base = BT_SNORM; break;
+    case CPV_FORMAT_R8G8B8A8_USCALED: ch = 4; base = BT_USCALED; break;
+    case CPV_FORMAT_R8G8B8A8_SSCALED: ch = 4; base = BT_SSCALED; break;
+    case CPV_FORMAT_R8G8B8A8_UINT: ch = 4; base = BT_UINT; break;
+    case CPV_FORMAT_R8G8B8A8_SINT: ch = 4; base = BT_SINT; break;
+    case CPV_FORMAT_B8G8R8A8_UNORM: ch = 4; base = BT_UNORM; order = BGRA; break;
+    case CPV_FORMAT_B8G8R8A8_SNORM: ch = 4; base = BT_SNORM; order = BGRA; break;
+    case CPV_FORMAT_B8G8R8A8_UINT: ch = 4; base = BT_UINT; order = BGRA; break;
+    case CPV_FORMAT_B8G8R8A8_SINT: ch = 4; base = BT_SINT; order = BGRA; break;
+    case CPV_FORMAT_R16_UNORM: ch = 1; el = 2; base = BT_UNORM; break;
+    case CPV_FORMAT_R16_SNORM: ch = 1; el = 2; base = BT_SNORM; break;
+    case CPV_FORMAT_R16_UINT: ch = 1; el = 2; base = BT_UINT; break;
+    case CPV_FORMAT_R16_SINT: ch = 1; el = 2; base = BT_SINT; break;
+    case CPV_FORMAT_R16G16_UNORM: ch = 2; el = 2; base = BT_UNORM; break;
+    case CPV_FORMAT_R16G16_UINT: ch = 2; el = 2; base = BT_UINT; break;
+    case CPV_FORMAT_R16G16_SINT: ch = 2; el = 2; base = BT_SINT; break;
+    case CPV_FORMAT_R16G16B16A16_UNORM: ch = 4; el = 2; base = BT_UNORM; break;
+    case CPV_FORMAT_R16G16B16A16_SNORM: ch = 4; el = 2; base = BT_SNORM; break;
+    case CPV_FORMAT_R16G16B16A16_UINT: ch = 4; el = 2; base = BT_UINT; break;
+    case CPV_FORMAT_R16G16B16A16_SINT: ch = 4; el = 2; base = BT_SINT; break;
+    case CPV_FORMAT_R32_UINT: ch = 1; el = 4; base = BT_UINT; break;
+    case CPV_FORMAT_R32_SINT: ch = 1; el = 4; base = BT_SINT; break;
+    case CPV_FORMAT_R32_SFLOAT: ch = 1; el = 4; base = BT_SFLOAT; break;
+    case CPV_FORMAT_R32G32_UINT: ch = 2; el = 4; base = BT_UINT; break;
+    case CPV_FORMAT_R32G32_SINT: ch = 2; el = 4; base = BT_SINT; break;
+    case CPV_FORMAT_R32G32_SFLOAT: ch = 2; el = 4; base = BT_SFLOAT; break;
+    case CPV_FORMAT_R32G32B32_UINT: ch = 3; el = 4; base = BT_UINT; break;
+    case CPV_FORMAT_R32G32B32_SINT: ch = 3; el = 4; base = BT_SINT; break;
+    case CPV_FORMAT_R32G32B32_SFLOAT: ch = 3; el = 4; base = BT_SFLOAT; break;
+    case CPV_FORMAT_R32G32B32A32_UINT: ch = 4; el = 4; base = BT_UINT; break;
+    case CPV_FORMAT_R32G32B32A32_SINT: ch = 4; el = 4; base = BT_SINT; break;
+    case CPV_FORMAT_R32G32B32A32_SFLOAT: ch = 4; el = 4; base = BT_SFLOAT; break;
+    /* MakeDepthFormatInformation(format, total, element, base, depthOffset, stencilOffset) */
+    case CPV_FORMAT_D16_UNORM: o->type = FT_DEPTH_STENCIL; o->total = 2; o->element = 2; o->base = BT_UNORM; o->depth_off = 0; return 1;
+    case CPV_FORMAT_X8_D24_UNORM_PACK32: o->type = FT_DEPTH_STENCIL; o->total = 4; o->element = 4; o->base = BT_UNORM; o->depth_off = 0; return 1;
+    case CPV_FORMAT_D32_SFLOAT: o->type = FT_DEPTH_STENCIL; o->total = 4; o->element = 4; o->base = BT_SFLOAT; o->depth_off = 0; return 1;
+    case CPV_FORMAT_S8_UINT: o->type = FT_DEPTH_STENCIL; o->total = 1; o->element = 1; o->base = BT_UINT; o->stencil_off = 0; return 1;
+    case CPV_FORMAT_D16_UNORM_S8_UINT: o->type = FT_DEPTH_STENCIL; o->total = 3; o->element = 2; o->base = BT_UNORM; o->depth_off = 0; o->stencil_off = 2; return 1;
+    case CPV_FORMAT_D24_UNORM_S8_UINT: o->type = FT_DEPTH_STENCIL; o->total = 4; o->element = 3; o->base = BT_UNORM; o->depth_off = 0; o->stencil_off = 3; return 1;
+    case CPV_FORMAT_D32_SFLOAT_S8_UINT: o->type = FT_DEPTH_STENCIL; o->total = 8; o->element = 4; o->base = BT_SFLOAT; o->depth_off = 0; o->stencil_off = 4; return 1;
+    default: return 0;
+    }
+    o->base = base;
+    o->element = (uint32_t)el;
+    o->total = (uint32_t)(el * ch);
+    for (int i = 0; i < ch; i++) o->off[i] = order[i] * el;
+    return 1;
+}
+
+uint32_t cpvref_format_size(uint32_t format)
+{
+    fmt_t f;
+    return fmt_get(format, &f) ? f.total : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Conversions (LLVMRuntime/ImageCompiler.cpp:14-100)                   */
+/* ------------------------------------------------------------------ */
+
+/* llvm.maxnum / llvm.minnum: a NaN operand yields the other operand. */
+static float maxnum(float a, float b) { return fmaxf(a, b); }
+static float minnum(float a, float b) { return fminf(a, b); }
+
+/* EmitConvertFloatUInt: fmul by (float)max, llvm.round (half away from
+ * zero), fptoui. */
+static uint32_t f2unorm(float v, uint32_t maxv) { return (uint32_t)roundf(v * (float)maxv); }
+/* EmitConvertFloatInt: fmul, llvm.round, fptosi. */
+static int32_t f2snorm(float v, int32_t maxv) { return (int32_t)roundf(v * (float)maxv); }
+
+static uint32_t load_uint(const uint8_t* p, uint32_t el)
+{
+    if (el == 1) return p[0];
+    if (el == 2) { uint16_t v; memcpy(&v, p, 2); return v; }
+    uint32_t v; memcpy(&v, p, 4); return v;
+}
+static int32_t load_sint(const uint8_t* p, uint32_t el)
+{
+    if (el == 1) return (int8_t)p[0];
+    if (el == 2) { int16_t v; memcpy(&v, p, 2); return v; }
+    int32_t v; memcpy(&v, p, 4); return v;
+}
+static void store_uint(uint8_t* p, uint32_t el, uint32_t v)
+{
+    if (el == 1) p[0] = (uint8_t)v;
+    else if (el == 2) { uint16_t t = (uint16_t)v; memcpy(p, &t, 2); }
+    else memcpy(p, &v, 4);
+}
+
+/* GetPixelF32 of a normal format: EmitGetNormalPixel / EmitGetNormalChannel<float>
+ * (ImageCompiler.cpp:161-230, 303-347).  Missing channels are (0,0,0,1).
+ * Unsigned integer channels of any kind are converted with an fdiv by the
+ * type maximum (EmitConvertUIntFloat), signed ones likewise (EmitConvertIntFloat). */
+int cpvref_unpack_f32(uint32_t format, const uint8_t* src, float rgba[4])
+{
+    fmt_t f;
+    if (!fmt_get(format, &f) || f.type != FT_NORMAL) return CPV_ERROR_UNSUPPORTED;
+    rgba[0] = 0.0f; rgba[1] = 0.0f; rgba[2] = 0.0f; rgba[3] = 1.0f;
+    for (int i = 0; i < 4; i++) {
+        if (f.off[i] < 0) continue;
+        const uint8_t* p = src + f.off[i];
+        switch (f.base) {
+        case BT_UNORM: case BT_USCALED: case BT_UINT: {
+            uint32_t maxv = f.element == 1 ? 255u : f.element == 2 ? 65535u : 4294967295u;
+            rgba[i] = (float)load_uint(p, f.element) / (float)maxv;
+            break;
+        }
+        case BT_SNORM: case BT_SSCALED: case BT_SINT: {
+            int32_t maxv = f.element == 1 ? 127 : f.element == 2 ? 32767 : 2147483647;
+            rgba[i] = (float)load_sint(p, f.element) / (float)maxv;
+            break;
+        }
+        case BT_SFLOAT: memcpy(&rgba[i], p, 4); break;
+        default: return CPV_ERROR_UNSUPPORTED;
+        }
+    }
+    return CPV_OK;
+}
+
+/* SetPixelF32 of a normal format: CompileSetNormal/CompileSetNormalChannel
+ * (ImageCompiler.cpp:1000-1037, 1077-1114, 1179-1197). */
+int cpvref_pack_f32(uint32_t format, const float rgba[4], uint8_t* dst)
+{
+    fmt_t f;
+    if (!fmt_get(format, &f) || f.type != FT_NORMAL) return CPV_ERROR_UNSUPPORTED;
+    for (int i = 0; i < 4; i++) {
+        if (f.off[i] < 0) continue;
+        uint8_t* p = dst + f.off[i];
+        float v = rgba[i];
+        switch (f.base) {
+        case BT_UNORM: {
+            v = minnum(maxnum(v, 0.0f), 1.0f);
+            store_uint(p, f.element, f2unorm(v, f.element == 1 ? 255u : 65535u));
+            break;
+        }
+        case BT_SNORM: {
+            v = minnum(maxnum(v, -1.0f), 1.0f);
+            store_uint(p, f.element, (uint32_t)f2snorm(v, f.element == 1 ? 127 : 32767));
+            break;
+        }
+        case BT_SFLOAT: memcpy(p, &v, 4); break;
+        default: return CPV_ERROR_UNSUPPORTED;   /* USCALED/SSCALED: TODO_ERROR in the reference */
+        }
+    }
+    return CPV_OK;
+}
+
+/* SetPixelU32 (ImageCompiler.cpp:1121-1143): 8/16-bit channels saturate
+ * with an unsigned compare-select, 32-bit store as is. */
+int cpvref_pack_u32(uint32_t format, const uint32_t rgba[4], uint8_t* dst)
+{
+    fmt_t f;
+    if (!fmt_get(format, &f) || f.type != FT_NORMAL || f.base != BT_UINT) return CPV_ERROR_UNSUPPORTED;
+    for (int i = 0; i < 4; i++) {
+        if (f.off[i] < 0) continue;
+        uint32_t v = rgba[i];
+        if (f.element == 1) v = v < 255u ? v : 255u;
+        else if (f.element == 2) v = v < 65535u ? v : 65535u;
+        store_uint(dst + f.off[i], f.element, v);
+    }
+    return CPV_OK;
+}
+
+/* SetPixelI32 (ImageCompiler.cpp:1146-1175): signed clamp then truncate. */
+int cpvref_pack_i32(uint32_t format, const int32_t rgba[4], uint8_t* dst)
+{
+    fmt_t f;
+    if (!fmt_get(format, &f) || f.type != FT_NORMAL || f.base != BT_SINT) return CPV_ERROR_UNSUPPORTED;
+    for (int i = 0; i < 4; i++) {
+        if (f.off[i] < 0) continue;
+        int32_t v = rgba[i];
+        if (f.element == 1) { v = v > -128 ? v : -128; v = v < 127 ? v : 127; }
+        else if (f.element == 2) { v = v > -32768 ? v : -32768; v = v < 32767 ? v : 32767; }
+        store_uint(dst + f.off[i], f.element, (uint32_t)v);
+    }
+    return CPV_OK;
+}
+
+/* SetDSPixel (ImageCompiler.cpp:869-947). */
+int cpvref_pack_depth_stencil(uint32_t format, float depth, uint32_t stencil, uint8_t* dst)
+{
+    fmt_t f;
+    if (!fmt_get(format, &f) || f.type != FT_DEPTH_STENCIL) return CPV_ERROR_UNSUPPORTED;
+    if (f.depth_off >= 0) {
+        float v = depth;
+        switch (format) {
+        case CPV_FORMAT_D16_UNORM: case CPV_FORMAT_D16_UNORM_S8_UINT: {
+            v = minnum(maxnum(v, 0.0f), 1.0f);
+            uint16_t u = (uint16_t)f2unorm(v, 65535u);
+            memcpy(dst, &u, 2);
+            break;
+        }
+        case CPV_FORMAT_D24_UNORM_S8_UINT: case CPV_FORMAT_X8_D24_UNORM_PACK32: {
+            v = minnum(maxnum(v, 0.0f), 1.0f);
+            v = v * (float)0x00FFFFFF;
+            uint32_t u = (uint32_t)roundf(v);
+            if (format == CPV_FORMAT_D24_UNORM_S8_UINT) u |= (uint32_t)(uint8_t)stencil << 24;
+            memcpy(dst, &u, 4);
+            break;
+        }
+        default: memcpy(dst, &v, 4); break;      /* D32: raw store, no clamp */
+        }
+    }
+    if (f.stencil_off >= 0 && format != CPV_FORMAT_D24_UNORM_S8_UINT) dst[f.stencil_off] = (uint8_t)stencil;
+    return CPV_OK;
+}
+
+/* GetPixelDepth (ImageCompiler.cpp:541-606). */
+float cpvref_unpack_depth(uint32_t format, const uint8_t* src)
+{
+    switch (format) {
+    case CPV_FORMAT_D16_UNORM: case CPV_FORMAT_D16_UNORM_S8_UINT: {
+        uint16_t u; memcpy(&u, src, 2);
+        return (float)u / 65535.0f;
+    }
+    case CPV_FORMAT_D24_UNORM_S8_UINT: case CPV_FORMAT_X8_D24_UNORM_PACK32: {
+        uint32_t u; memcpy(&u, src, 4);
+        return (float)(u & 0x00FFFFFFu) / (float)0x00FFFFFF;
+    }
+    case CPV_FORMAT_D32_SFLOAT: case CPV_FORMAT_D32_SFLOAT_S8_UINT: {
+        float v; memcpy(&v, src, 4);
+        return v;
+    }
+    default: return 0.0f;
+    }
+}
+
+/* GetPixelStencil (ImageCompiler.cpp:608-645). */
+uint32_t cpvref_unpack_stencil(uint32_t format, const uint8_t* src)
+{
+    fmt_t f;
+    if (!fmt_get(format, &f) || f.stencil_off < 0) return 0;
+    return src[f.stencil_off];
+}
+
+/* GetNormalImageSize (Formats.cpp:455-482): contiguous mip chain, stride
+ * = pixel size * width, halving with a floor of 1. */
+uint64_t cpvref_image_layout(uint32_t format, uint32_t width, uint32_t height, uint32_t levels,
+                             uint64_t* level_offsets)
+{
+    uint64_t px = cpvref_format_size(format), layer = 0;
+    for (uint32_t i = 0; i < levels; i++) {
+        if (level_offsets) level_offsets[i] = layer;
+        layer += px * width * height;
+        width = width / 2 > 1 ? width / 2 : 1;
+        height = height / 2 > 1 ? height / 2 : 1;
+    }
+    return layer;
+}
+
+/* ------------------------------------------------------------------ */
+/* Sampling (CPVulkan/ImageSampler.cpp)                                */
+/* ------------------------------------------------------------------ */
+
+/* wrap (ImageSampler.cpp:12-38) */
+static int32_t wrap_coord(int32_t v, int32_t size, uint32_t mode)
+{
+    switch (mode) {
+    case CPV_ADDRESS_REPEAT: return (v % size + size) % size;
+    case CPV_ADDRESS_MIRRORED_REPEAT: {
+        int32_t two = 2 * size;
+        int32_t n = (v % two + two) % two - size;
+        return size - 1 - (n >= 0 ? n : -(1 + n));
+    }
+    case CPV_ADDRESS_CLAMP_TO_EDGE: return v < 0 ? 0 : v > size - 1 ? size - 1 : v;
+    case CPV_ADDRESS_CLAMP_TO_BORDER: return v < -1 ? -1 : v > size ? size : v;
+    case CPV_ADDRESS_MIRROR_CLAMP_TO_EDGE: {
+        int32_t m = v >= 0 ? v : -(1 + v);
+        return m < 0 ? 0 : m > size - 1 ? size - 1 : m;
+    }
+    default: return 0;
+    }
+}
+
+/* lerp (ImageSampler.cpp:51-55): dvec4(a) + dvec4(b - a) * delta, the
+ * difference taken in float, the rest in double, rounded back to float. */
+static void lerp4(const float a[4], const float b[4], double t, float out[4])
+{
+    for (int i = 0; i < 4; i++) {
+        float d = b[i] - a[i];
+        out[i] = (float)((double)a[i] + (double)d * t);
+    }
+}
+
+typedef struct { const uint8_t* data; uint32_t w, h; } level_t;
+
+/* GetPixel<fvec4, 2D> (ImageSampler.cpp:105-145). */
+static void texel(uint32_t format, const level_t* lv, int32_t x, int32_t y, const float border[4], float out[4])
+{
+    if (x < 0 || (uint32_t)x >= lv->w || y < 0 || (uint32_t)y >= lv->h) {
+        memcpy(out, border, 16);
+        return;
+    }
+    uint32_t px = cpvref_format_size(format);
+    cpvref_unpack_f32(format, lv->data + (uint64_t)y * lv->w * px + (uint64_t)x * px, out);
+}
+
+static const float k_border[6][4] = {
+    {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 1}, {0, 0, 0, 1}, {1, 1, 1, 1}, {1, 1, 1, 1}};
+
+/* SampleImageOfLevel, 2D, weighted average (ImageSampler.cpp:461-579). */
+static void sample_level(uint32_t format, const level_t* lv, float u, float v, uint32_t filter,
+                         const cpv_sampler_desc* s, float out[4])
+{
+    const float* border = k_border[s->border_color <= 5 ? s->border_color : 0];
+    const uint32_t modes[2] = {s->address_u, s->address_v};
+    const float coords[2] = {u, v};
+    const uint32_t range[2] = {lv->w, lv->h};
+    if (filter == CPV_FILTER_NEAREST) {
+        int32_t c[2];
+        for (int i = 0; i < 2; i++) {
+            c[i] = (int32_t)floorf(coords[i] * (float)range[i] + 0.0f);
+            c[i] = wrap_coord(c[i], (int32_t)range[i], modes[i]);
+        }
+        texel(format, lv, c[0], c[1], border, out);
+        return;
+    }
+    int32_t c0[2], c1[2];
+    float t[2];
+    for (int i = 0; i < 2; i++) {
+        c0[i] = (int32_t)floorf(coords[i] * (float)range[i] - 0.5f);
+        c1[i] = wrap_coord(c0[i] + 1, (int32_t)range[i], modes[i]);
+        c0[i] = wrap_coord(c0[i], (int32_t)range[i], modes[i]);
+    }
+    for (int i = 0; i < 2; i++) {
+        float q = coords[i] * (float)range[i] - 0.5f;
+        t[i] = q - floorf(q);
+    }
+    /* GetPixelLinear, length 2 (ImageSampler.cpp:363-410) */
+    float p00[4], p01[4], p10[4], p11[4], a[4], b[4];
+    texel(format, lv, c0[0], c0[1], border, p00);
+    texel(format, lv, c0[0], c1[1], border, p01);
+    texel(format, lv, c1[0], c0[1], border, p10);
+    texel(format, lv, c1[0], c1[1], border, p11);
+    lerp4(p00, p10, (double)t[0], a);
+    lerp4(p01, p11, (double)t[0], b);
+    lerp4(a, b, (double)t[1], out);
+}
+
+static void get_level(const cpv_texture_desc* t, uint32_t view_level, level_t* lv)
+{
+    uint64_t offs[CPV_MAX_MIP_LEVELS];
+    uint32_t levels = t->levels > CPV_MAX_MIP_LEVELS ? CPV_MAX_MIP_LEVELS : t->levels;
+    cpvref_image_layout(t->format, t->width, t->height, levels, offs);
+    uint32_t level = t->base_level + view_level;
+    uint32_t w = t->width, h = t->height;
+    for (uint32_t i = 0; i < level; i++) { w = w / 2 > 1 ? w / 2 : 1; h = h / 2 > 1 ? h / 2 : 1; }
+    lv->data = (const uint8_t*)(uintptr_t)t->address + offs[level];
+    lv->w = w;
+    lv->h = h;
+}
+
+/* ImageSampleImplicitLod -> ImageSampleExplicitLod with lambda base 0
+ * (GlslFunctions.cpp:598-671) -> SampleImage (ImageSampler.cpp:581-673). */
+int cpvref_sample_implicit(const cpv_texture_desc* t, float u, float v, float out[4])
+{
+    const cpv_sampler_desc* s = &t->sampler;
+    fmt_t f;
+    if (!fmt_get(t->format, &f) || f.type != FT_NORMAL) return CPV_ERROR_UNSUPPORTED;
+    float bias = s->mip_lod_bias + 0.0f;
+    bias = bias < -32.0f ? -32.0f : bias > 32.0f ? 32.0f : bias;   /* MAX_SAMPLER_LOD_BIAS */
+    float lambda = 0.0f + bias;
+    lambda = lambda < s->min_lod ? s->min_lod : lambda > s->max_lod ? s->max_lod : lambda;
+    uint32_t levels = t->level_count;
+    level_t lv;
+    if (lambda <= 0) {
+        get_level(t, 0, &lv);
+        sample_level(t->format, &lv, u, v, s->mag_filter, s, out);
+    } else {
+        float maxl = (float)(levels - 1);
+        float ml = lambda < 0.0f ? 0.0f : lambda > maxl ? maxl : lambda;
+        if (s->mipmap_mode == CPV_MIPMAP_NEAREST) {
+            uint32_t real = (uint32_t)ceilf(ml + 0.5f) - 1;
+            get_level(t, real, &lv);
+            sample_level(t->format, &lv, u, v, s->min_filter, s, out);
+        } else {
+            uint32_t l1 = (uint32_t)floorf(ml);
+            float delta = ml - (float)l1;
+            float p1[4];
+            get_level(t, l1, &lv);
+            sample_level(t->format, &lv, u, v, s->min_filter, s, p1);
+            if (delta == 0) {
+                memcpy(out, p1, 16);
+            } else {
+                float p2[4];
+                get_level(t, l1 + 1, &lv);
+                sample_level(t->format, &lv, u, v, s->min_filter, s, p2);
+                lerp4(p1, p2, (double)delta, out);
+            }
+        }
+    }
+    if (f.off[0] < 0) out[0] = 0;
+    if (f.off[1] < 0) out[1] = 0;
+    if (f.off[2] < 0) out[2] = 0;
+    if (f.off[3] < 0) out[3] = 1;
+    return CPV_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Blend (CommandBuffer.Draw.cpp:956-1262)                             */
+/* ------------------------------------------------------------------ */
+
+static void blend_factor(const float s[4], const float d[4], const float c[4], uint32_t colour_f,
+                         uint32_t alpha_f, float out[4])
+{
+    int i;
+    for (i = 0; i < 4; i++) out[i] = 0.0f;
+    switch (colour_f) {
+    case CPV_BLEND_ZERO: break;
+    case CPV_BLEND_ONE: for (i = 0; i < 4; i++) out[i] = 1.0f; break;
+    case CPV_BLEND_SRC_COLOR: for (i = 0; i < 4; i++) out[i] = s[i]; break;
+    case CPV_BLEND_ONE_MINUS_SRC_COLOR: for (i = 0; i < 4; i++) out[i] = 1.0f - s[i]; break;
+    case CPV_BLEND_DST_COLOR: for (i = 0; i < 4; i++) out[i] = d[i]; break;
+    case CPV_BLEND_ONE_MINUS_DST_COLOR: for (i = 0; i < 4; i++) out[i] = 1.0f - d[i]; break;
+    case CPV_BLEND_SRC_ALPHA: for (i = 0; i < 4; i++) out[i] = s[3]; break;
+    case CPV_BLEND_ONE_MINUS_SRC_ALPHA: for (i = 0; i < 4; i++) out[i] = 1.0f - s[3]; break;
+    case CPV_BLEND_DST_ALPHA: for (i = 0; i < 4; i++) out[i] = d[3]; break;
+    case CPV_BLEND_ONE_MINUS_DST_ALPHA: for (i = 0; i < 4; i++) out[i] = 1.0f - d[3]; break;
+    case CPV_BLEND_CONSTANT_COLOR: for (i = 0; i < 4; i++) out[i] = c[i]; break;
+    case CPV_BLEND_ONE_MINUS_CONSTANT_COLOR: for (i = 0; i < 4; i++) out[i] = 1.0f - c[i]; break;
+    case CPV_BLEND_CONSTANT_ALPHA: for (i = 0; i < 4; i++) out[i] = c[3]; break;
+    case CPV_BLEND_ONE_MINUS_CONSTANT_ALPHA: for (i = 0; i < 4; i++) out[i] = 1.0f - c[3]; break;
+    case CPV_BLEND_SRC_ALPHA_SATURATE: {
+        /* std::min(source.a, 1 - destination.a) */
+        float a = s[3], b = 1.0f - d[3];
+        float m = b < a ? b : a;
+        out[0] = m; out[1] = m; out[2] = m; out[3] = 1.0f;
+        break;
+    }
+    default: break;
+    }
+    if (colour_f != alpha_f) {
+        switch (alpha_f) {
+        case CPV_BLEND_ZERO: out[3] = 0.0f; break;
+        case CPV_BLEND_ONE: case CPV_BLEND_SRC_ALPHA_SATURATE: out[3] = 1.0f; break;
+        case CPV_BLEND_SRC_COLOR: case CPV_BLEND_SRC_ALPHA: out[3] = s[3]; break;
+        case CPV_BLEND_ONE_MINUS_SRC_COLOR: case CPV_BLEND_ONE_MINUS_SRC_ALPHA: out[3] = 1.0f - s[3]; break;
+        case CPV_BLEND_DST_COLOR: case CPV_BLEND_DST_ALPHA: out[3] = d[3]; break;
+        case CPV_BLEND_ONE_MINUS_DST_COLOR: case CPV_BLEND_ONE_MINUS_DST_ALPHA: out[3] = 1.0f - d[3]; break;
+        case CPV_BLEND_CONSTANT_COLOR: case CPV_BLEND_CONSTANT_ALPHA: out[3] = c[3]; break;
+        case CPV_BLEND_ONE_MINUS_CONSTANT_COLOR: case CPV_BLEND_ONE_MINUS_CONSTANT_ALPHA: out[3] = 1.0f - c[3]; break;
+        default: break;
+        }
+    }
+}
+
+/* glm::min / glm::max per component: min(x,y) = y < x ? y : x,
+ * max(x,y) = x < y ? y : x. */
+static float gmin(float x, float y) { return y < x ? y : x; }
+static float gmax(float x, float y) { return x < y ? y : x; }
+
+void cpvref_apply_blend(const float s[4], const float d[4], const float c[4],
+                        const cpv_blend_attachment* st, float out[4])
+{
+    float sf[4], df[4];
+    blend_factor(s, d, c, st->src_color_factor, st->src_alpha_factor, sf);
+    blend_factor(s, d, c, st->dst_color_factor, st->dst_alpha_factor, df);
+    for (int i = 0; i < 4; i++) {
+        switch (st->color_op) {
+        case CPV_BLEND_OP_ADD: out[i] = s[i] * sf[i] + d[i] * df[i]; break;
+        case CPV_BLEND_OP_SUBTRACT: out[i] = s[i] * sf[i] - d[i] * df[i]; break;
+        case CPV_BLEND_OP_REVERSE_SUBTRACT: out[i] = d[i] * df[i] - s[i] * sf[i]; break;
+        case CPV_BLEND_OP_MIN: out[i] = gmin(s[i], d[i]); break;
+        case CPV_BLEND_OP_MAX: out[i] = gmax(s[i], d[i]); break;
+        default: out[i] = 0.0f; break;
+        }
+    }
+    if (st->color_op != st->alpha_op) {
+        switch (st->alpha_op) {
+        case CPV_BLEND_OP_ADD: out[3] = s[3] * sf[3] + d[3] * df[3]; break;
+        case CPV_BLEND_OP_SUBTRACT: out[3] = s[3] * sf[3] - d[3] * df[3]; break;
+        case CPV_BLEND_OP_REVERSE_SUBTRACT: out[3] = d[3] * df[3] - s[3] * sf[3]; break;
+        /* std::min / std::max on the alpha channel */
+        case CPV_BLEND_OP_MIN: out[3] = d[3] < s[3] ? d[3] : s[3]; break;
+        case CPV_BLEND_OP_MAX: out[3] = s[3] < d[3] ? d[3] : s[3]; break;
+        default: break;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Shaders of the benchmark workloads                                  */
+/* ------------------------------------------------------------------ */
+
+/* glm mat4 * vec4 (Samples/utils/glm/detail/type_mat4x4.inl:651-697):
+ * (m0*v0 + m1*v1) + (m2*v2 + m3*v3), column major. */
+static void mat4_mul_vec4(const float m[16], const float v[4], float out[4])
+{
+    for (int r = 0; r < 4; r++) {
+        float mul0 = m[0 * 4 + r] * v[0];
+        float mul1 = m[1 * 4 + r] * v[1];
+        float add0 = mul0 + mul1;
+        float mul2 = m[2 * 4 + r] * v[2];
+        float mul3 = m[3 * 4 + r] * v[3];
+        float add1 = mul2 + mul3;
+        out[r] = add0 + add1;
+    }
+}
+
+/* glm::dot for vec3: (x*x' + y*y') + z*z'. */
+static float dot3(const float a[3], const float b[3])
+{
+    float x = a[0] * b[0], y = a[1] * b[1], z = a[2] * b[2];
+    return (x + y) + z;
+}
+
+typedef struct {
+    int nout;               /* varying floats written by the VS, in declaration order */
+    int nin_attr;           /* number of vertex attributes read */
+    int loc[3];             /* attribute locations */
+    int comps[3];           /* float components per attribute */
+} vs_info_t;
+
+static int vs_info(uint32_t kind, vs_info_t* o)
+{
+    memset(o, 0, sizeof(*o));
+    switch (kind) {
+    case CPV_VS_PASSTHROUGH:
+    case CPV_VS_MVP_COLOR:
+        o->nout = 4; o->nin_attr = 2;
+        o->loc[0] = 0; o->comps[0] = 4; o->loc[1] = 1; o->comps[1] = 4;
+        return 1;
+    case CPV_VS_MVP_LAMBERT:
+        o->nout = 5; o->nin_attr = 3;
+        o->loc[0] = 0; o->comps[0] = 3; o->loc[1] = 1; o->comps[1] = 3; o->loc[2] = 2; o->comps[2] = 2;
+        return 1;
+    default: return 0;
+    }
+}
+
+static int fs_inputs(uint32_t kind)
+{
+    switch (kind) {
+    case CPV_FS_COLOR: return 4;
+    case CPV_FS_TEX_LAMBERT: return 5;
+    default: return -1;
+    }
+}
+
+static uint32_t float_format_for(int comps)
+{
+    switch (comps) {
+    case 1: return CPV_FORMAT_R32_SFLOAT;
+    case 2: return CPV_FORMAT_R32G32_SFLOAT;
+    case 3: return CPV_FORMAT_R32G32B32_SFLOAT;
+    default: return CPV_FORMAT_R32G32B32A32_SFLOAT;
+    }
+}
+
+/* EmitCopyInput (PipelineCompiler.cpp:821-896) for float shader inputs:
+ * identical formats are a straight load, normalised/scaled formats go
+ * through GetPixel and the first components are kept. */
+static int fetch_attribute(const cpv_pipeline_desc* p, const cpv_draw_desc* d, int location, int comps,
+                           uint32_t vertex_id, uint32_t instance_id, float* out)
+{
+    const cpv_vertex_attribute_desc* a = NULL;
+    const cpv_vertex_binding_desc* b = NULL;
+    for (uint32_t i = 0; i < p->attribute_count; i++)
+        if ((int)p->attributes[i].location == location) { a = &p->attributes[i]; break; }
+    if (!a) return CPV_ERROR_INVALID;
+    for (uint32_t i = 0; i < p->binding_count; i++)
+        if (p->bindings[i].binding == a->binding) { b = &p->bindings[i]; break; }
+    if (!b || b->binding >= CPV_MAX_VERTEX_BINDINGS) return CPV_ERROR_INVALID;
+    uint64_t index = b->input_rate == CPV_INPUT_RATE_VERTEX ? vertex_id : instance_id;
+    uint64_t off = (uint64_t)b->stride * index + a->offset;
+    const uint8_t* base = (const uint8_t*)(uintptr_t)d->vertex_address[b->binding];
+    if (off + 4u * (uint32_t)comps > d->vertex_bytes[b->binding] && a->format == float_format_for(comps))
+        return CPV_ERROR_INVALID;
+    if (a->format == float_format_for(comps)) {
+        memcpy(out, base + off, 4u * (uint32_t)comps);
+        return CPV_OK;
+    }
+    fmt_t f;
+    if (!fmt_get(a->format, &f) || f.type != FT_NORMAL) return CPV_ERROR_UNSUPPORTED;
+    if (f.base != BT_UNORM && f.base != BT_SNORM && f.base != BT_USCALED && f.base != BT_SSCALED)
+        return CPV_ERROR_UNSUPPORTED;
+    if (off + f.total > d->vertex_bytes[b->binding]) return CPV_ERROR_INVALID;
+    float v[4];
+    cpvref_unpack_f32(a->format, base + off, v);
+    memcpy(out, v, 4u * (uint32_t)comps);
+    return CPV_OK;
+}
+
+static const float* uniform_ptr(const cpv_draw_desc* d, int binding)
+{
+    return (const float*)(uintptr_t)d->uniform_address[binding];
+}
+
+/* Vertex shader of one vertex: writes clip position + varyings. */
+static int run_vs(const cpv_pipeline_desc* p, const cpv_draw_desc* d, uint32_t vertex_id,
+                  uint32_t instance_id, float pos[4], float* vary)
+{
+    vs_info_t info;
+    if (!vs_info(p->vs_kind, &info)) return CPV_ERROR_UNSUPPORTED;
+    float in[3][4];
+    for (int i = 0; i < info.nin_attr; i++) {
+        int rc = fetch_attribute(p, d, info.loc[i], info.comps[i], vertex_id, instance_id, in[i]);
+        if (rc) return rc;
+    }
+    switch (p->vs_kind) {
+    case CPV_VS_PASSTHROUGH:
+        memcpy(pos, in[0], 16);
+        memcpy(vary, in[1], 16);
+        break;
+    case CPV_VS_MVP_COLOR: {
+        const float* mvp = uniform_ptr(d, 0);
+        if (!mvp) return CPV_ERROR_INVALID;
+        mat4_mul_vec4(mvp, in[0], pos);
+        memcpy(vary, in[1], 16);
+        break;
+    }
+    case CPV_VS_MVP_LAMBERT: {
+        const float* mvp = uniform_ptr(d, 0);
+        if (!mvp) return CPV_ERROR_INVALID;
+        float p4[4] = {in[0][0], in[0][1], in[0][2], 1.0f};
+        mat4_mul_vec4(mvp, p4, pos);
+        memcpy(vary, in[1], 12);
+        memcpy(vary + 3, in[2], 8);
+        break;
+    }
+    default: return CPV_ERROR_UNSUPPORTED;
+    }
+    return CPV_OK;
+}
+
+/* Fragment shader: returns 1 when the fragment is discarded (OpKill). */
+static int run_fs(const cpv_pipeline_desc* p, const cpv_draw_desc* d, const float* in, float out[4])
+{
+    switch (p->fs_kind) {
+    case CPV_FS_COLOR:
+        memcpy(out, in, 16);
+        return 0;
+    case CPV_FS_TEX_LAMBERT: {
+        float t[4];
+        cpvref_sample_implicit(&d->textures[1], in[3], in[4], t);
+        /* glm::normalize(v) = v * inversesqrt(dot(v, v)), inversesqrt = 1 / sqrt */
+        float dd = dot3(in, in);
+        float inv = 1.0f / sqrtf(dd);
+        float n[3] = {in[0] * inv, in[1] * inv, in[2] * inv};
+        const float* L = uniform_ptr(d, 2);
+        float l = dot3(n, L);
+        /* FMax -> std::max(l, 0.0f) (GlslFunctions.cpp:122-126) */
+        float m = l < 0.0f ? 0.0f : l;
+        for (int i = 0; i < 4; i++) out[i] = t[i] * m;
+        return 0;
+    }
+    default: return 0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Raster (CommandBuffer.Draw.cpp)                                     */
+/* ------------------------------------------------------------------ */
+
+/* static_cast<int32_t>(float) as x86-64 cvttss2si executes it: truncation,
+ * INT32_MIN for NaN and out-of-range values. */
+static int32_t f2i_trunc(float v)
+{
+    if (v >= -2147483648.0f && v < 2147483648.0f) return (int32_t)v;
+    return INT32_MIN;
+}
+
+/* EdgeFunction (CommandBuffer.Draw.cpp:410-418) */
+static float edge(const float* a, const float* b, float cx, float cy)
+{
+    return (cx - a[0]) * (b[1] - a[1]) - (cy - a[1]) * (b[0] - a[0]);
+}
+
+typedef struct {
+    const cpv_pass_desc* pass;
+    const cpv_pipeline_desc* pipe;
+    const cpv_draw_desc* draw;
+    fmt_t cfmt, dsfmt;
+    int has_color, has_ds;
+    uint64_t fragments;
+} ctx_t;
+
+static uint8_t* color_px(ctx_t* c, int x, int y)
+{
+    return (uint8_t*)(uintptr_t)c->pass->color[0].address + ((uint64_t)y * c->pass->width + (uint64_t)x) * c->cfmt.total;
+}
+static uint8_t* ds_px(ctx_t* c, int x, int y)
+{
+    return (uint8_t*)(uintptr_t)c->pass->depth_stencil.address + ((uint64_t)y * c->pass->width + (uint64_t)x) * c->dsfmt.total;
+}
+
+static int fcompare(float a, float b, uint32_t op)
+{
+    switch (op) {
+    case CPV_COMPARE_NEVER: return 0;
+    case CPV_COMPARE_LESS: return a < b;
+    case CPV_COMPARE_EQUAL: return a == b;
+    case CPV_COMPARE_LESS_OR_EQUAL: return a <= b;
+    case CPV_COMPARE_GREATER: return a > b;
+    case CPV_COMPARE_NOT_EQUAL: return a < b || a > b;     /* FCmpONE: ordered and unequal */
+    case CPV_COMPARE_GREATER_OR_EQUAL: return a >= b;
+    default: return 1;
+    }
+}
+
+static int icompare(uint32_t a, uint32_t b, uint32_t op)
+{
+    switch (op) {
+    case CPV_COMPARE_NEVER: return 0;
+    case CPV_COMPARE_LESS: return a < b;
+    case CPV_COMPARE_EQUAL: return a == b;
+    case CPV_COMPARE_LESS_OR_EQUAL: return a <= b;
+    case CPV_COMPARE_GREATER: return a > b;
+    case CPV_COMPARE_NOT_EQUAL: return a != b;
+    case CPV_COMPARE_GREATER_OR_EQUAL: return a >= b;
+    default: return 1;
+    }
+}
+
+/* CompileGetStencilResult (PipelineCompiler.cpp:1382-1413); INC/DEC_CLAMP
+ * are llvm.sadd.sat / llvm.ssub.sat on i8. */
+static uint8_t stencil_result(uint32_t op, uint8_t cur, uint8_t ref)
+{
+    switch (op) {
+    case CPV_STENCIL_KEEP: return cur;
+    case CPV_STENCIL_ZERO: return 0;
+    case CPV_STENCIL_REPLACE: return ref;
+    case CPV_STENCIL_INC_CLAMP: { int v = (int8_t)cur + 1; if (v > 127) v = 127; return (uint8_t)(int8_t)v; }
+    case CPV_STENCIL_DEC_CLAMP: { int v = (int8_t)cur - 1; if (v < -128) v = -128; return (uint8_t)(int8_t)v; }
+    case CPV_STENCIL_INVERT: return (uint8_t)~cur;
+    case CPV_STENCIL_INC_WRAP: return (uint8_t)(cur + 1);
+    case CPV_STENCIL_DEC_WRAP: return (uint8_t)(cur - 1);
+    default: return cur;
+    }
+}
+
+/* The JIT'd fragment wrapper (PipelineCompiler.cpp:1020-1085) after the
+ * shader returned (not discarded). */
+static void fragment_ops(ctx_t* c, int x, int y, float depth, int front, const float colour[4])
+{
+    const cpv_pipeline_desc* p = c->pipe;
+    const int ds_attached = c->has_ds;
+    const int has_depth = ds_attached && c->dsfmt.depth_off >= 0;
+    const int has_stencil = ds_attached && c->dsfmt.stencil_off >= 0;
+    const int not_s8 = ds_attached && p->depth_stencil_format != CPV_FORMAT_S8_UINT;
+
+    /* CompileGetCurrentData (:1087-1126) */
+    float current_depth = 0.0f;
+    uint8_t current_stencil = 0;
+    if ((p->depth_bounds_test || p->depth_test) && has_depth)
+        current_depth = cpvref_unpack_depth(p->depth_stencil_format, ds_px(c, x, y));
+    if (p->stencil_test && has_stencil)
+        current_stencil = (uint8_t)cpvref_unpack_stencil(p->depth_stencil_format, ds_px(c, x, y));
+
+    /* CompileDepthBoundsTest (:1128-1166): FCmpULT / FCmpUGT */
+    if (p->depth_bounds_test && ds_attached) {
+        int less = !(current_depth >= p->min_depth_bounds);
+        int greater = !(current_depth <= p->max_depth_bounds);
+        if (less || greater) return;
+    }
+
+    int stencil_ok = 1, depth_ok = 1;
+    /* CompileDepthTest (:1225-1244) */
+#define DEPTH_TEST() do { \
+        if (p->depth_test && not_s8) depth_ok = fcompare(depth, current_depth, p->depth_compare); \
+        else depth_ok = 1; } while (0)
+
+    if (p->stencil_test && has_stencil) {
+        /* CompileStencilTest (:1168-1192): the compare uses the facing
+         * state; the write below always uses the front state's ops and
+         * write mask, with the facing state's reference for REPLACE. */
+        const cpv_stencil_state* st = front ? &p->front : &p->back;
+        uint8_t ref = (uint8_t)st->reference, mask = (uint8_t)st->compare_mask;
+        stencil_ok = icompare((uint32_t)(ref & mask), (uint32_t)(current_stencil & mask), st->compare_op);
+        DEPTH_TEST();
+        /* CompileStencilWrite (:1294-1380) */
+        const cpv_stencil_state* ws = &p->front;
+        uint8_t fail = stencil_result(ws->fail_op, current_stencil, ref);
+        uint8_t zfail = stencil_result(ws->depth_fail_op, current_stencil, ref);
+        uint8_t pass = stencil_result(ws->pass_op, current_stencil, ref);
+        uint8_t value = stencil_ok ? (depth_ok ? pass : zfail) : fail;
+        uint8_t wmask = (uint8_t)ws->write_mask;
+        value = (uint8_t)((value & wmask) | (current_stencil & (uint8_t)~wmask));
+        int attempt_depth = p->depth_test && p->depth_write && not_s8;
+        if (depth_ok && attempt_depth) {
+            cpvref_pack_depth_stencil(p->depth_stencil_format, depth, value, ds_px(c, x, y));
+        } else {
+            /* SetStencilPixelXXX re-reads the depth (GlslFunctions.cpp:898-912) */
+            float d = c->dsfmt.depth_off < 0 ? 0.0f : cpvref_unpack_depth(p->depth_stencil_format, ds_px(c, x, y));
+            cpvref_pack_depth_stencil(p->depth_stencil_format, d, value, ds_px(c, x, y));
+        }
+    } else {
+        stencil_ok = 1;
+        DEPTH_TEST();
+        /* CompileDepthStencilWrite, depth only (:1255-1291); SetDepthPixelXXX
+         * keeps the stored stencil (GlslFunctions.cpp:880-896). */
+        if (p->depth_test && p->depth_write && not_s8 && depth_ok) {
+            uint32_t s = c->dsfmt.stencil_off < 0 ? 0u : cpvref_unpack_stencil(p->depth_stencil_format, ds_px(c, x, y));
+            cpvref_pack_depth_stencil(p->depth_stencil_format, depth, s, ds_px(c, x, y));
+        }
+    }
+#undef DEPTH_TEST
+
+    /* CompileWriteFragment (:1415-1429) */
+    if (!(stencil_ok && depth_ok) || !c->has_color) return;
+    const cpv_blend_attachment* b = &p->blend[0];
+    uint8_t* px = color_px(c, x, y);
+    float value[4] = {colour[0], colour[1], colour[2], colour[3]};
+    if (c->cfmt.base == BT_UINT) {
+        uint32_t u[4];
+        memcpy(u, colour, 16);
+        cpvref_pack_u32(p->color_formats[0], u, px);
+        return;
+    }
+    if (c->cfmt.base == BT_SINT) {
+        int32_t s[4];
+        memcpy(s, colour, 16);
+        cpvref_pack_i32(p->color_formats[0], s, px);
+        return;
+    }
+    if (b->blend_enable) {
+        float dst[4], out[4];
+        cpvref_unpack_f32(p->color_formats[0], px, dst);
+        cpvref_apply_blend(value, dst, p->blend_constants, b, out);
+        memcpy(value, out, 16);
+    }
+    cpvref_pack_f32(p->color_formats[0], value, px);
+}
+
+/* SetDatum (CommandBuffer.Draw.cpp:816-837) for one float component. */
+static float set_datum_perspective(const float pw[3], const float v[3], const float w[3])
+{
+    float num = 0.0f, den = 0.0f;
+    for (int i = 0; i < 3; i++) {
+        num += w[i] * v[i] / pw[i];
+        den += w[i] / pw[i];
+    }
+    return num / den;
+}
+
+/* ProcessTriangles (CommandBuffer.Draw.cpp:1510-1594) over primitives
+ * [pb, pe) of the assembled list. */
+static void process_triangles(ctx_t* c, const uint32_t (*prims)[4], uint64_t nprims, const float* vout,
+                              int vstride, int nvary, uint64_t pb, uint64_t pe)
+{
+    const cpv_pipeline_desc* p = c->pipe;
+    const float* vp = c->draw->viewport;
+    const float vpw = vp[2], vph = vp[3];
+    const float half_x = 1.0f / vpw * 0.5f, half_y = 1.0f / vph * 0.5f;
+    int32_t limit_x = f2i_trunc(vpw), limit_y = f2i_trunc(vph);
+    /* Guard the attachment extent (a viewport larger than the framebuffer
+     * writes out of bounds in the reference). */
+    if (limit_x > (int32_t)c->pass->width) limit_x = (int32_t)c->pass->width;
+    if (limit_y > (int32_t)c->pass->height) limit_y = (int32_t)c->pass->height;
+    if (pe > nprims) pe = nprims;
+
+    for (uint64_t pi = pb; pi < pe; pi++) {
+        uint32_t provoking = prims[pi][0];
+        uint32_t i0 = prims[pi][1], i1 = prims[pi][2], i2 = prims[pi][3];
+        if (p->front_face == CPV_FRONT_FACE_CW) { uint32_t t = i0; i0 = i2; i2 = t; }
+        const float* q[3] = {vout + (uint64_t)i0 * vstride, vout + (uint64_t)i1 * vstride, vout + (uint64_t)i2 * vstride};
+        float P[3][4];
+        for (int k = 0; k < 3; k++) {
+            P[k][0] = q[k][0] / q[k][3];
+            P[k][1] = q[k][1] / q[k][3];
+            P[k][2] = q[k][2] / q[k][3];
+            P[k][3] = q[k][3];
+        }
+        int32_t sx[3], sy[3];
+        for (int k = 0; k < 3; k++) {
+            sx[k] = f2i_trunc((P[k][0] + 1.0f) * 0.5f * vpw);
+            sy[k] = f2i_trunc((P[k][1] + 1.0f) * 0.5f * vph);
+        }
+        int32_t mnx = sx[0], mny = sy[0], mxx = sx[0], mxy = sy[0];
+        for (int k = 1; k < 3; k++) {
+            if (sx[k] < mnx) mnx = sx[k];
+            if (sy[k] < mny) mny = sy[k];
+            if (sx[k] > mxx) mxx = sx[k];
+            if (sy[k] > mxy) mxy = sy[k];
+        }
+        int32_t start_x = mnx > 0 ? mnx : 0, start_y = mny > 0 ? mny : 0;
+        int32_t end_x = (int32_t)((int64_t)mxx + 1 < limit_x ? (int64_t)mxx + 1 : limit_x);
+        int32_t end_y = (int32_t)((int64_t)mxy + 1 < limit_y ? (int64_t)mxy + 1 : limit_y);
+
+        for (int32_t y = start_y; y < end_y; y++) {
+            const float yf = ((float)y / vph + half_y) * 2.0f - 1.0f;
+            for (int32_t x = start_x; x < end_x; x++) {
+                const float xf = ((float)x / vpw + half_x) * 2.0f - 1.0f;
+                /* GetFragmentInput (:874-954) */
+                float area = edge(P[0], P[1], P[2][0], P[2][1]);
+                float w0, w1, w2;
+                int front;
+                if (area < 0) {
+                    area = -area;
+                    front = 0;
+                    w0 = edge(P[2], P[1], xf, yf);
+                    w1 = edge(P[0], P[2], xf, yf);
+                    w2 = edge(P[1], P[0], xf, yf);
+                } else {
+                    front = 1;
+                    w0 = edge(P[1], P[2], xf, yf);
+                    w1 = edge(P[2], P[0], xf, yf);
+                    w2 = edge(P[0], P[1], xf, yf);
+                }
+                if (w0 < 0 || w1 < 0 || w2 < 0 || ((p->cull_mode & CPV_CULL_BACK) && !front) ||
+                    ((p->cull_mode & CPV_CULL_FRONT) && front))
+                    continue;
+                c->fragments++;
+                w0 /= area;
+                w1 /= area;
+                w2 /= area;
+                float depth = P[0][2] * w0 + P[1][2] * w1 + P[2][2] * w2;
+                float in[16];
+                const float pw[3] = {P[0][3], P[1][3], P[2][3]};
+                const float w[3] = {w0, w1, w2};
+                for (int k = 0; k < nvary; k++) {
+                    const float v[3] = {q[0][4 + k], q[1][4 + k], q[2][4 + k]};
+                    in[k] = set_datum_perspective(pw, v, w);
+                }
+                (void)provoking;
+                /* DrawPixel (:1300-1313) */
+                float dt = (vp[5] - vp[4]) * depth + vp[4];
+                float colour[4];
+                if (run_fs(p, c->draw, in, colour)) continue;
+                fragment_ops(c, x, y, dt, front, colour);
+            }
+        }
+    }
+}
+
+/* CalculatePrimitives for triangle topologies (:567-673): rows of
+ * {provoking, v0, v1, v2} indexing the assembled vertex list. */
+static uint32_t (*assemble(uint32_t topology, uint64_t nverts, uint64_t* nprims))[4]
+{
+    uint64_t n = 0;
+    if (topology == CPV_TOPOLOGY_TRIANGLE_LIST) n = nverts / 3;
+    else if (topology == CPV_TOPOLOGY_TRIANGLE_STRIP || topology == CPV_TOPOLOGY_TRIANGLE_FAN) n = nverts > 2 ? nverts - 2 : 0;
+    uint32_t (*pr)[4] = (uint32_t (*)[4])malloc(sizeof(uint32_t[4]) * (n ? n : 1));
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t ii = (uint32_t)i;
+        if (topology == CPV_TOPOLOGY_TRIANGLE_LIST) {
+            pr[i][0] = ii * 3; pr[i][1] = ii * 3; pr[i][2] = ii * 3 + 1; pr[i][3] = ii * 3 + 2;
+        } else if (topology == CPV_TOPOLOGY_TRIANGLE_STRIP) {
+            pr[i][0] = ii; pr[i][1] = ii; pr[i][2] = ii + 1; pr[i][3] = ii + 2;
+        } else {
+            pr[i][0] = ii + 1; pr[i][1] = 0; pr[i][2] = ii + 1; pr[i][3] = ii + 2;
+        }
+    }
+    *nprims = n;
+    return pr;
+}
+
+static int draw_one(ctx_t* c, uint64_t pb, uint64_t pe)
+{
+    const cpv_pipeline_desc* p = c->pipe;
+    const cpv_draw_desc* d = c->draw;
+    if (p->topology != CPV_TOPOLOGY_TRIANGLE_LIST && p->topology != CPV_TOPOLOGY_TRIANGLE_STRIP &&
+        p->topology != CPV_TOPOLOGY_TRIANGLE_FAN)
+        return CPV_ERROR_UNSUPPORTED;
+    if (p->polygon_mode != CPV_POLYGON_FILL || p->rasterizer_discard || p->depth_clamp || p->logic_op_enable)
+        return CPV_ERROR_UNSUPPORTED;
+    if (p->color_count && p->blend[0].color_write_mask != CPV_COLOR_RGBA) return CPV_ERROR_UNSUPPORTED;
+    vs_info_t vi;
+    if (!vs_info(p->vs_kind, &vi) || fs_inputs(p->fs_kind) < 0) return CPV_ERROR_UNSUPPORTED;
+    int nvary = fs_inputs(p->fs_kind);
+    if (nvary > vi.nout) return CPV_ERROR_INVALID;
+    const int vstride = 4 + vi.nout;
+    for (uint32_t inst = 0; inst < d->instance_count; inst++) {
+        uint64_t n = d->count;
+        /* ProcessInputAssembler(Indexed): vertex ids per raw index */
+        uint32_t* vid = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+        for (uint64_t i = 0; i < n; i++) {
+            if (d->indexed) {
+                const uint8_t* ib = (const uint8_t*)(uintptr_t)d->index_address;
+                uint64_t at = d->first + i;
+                uint32_t index;
+                if (d->index_size == 1) index = ib[at];
+                else if (d->index_size == 2) { uint16_t t; memcpy(&t, ib + at * 2, 2); index = t; }
+                else { memcpy(&index, ib + at * 4, 4); }
+                if (p->primitive_restart &&
+                    index == (d->index_size == 1 ? 0xFFu : d->index_size == 2 ? 0xFFFFu : 0xFFFFFFFFu)) {
+                    free(vid);
+                    return CPV_ERROR_UNSUPPORTED;
+                }
+                vid[i] = (uint32_t)((int64_t)d->vertex_offset + index);
+            } else {
+                vid[i] = d->first + (uint32_t)i;
+            }
+        }
+        /* ProcessVertexShader: one invocation per assembled vertex */
+        float* vout = (float*)malloc(sizeof(float) * (size_t)vstride * (n ? n : 1));
+        for (uint64_t i = 0; i < n; i++) {
+            int rc = run_vs(p, d, vid[i], d->first_instance + inst, vout + i * vstride, vout + i * vstride + 4);
+            if (rc) { free(vid); free(vout); return rc; }
+        }
+        uint64_t nprims;
+        uint32_t (*prims)[4] = assemble(p->topology, n, &nprims);
+        process_triangles(c, (const uint32_t (*)[4])prims, nprims, vout, vstride, nvary, pb, pe);
+        free(prims);
+        free(vout);
+        free(vid);
+    }
+    return CPV_OK;
+}
+
+static int clear_pass(const cpv_pass_desc* pass)
+{
+    uint64_t npx = (uint64_t)pass->width * pass->height;
+    for (uint32_t a = 0; a < pass->color_count; a++) {
+        const cpv_attachment_desc* at = &pass->color[a];
+        if (at->load_op != CPV_LOAD_OP_CLEAR) continue;
+        fmt_t f;
+        if (!fmt_get(at->format, &f) || f.type != FT_NORMAL) return CPV_ERROR_UNSUPPORTED;
+        uint8_t px[16];
+        int rc;
+        /* SetPixel(VkClearColorValue) by base type (ImageSampler.cpp:749-774) */
+        if (f.base == BT_UINT) rc = cpvref_pack_u32(at->format, at->clear.u32, px);
+        else if (f.base == BT_SINT) rc = cpvref_pack_i32(at->format, at->clear.i32, px);
+        else rc = cpvref_pack_f32(at->format, at->clear.f32, px);
+        if (rc) return rc;
+        uint8_t* dst = (uint8_t*)(uintptr_t)at->address;
+        for (uint64_t i = 0; i < npx; i++) memcpy(dst + i * f.total, px, f.total);
+    }
+    if (pass->has_depth_stencil && pass->depth_stencil.load_op == CPV_LOAD_OP_CLEAR) {
+        const cpv_attachment_desc* at = &pass->depth_stencil;
+        fmt_t f;
+        if (!fmt_get(at->format, &f) || f.type != FT_DEPTH_STENCIL) return CPV_ERROR_UNSUPPORTED;
+        uint8_t px[8] = {0};
+        cpvref_pack_depth_stencil(at->format, at->clear.ds.depth, at->clear.ds.stencil, px);
+        uint8_t* dst = (uint8_t*)(uintptr_t)at->address;
+        for (uint64_t i = 0; i < npx; i++) memcpy(dst + i * f.total, px, f.total);
+    }
+    return CPV_OK;
+}
+
+int cpvref_render_pass_range(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                             const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
+                             uint32_t draw_count, uint64_t prim_begin, uint64_t prim_end,
+                             uint64_t* fragments)
+{
+    if (!pass || (draw_count && (!pipelines || !draws))) return CPV_ERROR_INVALID;
+    if (pass->color_count > CPV_MAX_COLOR_ATTACHMENTS) return CPV_ERROR_UNSUPPORTED;
+    int rc = clear_pass(pass);
+    if (rc) return rc;
+    uint64_t frags = 0;
+    for (uint32_t i = 0; i < draw_count; i++) {
+        ctx_t c;
+        memset(&c, 0, sizeof(c));
+        c.pass = pass;
+        c.pipe = &pipelines[draw_pipeline ? draw_pipeline[i] : 0];
+        c.draw = &draws[i];
+        c.has_color = pass->color_count > 0 && c.pipe->color_count > 0;
+        if (c.has_color && !fmt_get(pass->color[0].format, &c.cfmt)) return CPV_ERROR_UNSUPPORTED;
+        c.has_ds = pass->has_depth_stencil != 0;
+        if (c.has_ds && !fmt_get(pass->depth_stencil.format, &c.dsfmt)) return CPV_ERROR_UNSUPPORTED;
+        rc = draw_one(&c, prim_begin, prim_end);
+        if (rc) return rc;
+        frags += c.fragments;
+    }
+    if (fragments) *fragments = frags;
+    return CPV_OK;
+}
+
+int cpvref_render_pass(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                       const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
+                       uint32_t draw_count, uint64_t* fragments)
+{
+    return cpvref_render_pass_range(pass, pipelines, draws, draw_pipeline, draw_count, 0, UINT64_MAX, fragments);
+}
+
+int cpvref_pixel_coverage(const float positions[12], uint32_t cull_mode, uint32_t front_face,
+                          float vp_width, float vp_height, int32_t x, int32_t y,
+                          float* depth, int* front, float weights[3])
+{
+    int o[3] = {0, 1, 2};
+    if (front_face == CPV_FRONT_FACE_CW) { o[0] = 2; o[2] = 0; }
+    float P[3][4];
+    for (int k = 0; k < 3; k++) {
+        const float* q = positions + 4 * o[k];
+        P[k][0] = q[0] / q[3]; P[k][1] = q[1] / q[3]; P[k][2] = q[2] / q[3]; P[k][3] = q[3];
+    }
+    const float half_x = 1.0f / vp_width * 0.5f, half_y = 1.0f / vp_height * 0.5f;
+    const float yf = ((float)y / vp_height + half_y) * 2.0f - 1.0f;
+    const float xf = ((float)x / vp_width + half_x) * 2.0f - 1.0f;
+    float area = edge(P[0], P[1], P[2][0], P[2][1]);
+    float w0, w1, w2;
+    int fr;
+    if (area < 0) {
+        area = -area; fr = 0;
+        w0 = edge(P[2], P[1], xf, yf); w1 = edge(P[0], P[2], xf, yf); w2 = edge(P[1], P[0], xf, yf);
+    } else {
+        fr = 1;
+        w0 = edge(P[1], P[2], xf, yf); w1 = edge(P[2], P[0], xf, yf); w2 = edge(P[0], P[1], xf, yf);
+    }
+    *front = fr;
+    if (w0 < 0 || w1 < 0 || w2 < 0 || ((cull_mode & CPV_CULL_BACK) && !fr) || ((cull_mode & CPV_CULL_FRONT) && fr))
+        return 0;
+    w0 /= area; w1 /= area; w2 /= area;
+    weights[0] = w0; weights[1] = w1; weights[2] = w2;
+    *depth = P[0][2] * w0 + P[1][2] * w1 + P[2][2] * w2;
+    return 1;
+}

@@ -1,0 +1,73 @@
+/*
+ * cpvref -- CPU oracle for the CPVulkan draw path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP path is
+ * compared against; it is loaded by tests/, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg, and by nothing else.  The product (libcpv.so,
+ * the ICD) never links, loads or calls it.
+ *
+ * It is a single-threaded C restatement of the reference's draw loop and
+ * its JIT'd per-pipeline / per-format code, consuming the same descriptor
+ * structs as the C-ABI in include/cpv.h but with HOST addresses.  Each
+ * function cites the reference file:line it follows (paths relative to the
+ * reference repository).
+ */
+#ifndef CPVREF_H
+#define CPVREF_H
+
+#include <stdint.h>
+
+#include "../include/cpv.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Whole render pass: clears per load_op, then every draw in order.
+ * *fragments receives the number of (primitive, pixel) pairs that passed
+ * coverage and culling.  Returns 0 or a negative CPV_ERROR_* code. */
+int cpvref_render_pass(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                       const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
+                       uint32_t draw_count, uint64_t* fragments);
+
+/* Same, restricted to primitives [prim_begin, prim_end) of each draw
+ * (used to time a bounded sample of a large draw). */
+int cpvref_render_pass_range(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                             const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
+                             uint32_t draw_count, uint64_t prim_begin, uint64_t prim_end,
+                             uint64_t* fragments);
+
+/* Format conversions (exposed for the format tests). */
+int cpvref_pack_f32(uint32_t format, const float rgba[4], uint8_t* dst);
+int cpvref_unpack_f32(uint32_t format, const uint8_t* src, float rgba[4]);
+int cpvref_pack_u32(uint32_t format, const uint32_t rgba[4], uint8_t* dst);
+int cpvref_pack_i32(uint32_t format, const int32_t rgba[4], uint8_t* dst);
+int cpvref_pack_depth_stencil(uint32_t format, float depth, uint32_t stencil, uint8_t* dst);
+float cpvref_unpack_depth(uint32_t format, const uint8_t* src);
+uint32_t cpvref_unpack_stencil(uint32_t format, const uint8_t* src);
+uint32_t cpvref_format_size(uint32_t format);
+
+/* Mip chain layout of GetNormalImageSize: level offsets (bytes) and total. */
+uint64_t cpvref_image_layout(uint32_t format, uint32_t width, uint32_t height, uint32_t levels,
+                             uint64_t* level_offsets);
+
+/* Texture sampling through ImageSampleImplicitLod (lambda = 0). */
+int cpvref_sample_implicit(const cpv_texture_desc* texture, float u, float v, float rgba[4]);
+
+/* Blend of one fragment (ApplyBlend, dead code in the reference). */
+void cpvref_apply_blend(const float src[4], const float dst[4], const float constant[4],
+                        const cpv_blend_attachment* state, float out[4]);
+
+/* One edge function value and the coverage decision for one pixel of one
+ * triangle exactly as ProcessTriangles/GetFragmentInput compute them.
+ * positions: 3 clip-space vec4 (post-VS, before any swap). Returns 1 if the
+ * pixel is covered (and not culled); fills depth, front, weights. */
+int cpvref_pixel_coverage(const float positions[12], uint32_t cull_mode, uint32_t front_face,
+                          float vp_width, float vp_height, int32_t x, int32_t y,
+                          float* depth, int* front, float weights[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,35 @@
+"""Helpers shared by the GPU parity tests: render a scene on the HIP path and
+with the oracle, and compare."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle_lib
+from cpvulkan_amd.workloads import DeviceMemory, Scene, materialize
+
+
+def render_gpu(ctx, scene: Scene):
+    mem = DeviceMemory(ctx)
+    try:
+        m = materialize(scene, mem)
+        ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
+        ctx.synchronize()
+        frags = ctx.fragment_count()
+        return m.read_color(), m.read_ds(), frags
+    finally:
+        mem.release()
+
+
+def compare(scene: Scene, gpu, ref, label: str = "") -> None:
+    gc, gd, gf = gpu
+    rc, rd, rf = ref
+    if not np.array_equal(gc, rc):
+        diff = np.nonzero(gc != rc)[0]
+        px = diff // max(1, len(gc) // (scene.width * scene.height))
+        raise AssertionError(f"{label} colour differs in {diff.size} bytes; first pixels "
+                             f"{[(int(p % scene.width), int(p // scene.width)) for p in px[:5]]}; "
+                             f"gpu {gc[diff[:8]]} ref {rc[diff[:8]]}")
+    if not np.array_equal(gd, rd):
+        diff = np.nonzero(gd != rd)[0]
+        raise AssertionError(f"{label} depth/stencil differs in {diff.size} bytes (first at {diff[:5]})")
+    assert gf == rf, f"{label} fragment count {gf} != oracle {rf}"
