@@ -218,7 +218,6 @@ struct EmitArgs {
     const TargetParams* targets;
     uint32_t* keys;               // global tile index per pair
     uint32_t* vals;               // primitive index per pair
-    uint32_t* tile_counts;        // pairs per global tile
     uint32_t prims;
     int32_t tile_shift;
 };
@@ -241,11 +240,21 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
                 const uint32_t key = t.tile_base + ty * t.tiles_x + tx;
                 a.keys[off] = key;
                 a.vals[off] = gp;
-                atomicAdd(&a.tile_counts[key], 1u);
                 off++;
             }
         }
     }
+}
+
+// Start of every tile's range in the tile-sorted pair array: each boundary
+// between different keys writes the starts of the tiles it spans.
+__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t* keys, uint32_t n, uint32_t tiles, uint32_t* start)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i > n) return;
+    const uint32_t prev = i == 0 ? 0u : keys[i - 1] + 1u;   // first tile not before this boundary
+    const uint32_t cur = i == n ? tiles : keys[i];          // last tile starting here
+    for (uint32_t t = prev; t <= cur && t <= tiles; t++) start[t] = i;
 }
 
 // ---------------------------------------------------------- radix sort

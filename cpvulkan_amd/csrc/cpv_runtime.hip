@@ -168,7 +168,7 @@ struct cpv_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_raster0 = nullptr, ev_raster1 = nullptr, ev_end = nullptr;
     cpv_config cfg{};
-    DevBuf vout, recs, counts, offsets, keys[2], vals[2], tile_counts, tile_start, hist, hist_base, sums;
+    DevBuf vout, recs, counts, offsets, keys[2], vals[2], tile_start, hist, hist_base, sums;
     DevBuf arena, stats, traces;
     void* pinned = nullptr;
     size_t pinned_cap = 0;
@@ -210,7 +210,7 @@ extern "C" void cpv_context_destroy(cpv_context* c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->vout, &c->recs, &c->counts, &c->offsets, &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1],
-                      &c->tile_counts, &c->tile_start, &c->hist, &c->hist_base, &c->sums, &c->arena, &c->stats, &c->traces};
+                      &c->tile_start, &c->hist, &c->hist_base, &c->sums, &c->arena, &c->stats, &c->traces};
     for (DevBuf* b : bufs) b->release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->readback) (void)hipHostFree(c->readback);
@@ -670,10 +670,14 @@ static int run_launch(cpv_context* c, const Program& prog, const std::vector<Tar
         vout_total += nverts;
         prim_total += o.prim_count;
     }
+    if (prim_total >= (1u << 28)) return fail(CPV_ERROR_UNSUPPORTED, "more than 2^28 primitives in one launch");
     uint32_t tiles = 0;
     for (const TargetParams& t : targets) tiles = std::max(tiles, t.tile_base + t.tiles_x * t.tiles_y);
 
-    // descriptors -> device arena
+    // descriptors -> device arena (the block tables are never empty)
+    if (vblock.empty()) vblock.push_back(0);
+    if (sblock.empty()) sblock.push_back(0);
+    const bool any_vertices = vout_total > 0, any_prims = prim_total > 0;
     Staging st;
     const size_t o_env = st.put(envs.data(), envs.size() * sizeof(DrawEnv));
     const size_t o_dd = st.put(dd.data(), dd.size() * sizeof(DrawDesc));
@@ -705,17 +709,17 @@ static int run_launch(cpv_context* c, const Program& prog, const std::vector<Tar
     // buffers
     if (c->vout.ensure((size_t)std::max(vout_total, 1u) * stride * 4) || c->recs.ensure((size_t)std::max(prim_total, 1u) * sizeof(TriSetup)) ||
         c->counts.ensure((size_t)std::max(prim_total, 1u) * 4) || c->offsets.ensure(((size_t)prim_total + 1) * 4) ||
-        c->tile_counts.ensure((size_t)std::max(tiles, 1u) * 4) || c->tile_start.ensure(((size_t)tiles + 1) * 4) ||
+        c->tile_start.ensure(((size_t)tiles + 1) * 4) ||
         c->hist.ensure((size_t)256 * kSortSegments * 4) || c->hist_base.ensure(((size_t)256 * kSortSegments + 1) * 4))
         return CPV_ERROR_OUT_OF_MEMORY;
 
     // 1. vertex stage
-    if (!vblock.empty()) {
+    if (any_vertices) {
         VertexArgs va{d_dd, d_vb, c->vout.as<float>(), stride, 0};
         HIP_TRY(launch(prog.vertex, dim3((uint32_t)vblock.size()), dim3(256), &va, sizeof(va), c->stream));
     }
     // 2. triangle setup
-    if (!sblock.empty()) {
+    if (any_prims) {
         SetupArgs sa{d_dd, d_sb, d_tp, c->vout.as<float>(), c->recs.as<TriSetup>(), c->counts.as<uint32_t>(), stride, tile_shift};
         hipLaunchKernelGGL(setup_kernel, dim3((uint32_t)sblock.size()), dim3(256), 0, c->stream, sa);
         HIP_TRY(hipGetLastError());
@@ -729,17 +733,14 @@ static int run_launch(cpv_context* c, const Program& prog, const std::vector<Tar
     if (c->keys[0].ensure((size_t)std::max(pairs, 1u) * 4) || c->keys[1].ensure((size_t)std::max(pairs, 1u) * 4) ||
         c->vals[0].ensure((size_t)std::max(pairs, 1u) * 4) || c->vals[1].ensure((size_t)std::max(pairs, 1u) * 4))
         return CPV_ERROR_OUT_OF_MEMORY;
-    // 4. emit pairs in primitive order, count per tile
-    fill_u32(c, c->tile_counts.p, 0u, tiles);
+    // 4. emit pairs in primitive order
     if (prim_total) {
         EmitArgs ea{c->recs.as<TriSetup>(), c->counts.as<uint32_t>(), c->offsets.as<uint32_t>(), d_dd, d_tp,
-                    c->keys[0].as<uint32_t>(), c->vals[0].as<uint32_t>(), c->tile_counts.as<uint32_t>(), prim_total, tile_shift};
+                    c->keys[0].as<uint32_t>(), c->vals[0].as<uint32_t>(), prim_total, tile_shift};
         const uint32_t blocks = std::min<uint32_t>((prim_total + 255) / 256, 8192);
         hipLaunchKernelGGL(emit_kernel, dim3(blocks), dim3(256), 0, c->stream, ea);
         HIP_TRY(hipGetLastError());
     }
-    rc = scan_u32(c, c->tile_counts.as<uint32_t>(), tiles, c->tile_start.as<uint32_t>(), true);
-    if (rc) return rc;
     // 5. stable radix sort of the pairs by tile
     int src = 0;
     if (pairs > 1 && tiles > 1) {
@@ -759,6 +760,10 @@ static int run_launch(cpv_context* c, const Program& prog, const std::vector<Tar
             src = 1 - src;
         }
     }
+    // tile ranges from the sorted keys
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3(pairs / 256 + 1), dim3(256), 0, c->stream, (const uint32_t*)c->keys[src].p, pairs,
+                       tiles, c->tile_start.as<uint32_t>());
+    HIP_TRY(hipGetLastError());
     // 6. raster
     const int ci = tile_config_index(c->cfg.tile_size, c->cfg.waves_per_tile);
     const KernelRef& rk = prog.raster[ci];

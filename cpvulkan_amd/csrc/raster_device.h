@@ -68,8 +68,13 @@ CPV_DEV void unpack_color(const FormatDesc& f, const uint32_t px[4], float out[4
         if (f.base == kBaseSFloat) {
             v = __uint_as_float(raw);
         } else if (f.base == kBaseUNorm || f.base == kBaseUScaled || f.base == kBaseUInt) {
-            const float maxv = f.element == 1 ? 255.0f : f.element == 2 ? 65535.0f : 4294967296.0f;
-            v = (float)raw / maxv;
+            // (float)u / 255.0f and (float)u / 65535.0f, computed as one double
+            // multiply rounded to float: identical for every 8- and 16-bit code
+            // (checked exhaustively), a quarter of the instructions of a
+            // correctly rounded fp32 division.
+            if (f.element == 1) v = (float)((double)raw * (1.0 / 255.0));
+            else if (f.element == 2) v = (float)((double)raw * (1.0 / 65535.0));
+            else v = (float)raw / 4294967296.0f;
         } else {
             const int32_t s = f.element == 1 ? (int32_t)(int8_t)raw : f.element == 2 ? (int32_t)(int16_t)raw : (int32_t)raw;
             const float maxv = f.element == 1 ? 127.0f : f.element == 2 ? 32767.0f : 2147483648.0f;
@@ -130,8 +135,10 @@ CPV_DEV void pack_color_i32(const FormatDesc& f, const int32_t in[4], uint32_t p
 // as two dwords: dword 0 = bytes 0-3, dword 1 = bytes 4-7.
 CPV_DEV float unpack_depth(const FormatDesc& f, uint32_t d0)
 {
-    if (f.depth_kind == 16) return (float)(d0 & 0xFFFFu) / 65535.0f;
-    if (f.depth_kind == 24) return (float)(d0 & 0x00FFFFFFu) / 16777215.0f;
+    // same double-multiply form as unpack_color, exhaustively equal to the
+    // fp32 divisions u / 65535.0f and u / 16777215.0f
+    if (f.depth_kind == 16) return (float)((double)(d0 & 0xFFFFu) * (1.0 / 65535.0));
+    if (f.depth_kind == 24) return (float)((double)(d0 & 0x00FFFFFFu) * (1.0 / 16777215.0));
     if (f.depth_kind == 32) return __uint_as_float(d0);
     return 0.0f;
 }
@@ -476,6 +483,65 @@ CPV_DEV bool depth_stencil_ops(const FragmentState& st, PixelState& px, float de
         pack_depth_stencil(f, depth, s, px.ds0, px.ds1);
     }
     return depth_ok;
+}
+
+// Conservative early depth rejection.  The reference computes
+//   w_i = e_i / area (correctly rounded), depth = (z0*w0 + z1*w1) + z2*w2,
+//   dt = (maxD - minD) * depth + minD
+// and then tests dt against the stored depth.  Here the same depth is
+// approximated with one reciprocal (v_rcp_f32, <= 2 ulp assumed) instead of
+// three divisions.  Both values lie within 12.2u * S of the real-number
+// depth sum(z_i e_i / area), u = 2^-24, S = sum |z_i w_i|, so the exact
+// depth lies in [da - B, da + B] with B = 32u * S + 2^-120 (the constant
+// covers subnormal results; the fp32 rounding of da +- B is absorbed by the
+// factor-2.6 margin).  dt is a monotone function of depth, so the interval
+// maps to [tmin, tmax].  Returns true only when the depth test fails for
+// every depth in the interval; any NaN or infinity returns false and the
+// caller takes the exact path.
+CPV_DEV bool depth_surely_fails(int32_t op, float cur, float e0, float e1, float e2, float area,
+                                float z0, float z1, float z2, float vp_min, float vp_max)
+{
+    const float r = __builtin_amdgcn_rcpf(area);
+    const float t0 = z0 * (e0 * r), t1 = z1 * (e1 * r), t2 = z2 * (e2 * r);
+    const float da = (t0 + t1) + t2;
+    const float S = (fabsf(t0) + fabsf(t1)) + fabsf(t2);
+    const float B = S * 0x1p-19f + 0x1p-120f;
+    const float lo = da - B, hi = da + B;
+    const float sc = vp_max - vp_min;
+    const float tl = sc * lo + vp_min, th = sc * hi + vp_min;
+    if (!(__builtin_isfinite(tl) && __builtin_isfinite(th))) return false;
+    const float tmin = fminf(tl, th), tmax = fmaxf(tl, th);
+    switch (op) {
+    case 0: return true;                           // NEVER
+    case 1: return tmin >= cur;                    // LESS fails when dt >= cur
+    case 2: return tmin > cur || tmax < cur;       // EQUAL
+    case 3: return tmin > cur;                     // LESS_OR_EQUAL
+    case 4: return tmax <= cur;                    // GREATER
+    case 6: return tmax < cur;                     // GREATER_OR_EQUAL
+    default: return false;                         // NOT_EQUAL, ALWAYS
+    }
+}
+
+// depth_surely_fails for the viewport depth range [0, 1], where
+// (1 - 0) * depth + 0 equals depth for every finite depth.
+CPV_DEV bool depth_surely_fails_unit(int32_t op, float cur, float e0, float e1, float e2, float area, float z0, float z1, float z2)
+{
+    const float r = __builtin_amdgcn_rcpf(area);
+    const float t0 = z0 * (e0 * r), t1 = z1 * (e1 * r), t2 = z2 * (e2 * r);
+    const float da = (t0 + t1) + t2;
+    const float S = (fabsf(t0) + fabsf(t1)) + fabsf(t2);
+    const float B = S * 0x1p-19f + 0x1p-120f;
+    const float lo = da - B, hi = da + B;
+    if (!(__builtin_isfinite(lo) && __builtin_isfinite(hi))) return false;
+    switch (op) {
+    case 0: return true;
+    case 1: return lo >= cur;
+    case 2: return lo > cur || hi < cur;
+    case 3: return lo > cur;
+    case 4: return hi <= cur;
+    case 6: return hi < cur;
+    default: return false;
+    }
 }
 
 // Colour write of CompileWriteFragmentAttachment/Blend (PipelineCompiler.cpp:1528-1727)

@@ -98,6 +98,20 @@ __device__ __forceinline__ float interpolate(int mode, const float w[3], const f
     return num / den;
 }
 
+// Perspective SetDatum with all three clip w equal to 1: (w*v)/1 and w/1 are
+// exact, so only the final division remains.
+__device__ __forceinline__ float interpolate_unit_w(const float w[3], float v0, float v1, float v2)
+{
+    float num = 0.0f, den = 0.0f;
+    num += w[0] * v0;
+    den += w[0];
+    num += w[1] * v1;
+    den += w[1];
+    num += w[2] * v2;
+    den += w[2];
+    return num / den;
+}
+
 __device__ __forceinline__ uint32_t find_target(const TargetParams* t, uint32_t n, uint32_t tile)
 {
     uint32_t lo = 0, hi = n;
@@ -117,9 +131,14 @@ __global__ __launch_bounds__(64 * WAVES) void raster_kernel(RasterArgs a)
     constexpr int BPW = BLOCKS / WAVES;        // blocks (pixels) per lane
     static_assert(BLOCKS % WAVES == 0, "tile blocks must divide among the waves");
 
-    __shared__ TriSetup s_rec[NT];
-    __shared__ uint32_t s_stage[TILE * TILE * 4];   // up to 16 bytes per pixel
+    // The record batch and the framebuffer staging area are never live at
+    // the same time, so they share LDS.
+    constexpr int kRecWords = NT * 32;
+    constexpr int kStageWords = TILE * TILE * 4;     // up to 16 bytes per pixel
+    __shared__ uint32_t s_lds[kRecWords > kStageWords ? kRecWords : kStageWords];
     __shared__ uint32_t s_item;
+    TriSetup* s_rec = (TriSetup*)s_lds;
+    uint32_t* s_stage = s_lds;
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -215,10 +234,71 @@ __global__ __launch_bounds__(64 * WAVES) void raster_kernel(RasterArgs a)
         const uint32_t lbeg = a.tile_start[tile], lend = a.tile_start[tile + 1];
         uint32_t cur_draw = 0xFFFFFFFFu;
         float xf[BPW], yf[BPW];
-        float vp_min = 0.0f, vp_max = 1.0f, vp_w = 1.0f;
+        float vp_min = 0.0f, vp_max = 1.0f;
+        bool vp_identity = true;
         const FragmentState* fst = nullptr;
         const DrawEnv* env = nullptr;
         unsigned long long nfrag = 0;
+        bool early = false;   // conservative depth rejection allowed for this draw
+        int32_t depth_op = 7;
+        int32_t depth_kind = 0;
+
+        // Deferred colour work: a fragment that passed the depth/stencil
+        // tests has its depth/stencil written at once, and its colour
+        // (interpolation, fragment shader, blend, pack) queued.  Queues drain
+        // in order, so every pixel still sees its colour writes in
+        // primitive order; draining for all lanes at once keeps the shading
+        // path from running once per primitive whenever any lane passes.
+        constexpr int Q = FS::kMayDiscard ? 1 : 4;
+        uint32_t q_prim[Q];
+        float q_w0[Q], q_w1[Q], q_w2[Q];
+        int qn = 0;
+
+        auto shade = [&](uint32_t gprim, int k, float w0, float w1, float w2) {
+            const TriSetup& g = a.recs[gprim];
+            const DrawDesc& d = a.draws[g.draw];
+            const FragmentState& fs = a.fstates[d.fstate];
+            const DrawEnv& de = *(const DrawEnv*)d.env;
+            const float w[3] = {w0, w1, w2};
+            const float pw[3] = {g.w[0], g.w[1], g.w[2]};
+            const float* q0 = a.vout + (uint64_t)g.v[0] * a.vout_stride + 4;
+            const float* q1 = a.vout + (uint64_t)g.v[1] * a.vout_stride + 4;
+            const float* q2 = a.vout + (uint64_t)g.v[2] * a.vout_stride + 4;
+            const float* qp = a.vout + (uint64_t)g.provoking * a.vout_stride + 4;
+            float in[FS::kInputs > 0 ? FS::kInputs : 1];
+            // x / 1.0f == x exactly, so clip w == 1 drops the divisions
+            const bool unit_w = pw[0] == 1.0f && pw[1] == 1.0f && pw[2] == 1.0f;
+#pragma unroll
+            for (int c = 0; c < FS::kInputs; c++) {
+                const int mode = FS::interp(c);
+                in[c] = (mode == 0 && unit_w) ? interpolate_unit_w(w, q0[c], q1[c], q2[c])
+                                              : interpolate(mode, w, pw, q0[c], q1[c], q2[c], qp[c]);
+            }
+            const float depth = g.z[0] * w0 + g.z[1] * w1 + g.z[2] * w2;
+            const float fc[4] = {fs.origin_upper_left ? (float)pxx[k] : d.vp_w - (float)pxx[k] - 1.0f, (float)pxy[k], depth, 1.0f};
+            float colour[4];
+            FS::run(de, in, fc, (g.flags & 1u) != 0, colour);
+#pragma unroll
+            for (int kk = 0; kk < BPW; kk++)
+                if (kk == k) color_write(fs, px[kk], colour);
+        };
+        auto drain = [&]() {
+#pragma unroll
+            for (int e = 0; e < Q; e++) {
+                if (!__any(e < qn)) break;
+                if (e < qn) shade(q_prim[e] & 0x0FFFFFFFu, (int)(q_prim[e] >> 28), q_w0[e], q_w1[e], q_w2[e]);
+            }
+            qn = 0;
+        };
+
+        // wave bounds (union of its blocks) for the per-wave primitive filter
+        int wx0 = 1 << 30, wy0 = 1 << 30, wx1 = -(1 << 30), wy1 = -(1 << 30);
+#pragma unroll
+        for (int k = 0; k < BPW; k++) {
+            const int blk = wave + k * WAVES;
+            const int bx0 = tile_x0 + (blk % BW) * 8, by0 = tile_y0 + (blk / BW) * 8;
+            wx0 = min(wx0, bx0); wy0 = min(wy0, by0); wx1 = max(wx1, bx0 + 8); wy1 = max(wy1, by0 + 8);
+        }
 
         for (uint32_t base = lbeg; base < lend; base += NT) {
             const int n = (int)min((uint32_t)NT, lend - base);
@@ -230,74 +310,98 @@ __global__ __launch_bounds__(64 * WAVES) void raster_kernel(RasterArgs a)
                 for (int q = 0; q < 8; q++) dst[q] = src[q];
             }
             __syncthreads();
-            for (int j = 0; j < n; j++) {
-                const TriSetup& r = s_rec[j];
-                if (r.draw != cur_draw) {
-                    cur_draw = r.draw;
-                    const DrawDesc& d = a.draws[cur_draw];
-                    // pixel centres (ProcessTriangles :1571-1580)
+            for (int c0 = 0; c0 < n; c0 += 64) {
+                // each lane tests one primitive of the chunk against this wave's area
+                bool ov = false;
+                if (c0 + lane < n) {
+                    const TriSetup& t = s_rec[c0 + lane];
+                    ov = !(t.x1 <= wx0 || t.x0 >= wx1 || t.y1 <= wy0 || t.y0 >= wy1);
+                }
+                uint64_t mask = __ballot(ov);
+                while (mask) {
+                    const int j = c0 + (int)__builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const TriSetup& r = s_rec[j];
+                    const uint32_t rdraw = r.draw;
+                    if (rdraw != cur_draw) {
+                        cur_draw = rdraw;
+                        const DrawDesc& d = a.draws[cur_draw];
+                        // pixel centres (ProcessTriangles :1571-1580)
+#pragma unroll
+                        for (int k = 0; k < BPW; k++) {
+                            xf[k] = ((float)pxx[k] / d.vp_w + d.half_x) * 2.0f - 1.0f;
+                            yf[k] = ((float)pxy[k] / d.vp_h + d.half_y) * 2.0f - 1.0f;
+                        }
+                        vp_min = d.vp_min_depth; vp_max = d.vp_max_depth;
+                        vp_identity = vp_min == 0.0f && vp_max == 1.0f;
+                        fst = &a.fstates[d.fstate];
+                        env = (const DrawEnv*)d.env;
+                        // Rejecting on the depth test alone is exact when nothing
+                        // else observes the fragment: no depth-bounds or stencil
+                        // test (and so no stencil write).
+                        early = fst->depth_test && fst->has_ds && fst->ds.depth_off >= 0 && fst->ds.vk_format != 127u &&
+                                !fst->depth_bounds_test && !fst->stencil_test;
+                        depth_op = fst->depth_compare;
+                        depth_kind = fst->ds.depth_kind;
+                    }
+                    const int rx0 = r.x0, ry0 = r.y0, rx1 = r.x1, ry1 = r.y1;
+                    const uint32_t gprim = a.tile_prims[base + j];
 #pragma unroll
                     for (int k = 0; k < BPW; k++) {
-                        xf[k] = ((float)pxx[k] / d.vp_w + d.half_x) * 2.0f - 1.0f;
-                        yf[k] = ((float)pxy[k] / d.vp_h + d.half_y) * 2.0f - 1.0f;
-                    }
-                    vp_min = d.vp_min_depth; vp_max = d.vp_max_depth; vp_w = d.vp_w;
-                    fst = &a.fstates[d.fstate];
-                    env = (const DrawEnv*)d.env;
-                }
-                const int rx0 = r.x0, ry0 = r.y0, rx1 = r.x1, ry1 = r.y1;
-#pragma unroll
-                for (int k = 0; k < BPW; k++) {
-                    const int blk = wave + k * WAVES;
-                    const int bx0 = tile_x0 + (blk % BW) * 8, by0 = tile_y0 + (blk / BW) * 8;
-                    if (rx1 <= bx0 || rx0 >= bx0 + 8 || ry1 <= by0 || ry0 >= by0 + 8) continue;   // wave-uniform
-                    const int x = pxx[k], y = pxy[k];
-                    if (x < rx0 || x >= rx1 || y < ry0 || y >= ry1) continue;
-                    // GetFragmentInput (:874-903); NaN edge values are not < 0 and pass
-                    const float e0 = (xf[k] - r.ax[0]) * r.dy[0] - (yf[k] - r.ay[0]) * r.dx[0];
-                    const float e1 = (xf[k] - r.ax[1]) * r.dy[1] - (yf[k] - r.ay[1]) * r.dx[1];
-                    const float e2 = (xf[k] - r.ax[2]) * r.dy[2] - (yf[k] - r.ay[2]) * r.dx[2];
-                    if (e0 < 0 || e1 < 0 || e2 < 0) continue;
-                    nfrag++;
-                    const float w[3] = {e0 / r.area, e1 / r.area, e2 / r.area};
-                    const float depth = r.z[0] * w[0] + r.z[1] * w[1] + r.z[2] * w[2];
-                    // DrawPixel (:1300-1313)
-                    const float dt = (vp_max - vp_min) * depth + vp_min;
-                    const bool front = (r.flags & 1u) != 0;
-                    float in[FS::kInputs > 0 ? FS::kInputs : 1];
-                    float colour[4];
-                    bool write;
-                    if (FS::kMayDiscard) {
-                        const float pw[3] = {r.w[0], r.w[1], r.w[2]};
-                        const float* q0 = a.vout + (uint64_t)r.v[0] * a.vout_stride + 4;
-                        const float* q1 = a.vout + (uint64_t)r.v[1] * a.vout_stride + 4;
-                        const float* q2 = a.vout + (uint64_t)r.v[2] * a.vout_stride + 4;
-                        const float* qp = a.vout + (uint64_t)r.provoking * a.vout_stride + 4;
-#pragma unroll
-                        for (int c = 0; c < FS::kInputs; c++) in[c] = interpolate(FS::interp(c), w, pw, q0[c], q1[c], q2[c], qp[c]);
-                        const float fc[4] = {fst->origin_upper_left ? (float)x : vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
-                        if (FS::run(*env, in, fc, front, colour)) continue;
-                        write = depth_stencil_ops(*fst, px[k], dt, front);
-                    } else {
-                        // no discard: the shader result only feeds the colour
-                        // write, so the depth/stencil operations run first
-                        write = depth_stencil_ops(*fst, px[k], dt, front);
-                        if (write && fst->has_color) {
+                        // every lane is active here; a lane queues at most one
+                        // entry per iteration, so a free slot always exists
+                        if (!FS::kMayDiscard && __any(qn == Q)) drain();
+                        const int blk = wave + k * WAVES;
+                        const int bx0 = tile_x0 + (blk % BW) * 8, by0 = tile_y0 + (blk / BW) * 8;
+                        if (BPW > 1 && (rx1 <= bx0 || rx0 >= bx0 + 8 || ry1 <= by0 || ry0 >= by0 + 8)) continue;   // wave-uniform
+                        const int x = pxx[k], y = pxy[k];
+                        if (x < rx0 || x >= rx1 || y < ry0 || y >= ry1) continue;
+                        // GetFragmentInput (:874-903); NaN edge values are not < 0 and pass
+                        const float e0 = (xf[k] - r.ax[0]) * r.dy[0] - (yf[k] - r.ay[0]) * r.dx[0];
+                        const float e1 = (xf[k] - r.ax[1]) * r.dy[1] - (yf[k] - r.ay[1]) * r.dx[1];
+                        const float e2 = (xf[k] - r.ax[2]) * r.dy[2] - (yf[k] - r.ay[2]) * r.dx[2];
+                        if (e0 < 0 || e1 < 0 || e2 < 0) continue;
+                        nfrag++;
+                        if (early) {
+                            const float cur = depth_kind == 32 ? __uint_as_float(px[k].ds0) : unpack_depth(fst->ds, px[k].ds0);
+                            if (vp_identity ? depth_surely_fails_unit(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2])
+                                            : depth_surely_fails(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2], vp_min, vp_max))
+                                continue;
+                        }
+                        const float w0 = e0 / r.area, w1 = e1 / r.area, w2 = e2 / r.area;
+                        const float depth = r.z[0] * w0 + r.z[1] * w1 + r.z[2] * w2;
+                        // DrawPixel (:1300-1313)
+                        const float dt = (vp_max - vp_min) * depth + vp_min;
+                        const bool front = (r.flags & 1u) != 0;
+                        if (FS::kMayDiscard) {
+                            // the shader decides first whether anything is written
+                            const float wv[3] = {w0, w1, w2};
                             const float pw[3] = {r.w[0], r.w[1], r.w[2]};
                             const float* q0 = a.vout + (uint64_t)r.v[0] * a.vout_stride + 4;
                             const float* q1 = a.vout + (uint64_t)r.v[1] * a.vout_stride + 4;
                             const float* q2 = a.vout + (uint64_t)r.v[2] * a.vout_stride + 4;
                             const float* qp = a.vout + (uint64_t)r.provoking * a.vout_stride + 4;
+                            float in[FS::kInputs > 0 ? FS::kInputs : 1];
 #pragma unroll
-                            for (int c = 0; c < FS::kInputs; c++) in[c] = interpolate(FS::interp(c), w, pw, q0[c], q1[c], q2[c], qp[c]);
-                            const float fc[4] = {fst->origin_upper_left ? (float)x : vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
-                            FS::run(*env, in, fc, front, colour);
+                            for (int c = 0; c < FS::kInputs; c++) in[c] = interpolate(FS::interp(c), wv, pw, q0[c], q1[c], q2[c], qp[c]);
+                            const float fc[4] = {fst->origin_upper_left ? (float)x : a.draws[cur_draw].vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
+                            float colour[4];
+                            if (FS::run(*env, in, fc, front, colour)) continue;
+                            if (depth_stencil_ops(*fst, px[k], dt, front) && fst->has_color) color_write(*fst, px[k], colour);
+                        } else if (depth_stencil_ops(*fst, px[k], dt, front) && fst->has_color) {
+                            const uint32_t tag = gprim | ((uint32_t)k << 28);
+#pragma unroll
+                            for (int e = 0; e < Q; e++) {
+                                if (e == qn) { q_prim[e] = tag; q_w0[e] = w0; q_w1[e] = w1; q_w2[e] = w2; }
+                            }
+                            qn++;
                         }
                     }
-                    if (write && fst->has_color) color_write(*fst, px[k], colour);
                 }
             }
         }
+        if (!FS::kMayDiscard) drain();
+        (void)env;
 
         // ---- write back through LDS, rows stored with dword accesses
         for (int att = 0; att < 2; att++) {

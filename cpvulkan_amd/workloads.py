@@ -68,6 +68,8 @@ class Scene:
     clear_stencil: int = 0
     initial_color: np.ndarray | None = None
     initial_ds: np.ndarray | None = None
+    pipelines: list | None = None        # several pipelines (then draw_pipeline picks one per draw)
+    draw_pipeline: list | None = None
 
     @property
     def color_bytes(self) -> int:
@@ -183,6 +185,7 @@ class Materialized:
     ds_addr: int
     scene: Scene
     mem: object
+    draw_pipeline: list | None = None
 
     def read_color(self) -> np.ndarray:
         return self.mem.download(self.color_addr, self.scene.color_bytes) if self.color_addr else np.zeros(0, np.uint8)
@@ -218,7 +221,8 @@ def _sampler(s: dict) -> abi.SamplerDesc:
 
 def materialize(scene: Scene, mem) -> Materialized:
     """Copy the scene's buffers into `mem` and build the C-ABI descriptors."""
-    pd = pipeline_desc(dict(scene.pipeline, color_format=scene.color_format, ds_format=scene.ds_format))
+    pds = [pipeline_desc(dict(p, color_format=scene.color_format, ds_format=scene.ds_format))
+           for p in (scene.pipelines or [scene.pipeline])]
     ps = abi.PassDesc()
     ps.width, ps.height = scene.width, scene.height
     color_addr = ds_addr = 0
@@ -273,7 +277,7 @@ def materialize(scene: Scene, mem) -> Materialized:
             td.level_count = t.level_count or (t.levels - t.base_level)
             td.sampler = _sampler(t.sampler)
         draws.append(dd)
-    m = Materialized(ps, [pd], draws, color_addr, ds_addr, scene, mem)
+    m = Materialized(ps, pds, draws, color_addr, ds_addr, scene, mem, scene.draw_pipeline)
     m.reset_attachments()
     return m
 
@@ -399,8 +403,9 @@ def soup_triangles(n: int, width: int, height: int, seed: int, centres: np.ndarr
 
     Centres uniform in NDC (or given), edge length L ~ LogNormal(mu, sigma)
     pixels, vertices c + (L/sqrt(3)) (cos t_k, sin t_k) (2/W, 2/H) with
-    t_k = t_0 + 2 pi k / 3 + U(-0.3, 0.3); z ~ U[0,1]; w = 1; rgb ~ U[0,1],
-    a ~ U[0.2, 1] per vertex."""
+    t_k = t_0 + 2 pi k / 3 + U(-0.3, 0.3), vertices 1 and 2 swapped for a
+    random half of the triangles (both windings); z ~ U[0,1]; w = 1;
+    rgb ~ U[0,1], a ~ U[0.2, 1] per vertex."""
     rng = np.random.Generator(np.random.PCG64(seed))
     c = rng.uniform(-1.0, 1.0, size=(n, 2)) if centres is None else centres
     L = rng.lognormal(mu_px, sigma, size=n)
@@ -410,6 +415,10 @@ def soup_triangles(n: int, width: int, height: int, seed: int, centres: np.ndarr
     r = (L / math.sqrt(3.0))[:, None]
     x = c[:, 0:1] + r * np.cos(ang) * (2.0 / width)
     y = c[:, 1:2] + r * np.sin(ang) * (2.0 / height)
+    # random winding: half the triangles face each way
+    flip = rng.random(n) < 0.5
+    x[flip, 1], x[flip, 2] = x[flip, 2].copy(), x[flip, 1].copy()
+    y[flip, 1], y[flip, 2] = y[flip, 2].copy(), y[flip, 1].copy()
     z = rng.uniform(0.0, 1.0, size=(n, 3))
     rgb = rng.uniform(0.0, 1.0, size=(n, 3, 3))
     a = rng.uniform(0.2, 1.0, size=(n, 3))

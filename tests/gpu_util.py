@@ -12,7 +12,7 @@ def render_gpu(ctx, scene: Scene):
     mem = DeviceMemory(ctx)
     try:
         m = materialize(scene, mem)
-        ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
+        ctx.render_pass(m.pass_desc, m.pipelines, m.draws, m.draw_pipeline)
         ctx.synchronize()
         frags = ctx.fragment_count()
         return m.read_color(), m.read_ds(), frags

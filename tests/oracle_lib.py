@@ -61,7 +61,7 @@ def render(m: Materialized, prim_range: tuple[int, int] | None = None) -> int:
     n = len(m.draws)
     parr = (abi.PipelineDesc * len(m.pipelines))(*m.pipelines)
     darr = (abi.DrawDesc * max(n, 1))(*m.draws)
-    dp = (C.c_uint32 * max(n, 1))(*([0] * n))
+    dp = (C.c_uint32 * max(n, 1))(*(m.draw_pipeline or [0] * n))
     frags = C.c_uint64()
     if prim_range is None:
         rc = L.cpvref_render_pass(C.byref(m.pass_desc), parr, darr, dp, n, C.byref(frags))
