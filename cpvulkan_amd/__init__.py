@@ -33,6 +33,7 @@ def _declare(lib: C.CDLL) -> None:
     lib.cpv_upload.argtypes = [P, C.c_uint64, P, C.c_uint64]
     lib.cpv_download.argtypes = [P, P, C.c_uint64, C.c_uint64]
     lib.cpv_synchronize.argtypes = [P]
+    lib.cpv_device_copy.argtypes = [P, C.c_uint64, C.c_uint64, C.c_uint64]
     lib.cpv_set_config.argtypes = [P, C.POINTER(abi.Config)]
     lib.cpv_get_config.argtypes = [P, C.POINTER(abi.Config)]
     lib.cpv_render_pass.argtypes = [P, C.POINTER(abi.PassDesc), C.POINTER(abi.PipelineDesc), C.POINTER(abi.DrawDesc),
@@ -121,9 +122,13 @@ class Context:
     def synchronize(self) -> None:
         _check(self.lib, self.lib.cpv_synchronize(self.handle), "cpv_synchronize")
 
+    def device_copy(self, dst: int, src: int, nbytes: int) -> None:
+        _check(self.lib, self.lib.cpv_device_copy(self.handle, dst, src, nbytes), "cpv_device_copy")
+
     # configuration
-    def set_config(self, tile_size=16, waves_per_tile=4, scheduler=0, split_threshold=0, record_costs=0) -> None:
-        cfg = abi.Config(tile_size, waves_per_tile, scheduler, split_threshold, record_costs)
+    def set_config(self, tile_size=16, waves_per_tile=4, scheduler=0, split_threshold=0, record_costs=0,
+                   count_fragments=1) -> None:
+        cfg = abi.Config(tile_size, waves_per_tile, scheduler, split_threshold, record_costs, count_fragments)
         _check(self.lib, self.lib.cpv_set_config(self.handle, C.byref(cfg)), "cpv_set_config")
 
     # rendering

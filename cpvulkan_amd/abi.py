@@ -132,7 +132,7 @@ class JobDesc(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("tile_size", u32), ("waves_per_tile", u32), ("scheduler", u32), ("split_threshold", u32),
-                ("record_costs", u32), ("reserved", u32 * 3)]
+                ("record_costs", u32), ("count_fragments", u32), ("reserved", u32 * 2)]
 
 
 class TileTrace(C.Structure):

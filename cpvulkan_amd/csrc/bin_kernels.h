@@ -35,6 +35,48 @@ __device__ __forceinline__ void put_edge(TriSetup& r, int k, const float* a, con
     r.dx[k] = b[0] - a[0];
 }
 
+// Interval holding the viewport-transformed depth dt of every pixel the
+// triangle covers, for exact hierarchical depth culling.
+//
+// A covered pixel has fp32 edge values e_i >= 0 and weights w_i = e_i / A
+// (A the fp32 area).  Each e_i differs from the exact edge function of the
+// same fp32 inputs by at most 5u * BW * BH * 2 (u = 2^-24, BW x BH the
+// bounding box in NDC grown by the pixels the truncated box can add), the
+// exact edge functions sum to the exact area, and A differs from that by
+// at most the same bound.  So S = sum w_i lies in [1 - rho, 1 + rho] with
+// rho = 41u * kappa (+u), kappa = BW * BH / A; rho = 64u * kappa + 8u is used.
+// With w_i >= 0 the real sum of z_i * w_i lies between zmin * S and zmax * S,
+// and the fp32 evaluation adds at most 4u * max|z| * S.  The bounds are
+// widened by 16u * max|z| to absorb their own rounding, then mapped through
+// the monotone viewport transform.  Degenerate, huge or non-finite cases
+// get the whole real line.
+__device__ __forceinline__ void depth_bounds(TriSetup& r, const float P[3][4], const DrawDesc& d)
+{
+    const float inf = __builtin_inff();
+    r.dt_lo = -inf;
+    r.dt_hi = inf;
+    const float z0 = P[0][2], z1 = P[1][2], z2 = P[2][2];
+    const float minx = fminf(P[0][0], fminf(P[1][0], P[2][0])), maxx = fmaxf(P[0][0], fmaxf(P[1][0], P[2][0]));
+    const float miny = fminf(P[0][1], fminf(P[1][1], P[2][1])), maxy = fmaxf(P[0][1], fmaxf(P[1][1], P[2][1]));
+    const float bw = (maxx - minx) + 8.0f / d.vp_w, bh = (maxy - miny) + 8.0f / d.vp_h;
+    const float area = r.area;
+    if (!(area > 0.0f) || !__builtin_isfinite(area) || !__builtin_isfinite(bw * bh) || !__builtin_isfinite(z0) ||
+        !__builtin_isfinite(z1) || !__builtin_isfinite(z2) || fmaxf(fabsf(minx), fmaxf(fabsf(maxx), fmaxf(fabsf(miny), fabsf(maxy)))) > 1e15f)
+        return;
+    const float u = 0x1p-24f;
+    const float rho = 64.0f * u * (bw * bh / area) + 8.0f * u;
+    if (!(rho < 0.25f)) return;
+    const float zmin = fminf(z0, fminf(z1, z2)), zmax = fmaxf(z0, fmaxf(z1, z2));
+    const float zabs = fmaxf(fabsf(zmin), fabsf(zmax));
+    const float lo = (zmin >= 0.0f ? zmin * (1.0f - rho) : zmin * (1.0f + rho)) - 16.0f * u * zabs * (1.0f + rho) - 0x1p-120f;
+    const float hi = (zmax >= 0.0f ? zmax * (1.0f + rho) : zmax * (1.0f - rho)) + 16.0f * u * zabs * (1.0f + rho) + 0x1p-120f;
+    const float sc = d.vp_max_depth - d.vp_min_depth;
+    const float tl = sc * lo + d.vp_min_depth, th = sc * hi + d.vp_min_depth;
+    if (!__builtin_isfinite(tl) || !__builtin_isfinite(th)) return;
+    r.dt_lo = fminf(tl, th);
+    r.dt_hi = fmaxf(tl, th);
+}
+
 // Per-primitive part of ProcessTriangles (CommandBuffer.Draw.cpp:1526-1569)
 // and the per-triangle constants of GetFragmentInput (:879-898, :900 cull).
 __global__ __launch_bounds__(256) void setup_kernel(SetupArgs a)
@@ -120,6 +162,7 @@ __global__ __launch_bounds__(256) void setup_kernel(SetupArgs a)
     r.flags = front ? 1u : 0u;
     r.provoking = slot[3];
     r.draw = di;
+    depth_bounds(r, P, d);
     uint4* dst = (uint4*)&a.recs[gp];
     const uint4* src = (const uint4*)&r;
 #pragma unroll
@@ -218,17 +261,26 @@ struct EmitArgs {
     const TargetParams* targets;
     uint32_t* keys;               // global tile index per pair
     uint32_t* vals;               // primitive index per pair
+    uint32_t* big;                // primitives left to emit_big_kernel; big[0] is the count
     uint32_t prims;
     int32_t tile_shift;
 };
 
+constexpr uint32_t kEmitSmall = 32;   // pairs a single thread writes
+
 // Writes the (tile, primitive) pairs of every primitive at its scanned
-// offset, so the pair array is in primitive order.
+// offset, so the pair array is in primitive order.  Primitives with more
+// than kEmitSmall tiles are queued for emit_big_kernel, which spreads them
+// over a whole workgroup.
 __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
 {
     for (uint32_t gp = blockIdx.x * 256u + threadIdx.x; gp < a.prims; gp += gridDim.x * 256u) {
         const uint32_t c = a.counts[gp];
         if (c == 0) continue;
+        if (c > kEmitSmall) {
+            a.big[1 + atomicAdd(&a.big[0], 1u)] = gp;
+            continue;
+        }
         const TriSetup& r = a.recs[gp];
         const TargetParams& t = a.targets[a.draws[r.draw].target];
         const int sh = a.tile_shift;
@@ -242,6 +294,29 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
                 a.vals[off] = gp;
                 off++;
             }
+        }
+    }
+}
+
+// Pairs of the large primitives: one workgroup per primitive at a time,
+// its threads striding over the primitive's tile rectangle (row-major, the
+// order emit_kernel uses).
+__global__ __launch_bounds__(256) void emit_big_kernel(EmitArgs a)
+{
+    const uint32_t n = a.big[0];
+    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x) {
+        const uint32_t gp = a.big[1 + b];
+        const TriSetup& r = a.recs[gp];
+        const TargetParams& t = a.targets[a.draws[r.draw].target];
+        const int sh = a.tile_shift;
+        const uint32_t tx0 = (uint32_t)r.x0 >> sh, tx1 = (uint32_t)(r.x1 - 1) >> sh;
+        const uint32_t ty0 = (uint32_t)r.y0 >> sh;
+        const uint32_t w = tx1 - tx0 + 1;
+        const uint32_t c = a.counts[gp], off = a.offsets[gp];
+        for (uint32_t k = threadIdx.x; k < c; k += 256u) {
+            const uint32_t key = t.tile_base + (ty0 + k / w) * t.tiles_x + tx0 + k % w;
+            a.keys[off + k] = key;
+            a.vals[off + k] = gp;
         }
     }
 }

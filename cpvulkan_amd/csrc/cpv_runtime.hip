@@ -168,13 +168,14 @@ struct cpv_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_raster0 = nullptr, ev_raster1 = nullptr, ev_end = nullptr;
     cpv_config cfg{};
-    DevBuf vout, recs, counts, offsets, keys[2], vals[2], tile_start, hist, hist_base, sums;
+    DevBuf vout, recs, counts, offsets, keys[2], vals[2], tile_start, hist, hist_base, sums, big;
     DevBuf arena, stats, traces;
     void* pinned = nullptr;
     size_t pinned_cap = 0;
     uint32_t* readback = nullptr;     // pinned scalar readbacks
     uint32_t last_tiles = 0;
     bool timing_valid = false;
+    bool last_count_exact = false;
     float raster_ms_acc = 0.0f;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> raster_events;
     std::mutex lock;
@@ -192,6 +193,7 @@ extern "C" int cpv_context_create(int device_ordinal, cpv_context** out)
     c->cfg.tile_size = 16;
     c->cfg.waves_per_tile = 4;
     c->cfg.scheduler = CPV_SCHED_GRID;
+    c->cfg.count_fragments = 1;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
         hipEventCreate(&c->ev_raster0) != hipSuccess || hipEventCreate(&c->ev_raster1) != hipSuccess ||
@@ -210,7 +212,7 @@ extern "C" void cpv_context_destroy(cpv_context* c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->vout, &c->recs, &c->counts, &c->offsets, &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1],
-                      &c->tile_start, &c->hist, &c->hist_base, &c->sums, &c->arena, &c->stats, &c->traces};
+                      &c->tile_start, &c->big, &c->hist, &c->hist_base, &c->sums, &c->arena, &c->stats, &c->traces};
     for (DevBuf* b : bufs) b->release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->readback) (void)hipHostFree(c->readback);
@@ -257,6 +259,25 @@ extern "C" int cpv_download(cpv_context* c, void* dst, uint64_t src, uint64_t by
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipMemcpyAsync(dst, (const void*)(uintptr_t)src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return CPV_OK;
+}
+
+__global__ void copy_u32x4_kernel(uint4* dst, const uint4* src, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+extern "C" int cpv_device_copy(cpv_context* c, uint64_t dst, uint64_t src, uint64_t bytes)
+{
+    if (!c) return fail(CPV_ERROR_INVALID, "null context");
+    if ((dst | src | bytes) & 15u) return fail(CPV_ERROR_INVALID, "device copy needs 16-byte aligned addresses and size");
+    HIP_TRY(hipSetDevice(c->device));
+    const uint64_t n = bytes / 16;
+    if (n) {
+        hipLaunchKernelGGL(copy_u32x4_kernel, dim3((uint32_t)std::min<uint64_t>((n + 255) / 256, 65536)), dim3(256), 0, c->stream,
+                           (uint4*)(uintptr_t)dst, (const uint4*)(uintptr_t)src, n);
+        HIP_TRY(hipGetLastError());
+    }
     return CPV_OK;
 }
 
@@ -735,10 +756,13 @@ static int run_launch(cpv_context* c, const Program& prog, const std::vector<Tar
         return CPV_ERROR_OUT_OF_MEMORY;
     // 4. emit pairs in primitive order
     if (prim_total) {
+        if (c->big.ensure(((size_t)prim_total + 1) * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+        fill_u32(c, c->big.p, 0u, 1);
         EmitArgs ea{c->recs.as<TriSetup>(), c->counts.as<uint32_t>(), c->offsets.as<uint32_t>(), d_dd, d_tp,
-                    c->keys[0].as<uint32_t>(), c->vals[0].as<uint32_t>(), prim_total, tile_shift};
+                    c->keys[0].as<uint32_t>(), c->vals[0].as<uint32_t>(), c->big.as<uint32_t>(), prim_total, tile_shift};
         const uint32_t blocks = std::min<uint32_t>((prim_total + 255) / 256, 8192);
         hipLaunchKernelGGL(emit_kernel, dim3(blocks), dim3(256), 0, c->stream, ea);
+        hipLaunchKernelGGL(emit_big_kernel, dim3(1024), dim3(256), 0, c->stream, ea);
         HIP_TRY(hipGetLastError());
     }
     // 5. stable radix sort of the pairs by tile
@@ -783,8 +807,27 @@ static int run_launch(cpv_context* c, const Program& prog, const std::vector<Tar
     ra.tiles = tiles;
     ra.target_count = (uint32_t)targets.size();
     ra.persistent = 0;
+    // Hierarchical depth culling is exact when only the depth test observes
+    // a fragment (no stencil, no depth bounds), every draw uses the same
+    // compare op and depth format, and nobody counts covered fragments.
+    ra.hiz_op = -1;
+    ra.hiz_depth_kind = 32;
+    if (!c->cfg.count_fragments && !fstates.empty()) {
+        bool ok = true;
+        for (const FragmentState& f : fstates) {
+            ok = ok && f.depth_test && f.has_ds && f.ds.depth_off >= 0 && f.ds.vk_format != CPV_FORMAT_S8_UINT &&
+                 !f.depth_bounds_test && !f.stencil_test && f.depth_compare == fstates[0].depth_compare &&
+                 f.ds.depth_kind == fstates[0].ds.depth_kind;
+        }
+        for (const TargetParams& t : targets) ok = ok && t.has_ds && t.ds_fmt.depth_kind == fstates[0].ds.depth_kind;
+        if (ok) {
+            ra.hiz_op = fstates[0].depth_compare;
+            ra.hiz_depth_kind = fstates[0].ds.depth_kind;
+        }
+    }
+    c->last_count_exact = c->cfg.count_fragments != 0;
     if (c->cfg.record_costs) {
-        if (c->traces.ensure((size_t)std::max(tiles, 1u) * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+        if (c->traces.ensure((size_t)std::max(tiles, 1u) * c->cfg.waves_per_tile * 4)) return CPV_ERROR_OUT_OF_MEMORY;
         ra.trace_cycles = c->traces.as<float>();
     }
     if (record_timing) HIP_TRY(hipEventRecord(c->ev_raster0, c->stream));
@@ -896,6 +939,7 @@ extern "C" int cpv_render_jobs(cpv_context* c, const cpv_job_desc* jobs, uint32_
 extern "C" int cpv_last_fragment_count(cpv_context* c, uint64_t* fragments)
 {
     if (!c || !fragments) return fail(CPV_ERROR_INVALID, "null argument");
+    if (!c->last_count_exact) return fail(CPV_ERROR_INVALID, "the last render call ran with count_fragments off");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipMemcpyAsync(c->readback + 16, c->stats.p, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -48,17 +48,18 @@ struct FormatDesc {
 // weight order and orientation already follow GetFragmentInput's sign
 // branch (:879-898), so the raster kernel never re-derives them.
 struct TriSetup {
-    float ax[3], ay[3];
-    float dy[3], dx[3];
-    float area;          // |EdgeFunction(p0, p1, p2)|
-    float z[3];          // p_k.z / p_k.w
-    float w[3];          // clip-space w of p_k
-    int32_t x0, y0, x1, y1;   // pixel bounding box, [x0, x1) x [y0, y1)
-    uint32_t v[3];       // vertex-output indices of p0, p1, p2 (after the CW swap)
-    uint32_t flags;      // bit 0: front facing
-    uint32_t provoking;  // vertex-output index of the provoking vertex
-    uint32_t draw;       // draw index within the launch
-    uint32_t pad[3];
+    float ax[3], ay[3];       // bytes 0-23
+    float dy[3], dx[3];       // 24-47
+    int32_t x0, y0, x1, y1;   // 48-63: pixel bounding box, [x0, x1) x [y0, y1)
+    float dt_lo, dt_hi;       // 64-71: bounds of the viewport-transformed depth of every covered pixel
+    float area;               // 72: |EdgeFunction(p0, p1, p2)|
+    uint32_t draw;            // 76: draw index within the launch
+    float z[3];               // 80-91: p_k.z / p_k.w
+    float w[3];               // 92-103: clip-space w of p_k
+    uint32_t v[3];            // 104-115: vertex-output indices of p0, p1, p2 (after the CW swap)
+    uint32_t flags;           // bit 0: front facing
+    uint32_t provoking;       // vertex-output index of the provoking vertex
+    uint32_t pad;
 };
 static_assert(sizeof(TriSetup) == 128, "TriSetup must be 128 bytes");
 

@@ -1,0 +1,340 @@
+// Fine raster / fragment kernel.
+//
+// Work unit: one wavefront owns BPW 8x8 pixel blocks of one tile and walks
+// that tile's primitive list in submission order.  Wavefronts never
+// synchronise with each other: the framebuffer pixels live in registers for
+// the whole tile (loaded or cleared at the start, stored at the end), the
+// primitive records are read with scalar loads (one record serves all 64
+// lanes), and the per-chunk filter reads bounding boxes with vector gathers.
+//
+// Per primitive and pixel the reference's order of operations is kept
+// (CPVulkan/CommandBuffer.Draw.cpp:874-954, 1300-1313 and
+// LLVMRuntime/PipelineCompiler.cpp:1020-1085); the shortcuts taken are
+// exact: hierarchical depth culling and the per-fragment early depth
+// rejection use proven error bounds, and the colour work of passing
+// fragments is queued and drained in order per pixel.
+#pragma once
+#pragma clang fp contract(off)
+
+namespace cpv {
+
+// A wave-uniform record through the constant address space, which lets the
+// compiler use scalar loads (the records are read-only during the kernel).
+__device__ __forceinline__ void load_record_uniform(const TriSetup* recs, uint32_t gp, TriSetup& r)
+{
+    uint32_t* dst = (uint32_t*)&r;
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef const __attribute__((address_space(4))) uint32_t* ScalarU32Ptr;
+    ScalarU32Ptr p = (ScalarU32Ptr)(recs + gp);
+#else
+    const uint32_t* p = (const uint32_t*)(recs + gp);
+#endif
+#pragma unroll
+    for (int q = 0; q < 32; q++) dst[q] = p[q];
+}
+
+__device__ __forceinline__ void load_pixel(const uint8_t* p, int bytes, uint32_t w[4])
+{
+    w[0] = w[1] = w[2] = w[3] = 0u;
+    if (bytes == 4) {
+        w[0] = *(const uint32_t*)p;
+    } else if (bytes == 2) {
+        w[0] = *(const uint16_t*)p;
+    } else if (bytes == 8) {
+        const uint2 v = *(const uint2*)p;
+        w[0] = v.x; w[1] = v.y;
+    } else if (bytes == 16) {
+        const uint4 v = *(const uint4*)p;
+        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+    } else {
+        for (int b = 0; b < bytes; b++) w[b >> 2] |= (uint32_t)p[b] << ((b & 3) * 8);
+    }
+}
+
+__device__ __forceinline__ void store_pixel(uint8_t* p, int bytes, const uint32_t w[4])
+{
+    if (bytes == 4) {
+        *(uint32_t*)p = w[0];
+    } else if (bytes == 2) {
+        *(uint16_t*)p = (uint16_t)w[0];
+    } else if (bytes == 8) {
+        *(uint2*)p = make_uint2(w[0], w[1]);
+    } else if (bytes == 16) {
+        *(uint4*)p = make_uint4(w[0], w[1], w[2], w[3]);
+    } else {
+        for (int b = 0; b < bytes; b++) p[b] = (uint8_t)(w[b >> 2] >> ((b & 3) * 8));
+    }
+}
+
+template <class FS, int TILE, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void raster_kernel(RasterArgs a)
+{
+    constexpr int BW = TILE / 8;               // 8x8 blocks per tile row
+    constexpr int BLOCKS = BW * BW;
+    constexpr int BPW = BLOCKS / WAVES;        // blocks (pixels) per lane
+    static_assert(BLOCKS % WAVES == 0, "tile blocks must divide among the waves");
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const uint32_t units = a.tiles * (uint32_t)WAVES;
+    uint32_t unit = blockIdx.x * WAVES + wave;
+
+    for (;;) {
+        if (a.persistent) {
+            uint32_t u = 0;
+            if (lane == 0) u = atomicAdd(a.queue, 1u);
+            unit = __builtin_amdgcn_readfirstlane(u);
+        }
+        if (unit >= units) break;
+        const uint32_t item = unit / WAVES;
+        const int slot = (int)(unit % WAVES);
+        const uint32_t tile = a.order ? a.order[item] : item;
+        const uint64_t t_begin = clock64();
+
+        const uint32_t ti = find_target(a.targets, a.target_count, tile);
+        const TargetParams& tp = a.targets[ti];
+        const uint32_t local = tile - tp.tile_base;
+        const int tile_x0 = (int)(local % tp.tiles_x) * TILE;
+        const int tile_y0 = (int)(local / tp.tiles_x) * TILE;
+
+        // ---- pixel state: loadOp CLEAR packs the clear value as ClearImage ->
+        // SetPixel does (CommandBuffer.Draw.cpp:117-149, ImageSampler.cpp:723-774);
+        // LOAD / DONT_CARE read the attachment.
+        PixelState clear;
+        clear.color[0] = clear.color[1] = clear.color[2] = clear.color[3] = 0u;
+        clear.ds0 = clear.ds1 = 0u;
+        if (tp.has_color && tp.color_load == 1) {
+            if (tp.color_fmt.base == kBaseUInt) {
+                pack_color_u32(tp.color_fmt, tp.clear_color, clear.color);
+            } else if (tp.color_fmt.base == kBaseSInt) {
+                const int32_t ci[4] = {(int32_t)tp.clear_color[0], (int32_t)tp.clear_color[1], (int32_t)tp.clear_color[2],
+                                       (int32_t)tp.clear_color[3]};
+                pack_color_i32(tp.color_fmt, ci, clear.color);
+            } else {
+                const float cf[4] = {__uint_as_float(tp.clear_color[0]), __uint_as_float(tp.clear_color[1]),
+                                     __uint_as_float(tp.clear_color[2]), __uint_as_float(tp.clear_color[3])};
+                pack_color_f32(tp.color_fmt, cf, clear.color);
+            }
+        }
+        if (tp.has_ds && tp.ds_load == 1)
+            pack_depth_stencil(tp.ds_fmt, __uint_as_float(tp.clear_ds[0]), tp.clear_ds[1], clear.ds0, clear.ds1);
+
+        PixelState px[BPW];
+        int pxx[BPW], pxy[BPW];
+        bool inside[BPW];
+        int wx0 = 1 << 30, wy0 = 1 << 30, wx1 = -(1 << 30), wy1 = -(1 << 30);
+#pragma unroll
+        for (int k = 0; k < BPW; k++) {
+            const int blk = slot + k * WAVES;
+            const int bx0 = tile_x0 + (blk % BW) * 8, by0 = tile_y0 + (blk / BW) * 8;
+            pxx[k] = bx0 + (lane & 7);
+            pxy[k] = by0 + (lane >> 3);
+            inside[k] = pxx[k] < (int)tp.width && pxy[k] < (int)tp.height;
+            wx0 = min(wx0, bx0); wy0 = min(wy0, by0); wx1 = max(wx1, bx0 + 8); wy1 = max(wy1, by0 + 8);
+            px[k] = clear;
+            const uint64_t pix = (uint64_t)pxy[k] * tp.width + (uint64_t)pxx[k];
+            if (inside[k] && tp.has_color && tp.color_load != 1)
+                load_pixel((const uint8_t*)tp.color_addr + pix * tp.color_bytes, tp.color_bytes, px[k].color);
+            if (inside[k] && tp.has_ds && tp.ds_load != 1) {
+                uint32_t w4[4];
+                load_pixel((const uint8_t*)tp.ds_addr + pix * tp.ds_bytes, tp.ds_bytes, w4);
+                px[k].ds0 = w4[0];
+                px[k].ds1 = w4[1];
+            }
+        }
+
+        // ---- per-draw state, refreshed when the draw changes
+        uint32_t cur_draw = 0xFFFFFFFFu;
+        float xf[BPW], yf[BPW];
+        float vp_min = 0.0f, vp_max = 1.0f;
+        bool vp_identity = true;
+        const FragmentState* fst = nullptr;
+        const DrawEnv* env = nullptr;
+        bool early = false;   // conservative depth rejection allowed for this draw
+        int32_t depth_op = 7;
+        int32_t depth_kind = 0;
+        unsigned long long nfrag = 0;
+
+        // Deferred colour work: a fragment that passed the depth/stencil
+        // tests has its depth/stencil written at once, and its colour
+        // (interpolation, fragment shader, blend, pack) queued.  Queues drain
+        // in order, so every pixel still sees its colour writes in primitive
+        // order; draining for all lanes at once keeps the shading path from
+        // running once per primitive whenever any lane passes.
+        constexpr int Q = FS::kMayDiscard ? 1 : 4;
+        uint32_t q_prim[Q];
+        float q_w0[Q], q_w1[Q], q_w2[Q];
+        int qn = 0;
+
+        auto shade = [&](uint32_t gprim, int k, float w0, float w1, float w2) {
+            PixelState ps = px[0];
+            int sx = pxx[0], sy = pxy[0];
+#pragma unroll
+            for (int kk = 1; kk < BPW; kk++)
+                if (kk == k) { ps = px[kk]; sx = pxx[kk]; sy = pxy[kk]; }
+            ps = shade_fragment<FS>(a.recs, a.draws, a.fstates, a.vout, a.vout_stride, gprim, w0, w1, w2, sx, sy, ps);
+#pragma unroll
+            for (int kk = 0; kk < BPW; kk++)
+                if (kk == k) px[kk] = ps;
+        };
+        auto drain = [&]() {
+#pragma unroll
+            for (int e = 0; e < Q; e++) {
+                if (!__any(e < qn)) break;
+                if (e < qn) shade(q_prim[e] & 0x0FFFFFFFu, (int)(q_prim[e] >> 28), q_w0[e], q_w1[e], q_w2[e]);
+            }
+            qn = 0;
+        };
+
+        // ---- walk the tile's primitives in submission order
+        const uint32_t lbeg = a.tile_start[tile], lend = a.tile_start[tile + 1];
+        for (uint32_t base = lbeg; base < lend; base += 64) {
+            const int n = (int)min(64u, lend - base);
+            // Depth range stored in this wave's pixels, for hierarchical
+            // culling: a primitive whose depth interval fails the test
+            // against all of them cannot change any of them.
+            float bmin = __builtin_inff(), bmax = -__builtin_inff();
+            if (a.hiz_op >= 0) {
+#pragma unroll
+                for (int k = 0; k < BPW; k++) {
+                    if (inside[k]) {
+                        const float cd = stored_depth(a.hiz_depth_kind, px[k].ds0);
+                        bmin = fminf(bmin, cd);
+                        bmax = fmaxf(bmax, cd);
+                    }
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    bmin = fminf(bmin, __shfl_xor(bmin, off, 64));
+                    bmax = fmaxf(bmax, __shfl_xor(bmax, off, 64));
+                }
+            }
+            // each lane tests one primitive of the chunk against the wave's area
+            uint32_t gp_l = 0;
+            bool ov = false;
+            if (lane < n) {
+                gp_l = a.tile_prims[base + lane];
+                const int4 bb = *(const int4*)&a.recs[gp_l].x0;
+                ov = !(bb.z <= wx0 || bb.x >= wx1 || bb.w <= wy0 || bb.y >= wy1);
+                if (ov && a.hiz_op >= 0) {
+                    const float2 dz = *(const float2*)&a.recs[gp_l].dt_lo;
+                    ov = !hiz_rejects(a.hiz_op, dz.x, dz.y, bmin, bmax);
+                }
+            }
+            uint64_t mask = __ballot(ov);
+            while (mask) {
+                const int j = (int)__builtin_ctzll(mask);
+                mask &= mask - 1;
+                const uint32_t gprim = __builtin_amdgcn_readlane(gp_l, j);
+                TriSetup r;
+                load_record_uniform(a.recs, gprim, r);
+                if (r.draw != cur_draw) {
+                    cur_draw = r.draw;
+                    const DrawDesc& d = a.draws[cur_draw];
+                    // pixel centres (ProcessTriangles :1571-1580)
+#pragma unroll
+                    for (int k = 0; k < BPW; k++) {
+                        xf[k] = ((float)pxx[k] / d.vp_w + d.half_x) * 2.0f - 1.0f;
+                        yf[k] = ((float)pxy[k] / d.vp_h + d.half_y) * 2.0f - 1.0f;
+                    }
+                    vp_min = d.vp_min_depth;
+                    vp_max = d.vp_max_depth;
+                    vp_identity = vp_min == 0.0f && vp_max == 1.0f;
+                    fst = &a.fstates[d.fstate];
+                    env = (const DrawEnv*)d.env;
+                    // Rejecting on the depth test alone is exact when nothing
+                    // else observes the fragment: no depth-bounds or stencil
+                    // test (and so no stencil write).
+                    early = fst->depth_test && fst->has_ds && fst->ds.depth_off >= 0 && fst->ds.vk_format != 127u &&
+                            !fst->depth_bounds_test && !fst->stencil_test;
+                    depth_op = fst->depth_compare;
+                    depth_kind = fst->ds.depth_kind;
+                }
+#pragma unroll
+                for (int k = 0; k < BPW; k++) {
+                    // every lane is active here; a lane queues at most one
+                    // entry per iteration, so a free slot always exists
+                    if (!FS::kMayDiscard && __any(qn == Q)) drain();
+                    const int blk = slot + k * WAVES;
+                    const int bx0 = tile_x0 + (blk % BW) * 8, by0 = tile_y0 + (blk / BW) * 8;
+                    if (BPW > 1 && (r.x1 <= bx0 || r.x0 >= bx0 + 8 || r.y1 <= by0 || r.y0 >= by0 + 8)) continue;   // wave-uniform
+                    const int x = pxx[k], y = pxy[k];
+                    if (x < r.x0 || x >= r.x1 || y < r.y0 || y >= r.y1) continue;
+                    if (a.hiz_op >= 0) {
+                        const float cd = stored_depth(a.hiz_depth_kind, px[k].ds0);
+                        if (hiz_rejects(a.hiz_op, r.dt_lo, r.dt_hi, cd, cd)) continue;
+                    }
+                    // GetFragmentInput (:874-903); NaN edge values are not < 0 and pass
+                    const float e0 = (xf[k] - r.ax[0]) * r.dy[0] - (yf[k] - r.ay[0]) * r.dx[0];
+                    const float e1 = (xf[k] - r.ax[1]) * r.dy[1] - (yf[k] - r.ay[1]) * r.dx[1];
+                    const float e2 = (xf[k] - r.ax[2]) * r.dy[2] - (yf[k] - r.ay[2]) * r.dx[2];
+                    if (e0 < 0 || e1 < 0 || e2 < 0) continue;
+                    nfrag++;
+                    if (early) {
+                        const float cur = depth_kind == 32 ? __uint_as_float(px[k].ds0) : unpack_depth(fst->ds, px[k].ds0);
+                        if (vp_identity ? depth_surely_fails_unit(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2])
+                                        : depth_surely_fails(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2], vp_min, vp_max))
+                            continue;
+                    }
+                    const float w0 = e0 / r.area, w1 = e1 / r.area, w2 = e2 / r.area;
+                    const float depth = r.z[0] * w0 + r.z[1] * w1 + r.z[2] * w2;
+                    // DrawPixel (:1300-1313)
+                    const float dt = (vp_max - vp_min) * depth + vp_min;
+                    const bool front = (r.flags & 1u) != 0;
+                    if (FS::kMayDiscard) {
+                        // the shader decides first whether anything is written
+                        const float wv[3] = {w0, w1, w2};
+                        const float pw[3] = {r.w[0], r.w[1], r.w[2]};
+                        const float* q0 = a.vout + (uint64_t)r.v[0] * a.vout_stride + 4;
+                        const float* q1 = a.vout + (uint64_t)r.v[1] * a.vout_stride + 4;
+                        const float* q2 = a.vout + (uint64_t)r.v[2] * a.vout_stride + 4;
+                        const float* qp = a.vout + (uint64_t)r.provoking * a.vout_stride + 4;
+                        float in[FS::kInputs > 0 ? FS::kInputs : 1];
+#pragma unroll
+                        for (int c = 0; c < FS::kInputs; c++) in[c] = interpolate(FS::interp(c), wv, pw, q0[c], q1[c], q2[c], qp[c]);
+                        const float fc[4] = {fst->origin_upper_left ? (float)x : a.draws[cur_draw].vp_w - (float)x - 1.0f, (float)y,
+                                             depth, 1.0f};
+                        float colour[4];
+                        if (FS::run(*env, in, fc, front, colour)) continue;
+                        const DepthStencilResult dr = depth_stencil_call(fst, px[k], dt, front ? 1u : 0u);
+                        px[k] = dr.px;
+                        if (dr.write && fst->has_color) color_write(*fst, px[k], colour);
+                    } else {
+                        const DepthStencilResult dr = depth_stencil_call(fst, px[k], dt, front ? 1u : 0u);
+                        px[k] = dr.px;
+                        if (!(dr.write && fst->has_color)) continue;
+                        const uint32_t tag = gprim | ((uint32_t)k << 28);
+#pragma unroll
+                        for (int e = 0; e < Q; e++) {
+                            if (e == qn) { q_prim[e] = tag; q_w0[e] = w0; q_w1[e] = w1; q_w2[e] = w2; }
+                        }
+                        qn++;
+                    }
+                }
+            }
+        }
+        if (!FS::kMayDiscard) drain();
+        (void)env;
+
+        // ---- write back
+#pragma unroll
+        for (int k = 0; k < BPW; k++) {
+            if (!inside[k]) continue;
+            const uint64_t pix = (uint64_t)pxy[k] * tp.width + (uint64_t)pxx[k];
+            if (tp.has_color) store_pixel((uint8_t*)tp.color_addr + pix * tp.color_bytes, tp.color_bytes, px[k].color);
+            if (tp.has_ds) {
+                const uint32_t w4[4] = {px[k].ds0, px[k].ds1, 0u, 0u};
+                store_pixel((uint8_t*)tp.ds_addr + pix * tp.ds_bytes, tp.ds_bytes, w4);
+            }
+        }
+
+        // ---- statistics
+        for (int off = 32; off > 0; off >>= 1) nfrag += __shfl_down(nfrag, off, 64);
+        if (lane == 0 && nfrag) atomicAdd(a.fragments, nfrag);
+        if (a.trace_cycles && lane == 0) a.trace_cycles[unit] = (float)(clock64() - t_begin);
+
+        if (!a.persistent) break;
+    }
+}
+
+}  // namespace cpv

@@ -205,7 +205,10 @@ typedef struct cpv_config {
     uint32_t scheduler;             /* CPV_SCHED_* */
     uint32_t split_threshold;       /* tile list length above which a tile is split into chunks; 0 = never */
     uint32_t record_costs;          /* record per-tile cost traces (cpv_cost_trace) */
-    uint32_t reserved[3];
+    uint32_t count_fragments;       /* count covered fragments exactly (cpv_last_fragment_count); this
+                                       evaluates coverage for every pixel and so disables the exact
+                                       hierarchical depth culling */
+    uint32_t reserved[2];
 } cpv_config;
 
 /* Per-tile record of one rendered tile, written when cpv_config.record_costs
@@ -226,6 +229,10 @@ int cpv_free(cpv_context* ctx, uint64_t address);
 int cpv_upload(cpv_context* ctx, uint64_t dst, const void* src, uint64_t bytes);
 int cpv_download(cpv_context* ctx, void* dst, uint64_t src, uint64_t bytes);
 int cpv_synchronize(cpv_context* ctx);
+/* Device-to-device copy on the context's stream (16-byte aligned addresses
+ * and size); the transfer half of vkCmdCopyBuffer
+ * (CPVulkan/CommandBuffer.Copy.cpp:15-60) for GPU-resident buffers. */
+int cpv_device_copy(cpv_context* ctx, uint64_t dst, uint64_t src, uint64_t bytes);
 int cpv_set_config(cpv_context* ctx, const cpv_config* config);
 int cpv_get_config(cpv_context* ctx, cpv_config* config);
 

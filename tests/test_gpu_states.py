@@ -173,3 +173,44 @@ def test_empty_pass_clears(gpu_ctx):
     ref = oracle_lib.render_scene(s)
     gpu = render_gpu(gpu_ctx, s)
     compare(s, gpu, ref, "clear only")
+
+
+# ---- hierarchical depth culling (count_fragments off): images stay bit-exact
+
+def _run_hiz(ctx, scene, label):
+    print(f"[hiz] {label}", flush=True)
+    ref = oracle_lib.render_scene(scene)
+    gpu = render_gpu(ctx, scene, count=False)
+    compare(scene, gpu, ref, label + " (hierarchical depth culling)")
+
+
+@pytest.mark.parametrize("op", list(CMP))
+def test_hiz_compare_ops(gpu_ctx, op):
+    s = S.with_pipeline(S.soup(4000, 160, 120, seed=21), depth_compare=CMP[op])
+    _run_hiz(gpu_ctx, S.with_formats(s, clear_depth=0.5), f"op {op}")
+
+
+@pytest.mark.parametrize("fmt", ["D16_UNORM", "X8_D24_UNORM_PACK32", "D32_SFLOAT"])
+def test_hiz_formats(gpu_ctx, fmt):
+    _run_hiz(gpu_ctx, S.with_formats(S.soup(4000, 160, 120, seed=22), ds=F[fmt]), f"format {fmt}")
+
+
+@pytest.mark.parametrize("vp", [(0.0, 0.0, 160.0, 120.0, 0.25, 0.75), (0.0, 0.0, 160.0, 120.0, 0.9, 0.1)])
+def test_hiz_viewport_depth(gpu_ctx, vp):
+    _run_hiz(gpu_ctx, S.with_viewport(S.soup(4000, 160, 120, seed=23), vp), f"viewport {vp}")
+
+
+def test_hiz_perspective_and_sizes(gpu_ctx):
+    _run_hiz(gpu_ctx, S.soup(4000, 160, 120, seed=24, w_range=(0.5, 2.5)), "perspective w")
+    _run_hiz(gpu_ctx, S.soup(600, 160, 120, seed=25, size_px=4.0, offscreen=0.6), "big + offscreen")
+    _run_hiz(gpu_ctx, S.soup(8000, 160, 120, seed=26, size_px=0.0), "tiny")
+
+
+def test_hiz_multi_draw_and_load(gpu_ctx):
+    _run_hiz(gpu_ctx, S.multi_draw(seed=27), "multi draw")
+    _run_hiz(gpu_ctx, S.load_op(S.soup(3000, 160, 120, seed=28), seed=2), "load op")
+
+
+def test_hiz_deep_1080p(gpu_ctx):
+    from cpvulkan_amd import workloads as W
+    _run_hiz(gpu_ctx, W.soup_scene(200000, 1920, 1080, seed=29), "soup 1080p 200k")
