@@ -1,6 +1,6 @@
 """Draw-path benchmark: frames/s of one workload on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload soup|cube|skewed]
+    python bench.py [--steps K] [--warmup W] [--workload soup|cube|grid|skewed]
                     [--cpu-baseline full|sample|none] [--no-check]
 
 A step is one frame: the render pass of the workload (clear, vertex stage,
@@ -38,9 +38,16 @@ def make_scene(name: str):
         return W.soup_scene(1_000_000, 1920, 1080, seed=2), "soup-1080p-1M-blend"
     if name == "cube":
         return W.cube_scene(), "cube-500x500"
+    if name == "grid":
+        return W.grid_scene(512, 3840, 2160, seed=3), "grid-4k-512x512-textured"
     if name == "skewed":
         return W.skewed_scene(4_000_000, 3840, 2160, seed=5), "skewed-4k-4M-blend"
     raise SystemExit(f"unknown workload {name}")
+
+
+def fmt_name(code: int) -> str:
+    from cpvulkan_amd import abi
+    return next((k for k, v in abi.FORMAT.items() if v == code), str(code))
 
 
 def cpu_model() -> str:
@@ -168,7 +175,8 @@ def main() -> int:
         ms_per_step=round(ms_per_step, 4), higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f32", data="synthetic",
         config=dict(workload=workload, width=scene.width, height=scene.height, triangles=scene.primitive_count,
-                    tile=args.tile, waves_per_tile=args.waves, color="R8G8B8A8_UNORM", depth="D32_SFLOAT"),
+                    tile=args.tile, waves_per_tile=args.waves, color=fmt_name(scene.color_format),
+                    depth=fmt_name(scene.ds_format)),
         fragments_per_frame=fragments, fragments_per_s=round(fragments * fps, 1),
         raster_ms=round(raster_avg, 4), frame_gpu_ms=round(float(np.mean(total_ms)), 4),
         roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",

@@ -465,3 +465,114 @@ def skewed_scene(ntris: int = 4_000_000, width: int = 3840, height: int = 2160, 
                     viewport=(0.0, 0.0, float(width), float(height), 0.0, 1.0))
     return Scene(f"skewed{ntris}", width, height, abi.FORMAT["R8G8B8A8_UNORM"], abi.FORMAT["D32_SFLOAT"],
                  soup_pipeline(), [draw], clear_color=(0.0, 0.0, 0.0, 0.0), clear_depth=1.0)
+
+
+def _perlin(x: np.ndarray, y: np.ndarray, seed: int) -> np.ndarray:
+    """Classic 2-D gradient (Perlin) noise, fp64, values in about [-0.7, 0.7]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    perm = rng.permutation(256)
+    perm = np.concatenate([perm, perm])
+    ang = rng.uniform(0.0, 2.0 * math.pi, size=256)
+    grad = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    xi, yi = np.floor(x).astype(np.int64), np.floor(y).astype(np.int64)
+    xf, yf = x - xi, y - yi
+    xi &= 255
+    yi &= 255
+
+    def g(ix, iy, dx, dy):
+        h = perm[perm[ix] + iy]
+        return grad[h, 0] * dx + grad[h, 1] * dy
+
+    def fade(t):
+        return t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
+
+    u, v = fade(xf), fade(yf)
+    n00 = g(xi, yi, xf, yf)
+    n10 = g(xi + 1, yi, xf - 1.0, yf)
+    n01 = g(xi, yi + 1, xf, yf - 1.0)
+    n11 = g(xi + 1, yi + 1, xf - 1.0, yf - 1.0)
+    a = n00 + u * (n10 - n00)
+    b = n01 + u * (n11 - n01)
+    return a + v * (b - a)
+
+
+def noise_texture(size: int = 1024, seed: int = 7) -> TextureData:
+    """size^2 RGBA8 texture of uniform U8 noise with its full mip chain, each
+    level the rounded 2x2 box filter of the one above, laid out level after
+    level as GetNormalImageSize places them (CPVulkanBase/Formats.cpp:455-482)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lv = rng.integers(0, 256, size=(size, size, 4), dtype=np.uint16)
+    levels = [lv.astype(np.uint8)]
+    while lv.shape[0] > 1 or lv.shape[1] > 1:
+        h, w = max(lv.shape[0] // 2, 1), max(lv.shape[1] // 2, 1)
+        src = lv
+        if src.shape[0] == 1:
+            src = np.concatenate([src, src], axis=0)
+        if src.shape[1] == 1:
+            src = np.concatenate([src, src], axis=1)
+        s = src[0::2, 0::2] + src[1::2, 0::2] + src[0::2, 1::2] + src[1::2, 1::2]
+        lv = ((s + 2) // 4)[:h, :w]
+        levels.append(lv.astype(np.uint8))
+    data = np.concatenate([l.reshape(-1) for l in levels])
+    return TextureData(data=data, format=abi.FORMAT["R8G8B8A8_UNORM"], width=size, height=size, levels=len(levels),
+                       sampler=dict(mag=abi.FILTER["LINEAR"], min=abi.FILTER["LINEAR"], mipmap=1,
+                                    address_u=abi.ADDRESS["REPEAT"], address_v=abi.ADDRESS["REPEAT"],
+                                    min_lod=0.0, max_lod=11.0, bias=0.0))
+
+
+def grid_mvp(width: int, height: int, orbit_deg: float = 34.0) -> np.ndarray:
+    """Vulkan clip * perspective(60 deg) * lookAt from an orbit around the grid."""
+    fov = _f32(60.0) * _f32(0.01745329251994329576923690768489)
+    proj = perspective(fov, _f32(width) / _f32(height), 0.1, 100.0)
+    th = math.radians(orbit_deg)
+    eye = (0.72 * math.sin(th), 0.8, -0.72 * math.cos(th))
+    view = look_at(eye, (0.0, -0.1, 0.0), (0.0, 1.0, 0.0))
+    return mat4_mul(mat4_mul(vulkan_clip(), proj), view)
+
+
+def grid_scene(n: int = 512, width: int = 3840, height: int = 2160, seed: int = 3, tex_size: int = 1024,
+               uv_repeat: float = 4.0, orbit_deg: float = 34.0) -> Scene:
+    """textured indexed heightfield with Lambert shading.
+
+    n x n vertices over [-1,1]^2 in xz with y = 0.2 * Perlin; vertex = pos vec3,
+    normal vec3, uv vec2 (32 B); uint32 indices, two triangles (a, b, c),
+    (c, b, d) per cell; texture tex_size^2 RGBA8 noise with a full mip chain,
+    LINEAR/LINEAR/MIPMAP_LINEAR, REPEAT; R8G8B8A8 + D32 LESS, no blend.
+    Fragment colour = texture(uv) * max(dot(normalize(n), L), 0)."""
+    lin = np.linspace(-1.0, 1.0, n)
+    X, Z = np.meshgrid(lin, lin)                       # row = z, column = x
+    f = 3.0
+    H = 0.2 * _perlin((X + 1.0) * f, (Z + 1.0) * f, seed)
+    eps = 1e-3
+    dhx = 0.2 * (_perlin((X + 1.0 + eps) * f, (Z + 1.0) * f, seed) - _perlin((X + 1.0 - eps) * f, (Z + 1.0) * f, seed)) / (2 * eps)
+    dhz = 0.2 * (_perlin((X + 1.0) * f, (Z + 1.0 + eps) * f, seed) - _perlin((X + 1.0) * f, (Z + 1.0 - eps) * f, seed)) / (2 * eps)
+    N = np.stack([-dhx, np.ones_like(H), -dhz], axis=-1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    v = np.empty((n, n, 8), np.float64)
+    v[..., 0], v[..., 1], v[..., 2] = X, H, Z
+    v[..., 3:6] = N
+    v[..., 6] = (X + 1.0) * 0.5 * uv_repeat
+    v[..., 7] = (Z + 1.0) * 0.5 * uv_repeat
+    verts = v.reshape(-1, 8).astype(np.float32)
+    i = np.arange(n - 1)
+    a = (i[:, None] * n + i[None, :]).reshape(-1)
+    b, c = a + 1, a + n
+    d = c + 1
+    idx = np.stack([a, b, c, c, b, d], axis=1).reshape(-1).astype(np.uint32)
+    light = np.array([0.3, 0.9, -0.2], np.float64)
+    light = np.concatenate([light / np.linalg.norm(light), [0.0]]).astype(np.float32)
+    pipeline = dict(
+        vs=abi.VS["MVP_LAMBERT"], fs=abi.FS["TEX_LAMBERT"],
+        bindings=[(0, 32, 0)],
+        attributes=[(0, 0, abi.FORMAT["R32G32B32_SFLOAT"], 0), (1, 0, abi.FORMAT["R32G32B32_SFLOAT"], 12),
+                    (2, 0, abi.FORMAT["R32G32_SFLOAT"], 24)],
+        cull=abi.CULL["NONE"], front_face=abi.FRONT_FACE["CCW"],
+        depth_test=1, depth_write=1, depth_compare=abi.COMPARE["LESS"],
+    )
+    draw = DrawData(vertices={0: verts.view(np.uint8).reshape(-1)}, count=idx.size, indices=idx,
+                    viewport=(0.0, 0.0, float(width), float(height), 0.0, 1.0),
+                    uniforms={0: grid_mvp(width, height, orbit_deg).reshape(-1).view(np.uint8),
+                              2: light.view(np.uint8)},
+                    textures={1: noise_texture(tex_size, seed + 100)})
+    return Scene(f"grid{n}", width, height, abi.FORMAT["R8G8B8A8_UNORM"], abi.FORMAT["D32_SFLOAT"], pipeline, [draw],
+                 clear_color=(0.0, 0.0, 0.0, 0.0), clear_depth=1.0)
