@@ -47,6 +47,25 @@ def _declare(lib: C.CDLL) -> None:
     lib.cpv_program_source.argtypes = [P]
     lib.cpv_program_source.restype = C.c_char_p
     lib.cpv_abi_sizes.argtypes = [C.POINTER(C.c_uint32), C.c_uint32]
+    lib.cpv_get_tile_traces.argtypes = [P, C.POINTER(abi.TileTrace), C.c_uint32, C.POINTER(C.c_uint32)]
+    lib.cpv_batch_create.argtypes = [P, C.POINTER(abi.JobDesc), C.c_uint32, C.POINTER(P)]
+    lib.cpv_batch_submit.argtypes = [P]
+    lib.cpv_batch_wait.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.cpv_batch_timing.argtypes = [P, C.POINTER(C.c_float)]
+    lib.cpv_batch_destroy.argtypes = [P]
+    lib.cpv_batch_destroy.restype = None
+    lib.cpv_autotune.argtypes = [P, C.POINTER(abi.JobDesc), C.c_uint32, C.c_uint32, C.POINTER(abi.AutotuneEntry), C.c_uint32,
+                                 C.POINTER(C.c_uint32), C.POINTER(abi.Config)]
+    FP = C.POINTER(C.c_float)
+    lib.cpv_mlp_create.argtypes = [P, C.c_uint64, C.POINTER(P)]
+    lib.cpv_mlp_destroy.argtypes = [P]
+    lib.cpv_mlp_destroy.restype = None
+    lib.cpv_mlp_get_params.argtypes = [P, FP, C.c_uint32]
+    lib.cpv_mlp_set_params.argtypes = [P, FP, C.c_uint32]
+    lib.cpv_mlp_predict.argtypes = [P, FP, C.c_uint32, FP]
+    lib.cpv_mlp_gradients.argtypes = [P, FP, FP, C.c_uint32, FP, FP]
+    lib.cpv_mlp_train.argtypes = [P, FP, FP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, FP]
+    lib.cpv_set_predictor.argtypes = [P, P]
 
 
 def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
@@ -153,3 +172,134 @@ class Context:
         r, t = C.c_float(), C.c_float()
         _check(self.lib, self.lib.cpv_last_timing(self.handle, C.byref(r), C.byref(t)), "cpv_last_timing")
         return r.value, t.value
+
+    def config(self) -> abi.Config:
+        cfg = abi.Config()
+        _check(self.lib, self.lib.cpv_get_config(self.handle, C.byref(cfg)), "cpv_get_config")
+        return cfg
+
+    def apply_config(self, cfg: abi.Config) -> None:
+        _check(self.lib, self.lib.cpv_set_config(self.handle, C.byref(cfg)), "cpv_set_config")
+
+    def tile_traces(self):
+        """Per-tile cost traces of the last launch (record_costs on): a dict
+        of numpy arrays features [n, 12], cycles [n], job [n], tile [n]."""
+        import numpy as np
+        n = C.c_uint32()
+        _check(self.lib, self.lib.cpv_get_tile_traces(self.handle, None, 0, C.byref(n)), "cpv_get_tile_traces")
+        arr = (abi.TileTrace * max(n.value, 1))()
+        _check(self.lib, self.lib.cpv_get_tile_traces(self.handle, arr, n.value, C.byref(n)), "cpv_get_tile_traces")
+        raw = np.frombuffer(arr, dtype=np.uint8).reshape(max(n.value, 1), C.sizeof(abi.TileTrace))[: n.value]
+        f = raw[:, :52].copy().view(np.float32)
+        u = raw[:, 52:64].copy().view(np.uint32)
+        return dict(features=f[:, :12].copy(), cycles=f[:, 12].copy(), job=u[:, 0].copy(), tile=u[:, 1].copy())
+
+    def batch(self, jobs) -> "Batch":
+        return Batch(self, jobs)
+
+    def autotune(self, jobs, repeats: int = 3):
+        """Sweeps the tile configurations over a recorded batch of jobs;
+        returns ([(tile, waves, ms)], best abi.Config)."""
+        arr = (abi.JobDesc * max(len(jobs), 1))(*jobs)
+        entries = (abi.AutotuneEntry * 16)()
+        n = C.c_uint32()
+        best = abi.Config()
+        rc = self.lib.cpv_autotune(self.handle, arr, len(jobs), repeats, entries, 16, C.byref(n), C.byref(best))
+        _check(self.lib, rc, "cpv_autotune")
+        return [(e.tile_size, e.waves_per_tile, e.ms) for e in entries[: n.value]], best
+
+    def set_predictor(self, mlp: "Mlp | None") -> None:
+        _check(self.lib, self.lib.cpv_set_predictor(self.handle, mlp.handle if mlp else None), "cpv_set_predictor")
+
+
+class Batch:
+    """A recorded set of render jobs replayed as one hipGraph (cpv_batch_*)."""
+
+    def __init__(self, ctx: Context, jobs):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self._jobs = (abi.JobDesc * max(len(jobs), 1))(*jobs)
+        self.handle = C.c_void_p()
+        _check(self.lib, self.lib.cpv_batch_create(ctx.handle, self._jobs, len(jobs), C.byref(self.handle)), "cpv_batch_create")
+
+    def submit(self) -> None:
+        _check(self.lib, self.lib.cpv_batch_submit(self.handle), "cpv_batch_submit")
+
+    def wait(self, count: bool = False):
+        v = C.c_uint64()
+        _check(self.lib, self.lib.cpv_batch_wait(self.handle, C.byref(v) if count else None), "cpv_batch_wait")
+        return v.value if count else None
+
+    def timing(self) -> float:
+        ms = C.c_float()
+        _check(self.lib, self.lib.cpv_batch_timing(self.handle, C.byref(ms)), "cpv_batch_timing")
+        return ms.value
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.cpv_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Mlp:
+    """The tile-cost predictor (cpv_mlp_*): BF16 MLP 12 -> 64 -> 64 -> 1 run
+    and trained on the GPU."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.handle = C.c_void_p()
+        _check(self.lib, self.lib.cpv_mlp_create(ctx.handle, seed, C.byref(self.handle)), "cpv_mlp_create")
+
+    @staticmethod
+    def _f(a):
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def params(self):
+        import numpy as np
+        out = np.empty(abi.MLP_PARAMS, np.float32)
+        _check(self.lib, self.lib.cpv_mlp_get_params(self.handle, self._f(out)[1], abi.MLP_PARAMS), "cpv_mlp_get_params")
+        return out
+
+    def set_params(self, p) -> None:
+        a, ptr = self._f(p)
+        _check(self.lib, self.lib.cpv_mlp_set_params(self.handle, ptr, a.size), "cpv_mlp_set_params")
+
+    def predict(self, x):
+        import numpy as np
+        xa, xp = self._f(x)
+        rows = xa.shape[0]
+        y = np.empty(rows, np.float32)
+        _check(self.lib, self.lib.cpv_mlp_predict(self.handle, xp, rows, self._f(y)[1]), "cpv_mlp_predict")
+        return y
+
+    def gradients(self, x, t):
+        import numpy as np
+        xa, xp = self._f(x)
+        ta, tp = self._f(t)
+        g = np.empty(abi.MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        rc = self.lib.cpv_mlp_gradients(self.handle, xp, tp, xa.shape[0], self._f(g)[1], C.byref(loss))
+        _check(self.lib, rc, "cpv_mlp_gradients")
+        return g, loss.value
+
+    def train(self, x, t, steps: int, batch: int = 1024, lr: float = 1e-3, seed: int = 0) -> float:
+        xa, xp = self._f(x)
+        ta, tp = self._f(t)
+        loss = C.c_float()
+        rc = self.lib.cpv_mlp_train(self.handle, xp, tp, xa.shape[0], steps, batch, lr, seed, C.byref(loss))
+        _check(self.lib, rc, "cpv_mlp_train")
+        return loss.value
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.cpv_mlp_destroy(self.handle)
+            self.handle = C.c_void_p()

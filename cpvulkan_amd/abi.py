@@ -139,7 +139,16 @@ class TileTrace(C.Structure):
     _fields_ = [("features", f32 * 12), ("cycles", f32), ("job", u32), ("tile", u32), ("reserved", u32)]
 
 
-STRUCT_ORDER = [PipelineDesc, DrawDesc, PassDesc, AttachmentDesc, TextureDesc, SamplerDesc, JobDesc, Config, TileTrace]
+class AutotuneEntry(C.Structure):
+    _fields_ = [("tile_size", u32), ("waves_per_tile", u32), ("ms", f32), ("reserved", u32)]
+
+
+MLP_FEATURES = 12
+MLP_HIDDEN = 64
+MLP_PARAMS = 12 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+
+STRUCT_ORDER = [PipelineDesc, DrawDesc, PassDesc, AttachmentDesc, TextureDesc, SamplerDesc, JobDesc, Config, TileTrace,
+                AutotuneEntry]
 
 
 def check_layout(lib: C.CDLL) -> None:

@@ -55,8 +55,9 @@ def _obj(src: Path, out_dir: Path, hip: bool) -> Path:
 
 def build_libcpv() -> Path:
     objdir = PKG / "_build" / "libcpv"
-    srcs = [(CSRC / "cpv_runtime.hip", True), (CSRC / "spirv_compile.cpp", False)]
-    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+    srcs = [(CSRC / "cpv_runtime.hip", True), (CSRC / "cpv_launch.hip", True), (CSRC / "cpv_mlp.hip", True),
+            (CSRC / "spirv_compile.cpp", False)]
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
         objs = list(ex.map(lambda s: _obj(s[0], objdir, s[1]), srcs))
     LIB.mkdir(parents=True, exist_ok=True)
     out = LIB / "libcpv.so"

@@ -264,6 +264,7 @@ struct EmitArgs {
     uint32_t* big;                // primitives left to emit_big_kernel; big[0] is the count
     uint32_t prims;
     int32_t tile_shift;
+    uint32_t cap;                 // capacity of keys / vals; pairs at or beyond it are dropped
 };
 
 constexpr uint32_t kEmitSmall = 32;   // pairs a single thread writes
@@ -290,8 +291,10 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
         for (uint32_t ty = ty0; ty <= ty1; ty++) {
             for (uint32_t tx = tx0; tx <= tx1; tx++) {
                 const uint32_t key = t.tile_base + ty * t.tiles_x + tx;
-                a.keys[off] = key;
-                a.vals[off] = gp;
+                if (off < a.cap) {
+                    a.keys[off] = key;
+                    a.vals[off] = gp;
+                }
                 off++;
             }
         }
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(256) void emit_big_kernel(EmitArgs a)
         const uint32_t ty0 = (uint32_t)r.y0 >> sh;
         const uint32_t w = tx1 - tx0 + 1;
         const uint32_t c = a.counts[gp], off = a.offsets[gp];
-        for (uint32_t k = threadIdx.x; k < c; k += 256u) {
+        for (uint32_t k = threadIdx.x; k < c && off + k < a.cap; k += 256u) {
             const uint32_t key = t.tile_base + (ty0 + k / w) * t.tiles_x + tx0 + k % w;
             a.keys[off + k] = key;
             a.vals[off + k] = gp;
@@ -321,11 +324,23 @@ __global__ __launch_bounds__(256) void emit_big_kernel(EmitArgs a)
     }
 }
 
+// Pair count kept for sorting: the scanned total clamped to the capacity of
+// the pair arrays; a larger total raises the overflow word.
+__global__ void clamp_pairs_kernel(const uint32_t* total, uint32_t cap, uint32_t* kept, uint32_t* overflow)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t t = *total;
+    *kept = t < cap ? t : cap;
+    if (t > cap) *overflow = 1u;
+}
+
 // Start of every tile's range in the tile-sorted pair array: each boundary
-// between different keys writes the starts of the tiles it spans.
-__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t* keys, uint32_t n, uint32_t tiles, uint32_t* start)
+// between different keys writes the starts of the tiles it spans.  The grid
+// covers the capacity; the pair count is read on the device.
+__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t* keys, const uint32_t* n_ptr, uint32_t tiles, uint32_t* start)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t n = *n_ptr;
     if (i > n) return;
     const uint32_t prev = i == 0 ? 0u : keys[i - 1] + 1u;   // first tile not before this boundary
     const uint32_t cur = i == n ? tiles : keys[i];          // last tile starting here

@@ -1,0 +1,114 @@
+// Host-side declarations shared by the translation units of libcpv.so
+// (runtime, launch sequence, tile-cost predictor).  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cpv.h"
+#include "cpv_program.h"
+#include "cpv_types.h"
+#include "raster_device.h"
+
+namespace cpvrt {
+
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                                            \
+    do {                                                                                                         \
+        hipError_t e_ = (expr);                                                                                  \
+        if (e_ != hipSuccess) return ::cpvrt::fail(CPV_ERROR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+// Grow-only device buffer.  With `frozen` set (while a graph is captured)
+// growing is an error instead of an allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes, bool frozen = false);
+    template <class T> T* as() const { return (T*)p; }
+    void release();
+};
+
+// Control words at the start of Buffers::stats.
+enum : uint32_t {
+    kStatFragments = 0,   // u64 at words 0-1: covered fragments
+    kStatQueue = 8,       // work counter of the persistent raster kernel
+    kStatPairs = 9,       // (tile, primitive) pairs kept, min(total, capacity)
+    kStatOverflow = 10,   // set when the pair total exceeded the capacity
+    kStatTiles = 11,      // tile count of the launch (length of the order sort)
+    kStatWords = 64,
+};
+
+// Working buffers of the launch sequence.  A context owns one set for its
+// immediate render calls; a recorded batch owns its own so that replaying
+// it never reallocates.
+struct Buffers {
+    DevBuf vout, recs, counts, offsets, keys[2], vals[2], tile_start, hist, hist_base, sums, big;
+    DevBuf arena, stats, traces, features, pred, okeys[2], ovals[2];
+    void release_all();
+};
+
+struct Program {
+    KernelRef vertex;
+    KernelRef raster[7];
+    int vs_outputs = 0;
+    int fs_inputs = 0;
+};
+
+struct MlpDevice;   // the tile-cost predictor's device state (cpv_mlp.hip)
+
+}  // namespace cpvrt
+
+struct cpv_mlp;
+
+struct cpv_context {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_raster0 = nullptr, ev_raster1 = nullptr, ev_end = nullptr;
+    cpv_config cfg{};
+    cpvrt::Buffers buf;
+    void* pinned = nullptr;
+    size_t pinned_cap = 0;
+    uint32_t* readback = nullptr;     // pinned scalar readbacks
+    // what the last render call left behind (for traces / counts)
+    uint32_t last_tiles = 0;
+    uint32_t last_waves = 0;
+    uint32_t last_tile_size = 0;
+    std::vector<uint32_t> last_target_base;   // first tile of every target of the last launch
+    bool last_traces = false;
+    bool timing_valid = false;
+    bool last_count_exact = false;
+    cpv_mlp* predictor = nullptr;     // tile-cost model of CPV_SCHED_PREDICTED
+    std::map<const void*, int> occupancy;   // resident workgroups per CU, by raster kernel
+    std::mutex lock;
+};
+
+namespace cpvrt {
+
+bool format_desc(uint32_t f, cpv::FormatDesc* o);
+int tile_config_index(uint32_t t, uint32_t w);
+bool valid_tile_config(uint32_t t, uint32_t w);
+bool builtin_program(uint32_t vs, uint32_t fs, Program* p);
+int resolve_program(const cpv_pipeline_desc& pd, Program* p);
+hipError_t launch(const KernelRef& k, dim3 grid, dim3 block, void* args, hipStream_t s);
+int build_fragment_state(const cpv_pipeline_desc& p, const cpv_pass_desc& pass, cpv::FragmentState* fs);
+int build_env(const cpv_pipeline_desc& p, const cpv_draw_desc& d, cpv::DrawEnv* e);
+int build_target(const cpv_pass_desc& pass, bool first_segment, int tile, uint32_t tile_base, cpv::TargetParams* t);
+uint32_t prim_count_of(uint32_t topology, uint32_t n);
+void fill_u32(hipStream_t s, void* p, uint32_t v, uint64_t n);
+int ensure_pinned(cpv_context* c, size_t bytes);
+
+// Tile-cost predictor (cpv_mlp.hip): enqueue inference of `tiles` feature
+// rows (16 floats each) into per-tile predicted log2 costs.
+int mlp_enqueue_predict(cpv_mlp* m, const float* features, uint32_t tiles, float* out, hipStream_t s);
+constexpr int kFeatures = 12;       // features per tile the predictor reads
+constexpr int kFeatureStride = 16;  // floats per feature row (padded to the MFMA depth)
+
+}  // namespace cpvrt

@@ -1,0 +1,860 @@
+// Host runtime of libcpv.so, part 2: the launch sequence.
+//
+// A render call becomes, per run of draws that share a shader program, one
+// launch of each stage on the context's stream:
+//   vertex kernel -> setup kernel -> scan -> emit -> radix sort by tile ->
+//   tile ranges -> [tile features -> cost prediction -> queue order] ->
+//   raster kernel
+// The raster kernel fuses the render pass clear (loadOp CLEAR) into its tile
+// initialisation, so a frame touches each framebuffer byte once on the way
+// in (LOAD only) and once on the way out.
+//
+// The sequence is split at the only point where the host needs a device
+// value -- the number of (tile, primitive) pairs, which sizes the sort -- so
+// that a recorded batch (cpv_batch_*) can capture the whole frame into one
+// hipGraph: its pair arrays are sized once at record time, and the emit
+// kernels drop pairs past that capacity and raise an overflow word instead
+// of writing out of bounds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "cpv_internal.h"
+#include "raster_kernels.h"
+#include "bin_kernels.h"
+#include "sched_kernels.h"
+
+using namespace cpv;
+using namespace cpvrt;
+
+namespace {
+
+// Host staging of one or more launches' descriptors, copied to the device
+// arena in one transfer.
+struct Staging {
+    std::vector<uint8_t> bytes;
+    size_t put(const void* p, size_t n, size_t align = 256)
+    {
+        size_t at = (bytes.size() + align - 1) & ~(align - 1);
+        bytes.resize(at + n);
+        if (n) std::memcpy(bytes.data() + at, p, n);
+        return at;
+    }
+};
+
+struct LaunchDraw {
+    const cpv_pipeline_desc* pipe;
+    const cpv_draw_desc* draw;
+    uint32_t instance;
+};
+
+// Everything one launch needs, resolved on the host.
+struct LaunchPlan {
+    Program prog;
+    std::vector<DrawDesc> dd;
+    std::vector<DrawEnv> envs;
+    std::vector<FragmentState> fstates;
+    std::vector<TargetParams> targets;
+    std::vector<uint32_t> vblock, sblock;
+    uint32_t vout_total = 0, prim_total = 0, tiles = 0, stride = 0;
+    int tile_shift = 4;
+    int32_t hiz_op = -1, hiz_kind = 32;
+    size_t o_env = 0, o_dd = 0, o_fs = 0, o_tp = 0, o_vb = 0, o_sb = 0;   // offsets in the arena
+};
+
+int plan_launch(const cpv_config& cfg, const Program& prog, const std::vector<TargetParams>& targets,
+                const std::vector<LaunchDraw>& lds, const std::vector<cpv_pass_desc>& passes,
+                const std::vector<uint32_t>& draw_target, LaunchPlan* P)
+{
+    const int tile = (int)cfg.tile_size;
+    P->prog = prog;
+    P->targets = targets;
+    P->tile_shift = tile == 8 ? 3 : tile == 16 ? 4 : 5;
+    P->stride = 4u + (uint32_t)prog.vs_outputs;
+    if (prog.fs_inputs > prog.vs_outputs) return fail(CPV_ERROR_INVALID, "fragment inputs exceed vertex outputs");
+    P->dd.assign(lds.size(), DrawDesc{});
+    P->envs.resize(lds.size());
+    P->fstates.resize(lds.size());
+    uint32_t vout_total = 0, prim_total = 0;
+    for (size_t i = 0; i < lds.size(); i++) {
+        const cpv_pipeline_desc& p = *lds[i].pipe;
+        const cpv_draw_desc& d = *lds[i].draw;
+        int rc = build_env(p, d, &P->envs[i]);
+        if (rc) return rc;
+        rc = build_fragment_state(p, passes[draw_target[i]], &P->fstates[i]);
+        if (rc) return rc;
+        if (p.topology < CPV_TOPOLOGY_TRIANGLE_LIST || p.topology > CPV_TOPOLOGY_TRIANGLE_FAN)
+            return fail(CPV_ERROR_UNSUPPORTED, "topology %u (lines and points are not on this path)", p.topology);
+        if (d.indexed && d.index_size != 1 && d.index_size != 2 && d.index_size != 4)
+            return fail(CPV_ERROR_INVALID, "index size %u", d.index_size);
+        if (d.indexed && p.primitive_restart) return fail(CPV_ERROR_UNSUPPORTED, "primitive restart");
+        if (!(d.viewport[2] > 0.0f) || !(d.viewport[3] > 0.0f)) return fail(CPV_ERROR_INVALID, "viewport extent");
+        if (!p.program) {
+            // resources the built-in shaders dereference
+            if ((p.vs_kind == CPV_VS_MVP_COLOR || p.vs_kind == CPV_VS_MVP_LAMBERT) && !d.uniform_address[0])
+                return fail(CPV_ERROR_INVALID, "vertex shader needs a uniform buffer at binding 0");
+            if (p.fs_kind == CPV_FS_TEX_LAMBERT && (!d.uniform_address[2] || !d.textures[1].address))
+                return fail(CPV_ERROR_INVALID, "fragment shader needs a uniform at binding 2 and a texture at binding 1");
+        }
+        DrawDesc& o = P->dd[i];
+        std::memset(&o, 0, sizeof(o));
+        o.target = draw_target[i];
+        o.fstate = (uint32_t)i;
+        o.topology = p.topology;
+        o.front_face = p.front_face;
+        o.cull_mode = p.cull_mode;
+        o.first = d.first;
+        o.vertex_offset = d.vertex_offset;
+        o.index_size = d.index_size;
+        o.index_addr = d.index_address;
+        o.instance_id = d.first_instance + lds[i].instance;
+        o.vp_w = d.viewport[2];
+        o.vp_h = d.viewport[3];
+        o.vp_min_depth = d.viewport[4];
+        o.vp_max_depth = d.viewport[5];
+        o.half_x = 1.0f / d.viewport[2] * 0.5f;
+        o.half_y = 1.0f / d.viewport[3] * 0.5f;
+        const float vw = d.viewport[2], vh = d.viewport[3];
+        const int32_t lx = (vw >= -2147483648.0f && vw < 2147483648.0f) ? (int32_t)vw : INT32_MIN;
+        const int32_t ly = (vh >= -2147483648.0f && vh < 2147483648.0f) ? (int32_t)vh : INT32_MIN;
+        const TargetParams& t = targets[o.target];
+        o.limit_x = std::min<int32_t>(lx, (int32_t)t.width);
+        o.limit_y = std::min<int32_t>(ly, (int32_t)t.height);
+        // vertex slots
+        uint32_t nverts = d.count;
+        o.vmode = d.indexed ? kSlotPerIndex : kSlotSequential;
+        if (d.indexed) {
+            // Shade every vertex of the bound buffers once when that is fewer
+            // invocations than one per index (results are identical: the
+            // vertex shader depends only on the vertex and instance ids).
+            uint64_t nv = UINT64_MAX;
+            bool any = false;
+            for (uint32_t a = 0; a < p.attribute_count; a++) {
+                const cpv_vertex_attribute_desc& at = p.attributes[a];
+                for (uint32_t b = 0; b < p.binding_count; b++) {
+                    if (p.bindings[b].binding != at.binding || p.bindings[b].input_rate != CPV_INPUT_RATE_VERTEX) continue;
+                    if (p.bindings[b].stride == 0) continue;
+                    any = true;
+                    const uint64_t bytes = d.vertex_bytes[at.binding & 15];
+                    nv = std::min<uint64_t>(nv, bytes / p.bindings[b].stride);
+                }
+            }
+            if (any && nv < d.count) { o.vmode = kSlotVertexId; nverts = (uint32_t)nv; }
+        }
+        o.vout_base = vout_total;
+        o.vout_count = nverts;
+        o.prim_base = prim_total;
+        o.prim_count = prim_count_of(p.topology, d.count);
+        o.vblock_base = (uint32_t)P->vblock.size();
+        o.sblock_base = (uint32_t)P->sblock.size();
+        for (uint32_t b = 0; b < (nverts + 255) / 256; b++) P->vblock.push_back((uint32_t)i);
+        for (uint32_t b = 0; b < (o.prim_count + 255) / 256; b++) P->sblock.push_back((uint32_t)i);
+        vout_total += nverts;
+        prim_total += o.prim_count;
+    }
+    if (prim_total >= (1u << 28)) return fail(CPV_ERROR_UNSUPPORTED, "more than 2^28 primitives in one launch");
+    P->vout_total = vout_total;
+    P->prim_total = prim_total;
+    uint32_t tiles = 0;
+    for (const TargetParams& t : targets) tiles = std::max(tiles, t.tile_base + t.tiles_x * t.tiles_y);
+    P->tiles = tiles;
+    // Hierarchical depth culling is exact when only the depth test observes
+    // a fragment (no stencil, no depth bounds), every draw uses the same
+    // compare op and depth format, and nobody counts covered fragments.
+    P->hiz_op = -1;
+    P->hiz_kind = 32;
+    if (!cfg.count_fragments && !P->fstates.empty()) {
+        const std::vector<FragmentState>& fs = P->fstates;
+        bool ok = true;
+        for (const FragmentState& f : fs) {
+            ok = ok && f.depth_test && f.has_ds && f.ds.depth_off >= 0 && f.ds.vk_format != CPV_FORMAT_S8_UINT &&
+                 !f.depth_bounds_test && !f.stencil_test && f.depth_compare == fs[0].depth_compare &&
+                 f.ds.depth_kind == fs[0].ds.depth_kind;
+        }
+        for (const TargetParams& t : targets) ok = ok && t.has_ds && t.ds_fmt.depth_kind == fs[0].ds.depth_kind;
+        if (ok) {
+            P->hiz_op = fs[0].depth_compare;
+            P->hiz_kind = fs[0].ds.depth_kind;
+        }
+    }
+    // the block tables are never empty
+    if (P->vblock.empty()) P->vblock.push_back(0);
+    if (P->sblock.empty()) P->sblock.push_back(0);
+    return CPV_OK;
+}
+
+// Places a plan's descriptors in the staging buffer (draw descriptors are
+// written again by patch_plan once the arena address is known).
+void stage_plan(LaunchPlan& P, Staging& st)
+{
+    P.o_env = st.put(P.envs.data(), P.envs.size() * sizeof(DrawEnv));
+    P.o_dd = st.put(P.dd.data(), P.dd.size() * sizeof(DrawDesc));
+    P.o_fs = st.put(P.fstates.data(), P.fstates.size() * sizeof(FragmentState));
+    P.o_tp = st.put(P.targets.data(), P.targets.size() * sizeof(TargetParams));
+    P.o_vb = st.put(P.vblock.data(), P.vblock.size() * 4);
+    P.o_sb = st.put(P.sblock.data(), P.sblock.size() * 4);
+}
+
+void patch_plan(LaunchPlan& P, Staging& st, uint8_t* arena)
+{
+    for (size_t i = 0; i < P.dd.size(); i++) P.dd[i].env = (uint64_t)(uintptr_t)(arena + P.o_env + i * sizeof(DrawEnv));
+    if (!P.dd.empty()) std::memcpy(st.bytes.data() + P.o_dd, P.dd.data(), P.dd.size() * sizeof(DrawDesc));
+}
+
+// Execution environment of an enqueue: the stream, the buffers and the
+// configuration the launch runs with.
+struct Exec {
+    hipStream_t s = nullptr;
+    Buffers* b = nullptr;
+    cpv_config cfg{};
+    cpv_mlp* predictor = nullptr;
+    cpv_context* ctx = nullptr;   // CU count and the occupancy cache
+    bool frozen = false;          // recording a graph: no allocation
+    hipEvent_t r0 = nullptr, r1 = nullptr;
+};
+
+int scan_u32(Exec& x, const uint32_t* in, uint32_t n, uint32_t* out, bool write_total)
+{
+    if (n == 0) {
+        if (write_total) fill_u32(x.s, out, 0u, 1);
+        return CPV_OK;
+    }
+    const uint32_t blocks = (n + kScanTile - 1) / kScanTile;
+    if (x.b->sums.ensure((size_t)blocks * 4, x.frozen)) return CPV_ERROR_OUT_OF_MEMORY;
+    uint32_t* sums = x.b->sums.as<uint32_t>();
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(blocks), dim3(kScanThreads), 0, x.s, in, n, sums);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanThreads), 0, x.s, sums, blocks);
+    hipLaunchKernelGGL(scan_down_kernel, dim3(blocks), dim3(kScanThreads), 0, x.s, in, n, (const uint32_t*)sums, out,
+                       write_total ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    return CPV_OK;
+}
+
+// Stable LSD radix sort of (key, value) pairs by the low `bits` key bits;
+// n is read on the device.  Returns the buffer index holding the result.
+int radix_sort(Exec& x, DevBuf* keys, DevBuf* vals, const uint32_t* d_n, int bits, int* result)
+{
+    Buffers& b = *x.b;
+    int src = 0;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(kSortSegments / kSortWaves), dim3(64 * kSortWaves), 0, x.s,
+                           (const uint32_t*)keys[src].p, d_n, shift, b.hist.as<uint32_t>());
+        int rc = scan_u32(x, b.hist.as<uint32_t>(), 256u * kSortSegments, b.hist_base.as<uint32_t>(), false);
+        if (rc) return rc;
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(kSortSegments / kSortWaves), dim3(64 * kSortWaves), 0, x.s,
+                           (const uint32_t*)keys[src].p, (const uint32_t*)vals[src].p, keys[1 - src].as<uint32_t>(),
+                           vals[1 - src].as<uint32_t>(), d_n, shift, (const uint32_t*)b.hist_base.p);
+        HIP_TRY(hipGetLastError());
+        src = 1 - src;
+    }
+    *result = src;
+    return CPV_OK;
+}
+
+int effective_scheduler(const Exec& x)
+{
+    if (x.cfg.scheduler == CPV_SCHED_PREDICTED && !x.predictor) return CPV_SCHED_COUNTER;
+    return (int)x.cfg.scheduler;
+}
+
+bool needs_features(const Exec& x) { return effective_scheduler(x) == CPV_SCHED_PREDICTED || x.cfg.record_costs; }
+
+int resident_workgroups(Exec& x, const KernelRef& k, int block)
+{
+    const void* key = k.host_fn ? k.host_fn : (const void*)k.module_fn;
+    auto it = x.ctx->occupancy.find(key);
+    int per_cu = 0;
+    if (it != x.ctx->occupancy.end()) {
+        per_cu = it->second;
+    } else {
+        if (k.host_fn) {
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k.host_fn, block, 0));
+        } else {
+            HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k.module_fn, block, 0));
+        }
+        per_cu = std::max(per_cu, 1);
+        x.ctx->occupancy[key] = per_cu;
+    }
+    return per_cu * std::max(x.ctx->cu_count, 1);
+}
+
+// Buffers whose size depends only on the plan.
+int ensure_front(Exec& x, const LaunchPlan& P)
+{
+    Buffers& b = *x.b;
+    const bool f = x.frozen;
+    if (b.vout.ensure((size_t)std::max(P.vout_total, 1u) * P.stride * 4, f) ||
+        b.recs.ensure((size_t)std::max(P.prim_total, 1u) * sizeof(TriSetup), f) ||
+        b.counts.ensure((size_t)std::max(P.prim_total, 1u) * 4, f) || b.offsets.ensure(((size_t)P.prim_total + 1) * 4, f) ||
+        b.stats.ensure(kStatWords * 4, f))
+        return CPV_ERROR_OUT_OF_MEMORY;
+    return CPV_OK;
+}
+
+// Buffers sized by the pair capacity and the tile count.
+int ensure_back(Exec& x, const LaunchPlan& P, uint32_t pair_cap)
+{
+    Buffers& b = *x.b;
+    const bool f = x.frozen;
+    const size_t pairs = (size_t)std::max(pair_cap, 1u) * 4;
+    const size_t tiles = (size_t)std::max(P.tiles, 1u);
+    if (b.keys[0].ensure(pairs, f) || b.keys[1].ensure(pairs, f) || b.vals[0].ensure(pairs, f) || b.vals[1].ensure(pairs, f) ||
+        b.tile_start.ensure((tiles + 1) * 4, f) || b.hist.ensure((size_t)256 * kSortSegments * 4, f) ||
+        b.hist_base.ensure(((size_t)256 * kSortSegments + 1) * 4, f) || b.big.ensure(((size_t)P.prim_total + 1) * 4, f))
+        return CPV_ERROR_OUT_OF_MEMORY;
+    if (needs_features(x)) {
+        if (b.features.ensure(tiles * kFeatureStride * 4, f) || b.pred.ensure(tiles * 4, f) || b.okeys[0].ensure(tiles * 4, f) ||
+            b.okeys[1].ensure(tiles * 4, f) || b.ovals[0].ensure(tiles * 4, f) || b.ovals[1].ensure(tiles * 4, f))
+            return CPV_ERROR_OUT_OF_MEMORY;
+    }
+    if (x.cfg.record_costs && b.traces.ensure(tiles * x.cfg.waves_per_tile * 4, f)) return CPV_ERROR_OUT_OF_MEMORY;
+    return CPV_OK;
+}
+
+// Vertex stage, triangle setup and the pair-count scan; the pair total is
+// left at offsets[prim_total].
+int enqueue_front(Exec& x, const LaunchPlan& P, uint8_t* arena)
+{
+    int rc = ensure_front(x, P);
+    if (rc) return rc;
+    Buffers& b = *x.b;
+    const DrawDesc* d_dd = (const DrawDesc*)(arena + P.o_dd);
+    const TargetParams* d_tp = (const TargetParams*)(arena + P.o_tp);
+    if (P.vout_total > 0) {
+        VertexArgs va{d_dd, (const uint32_t*)(arena + P.o_vb), b.vout.as<float>(), P.stride, 0};
+        HIP_TRY(launch(P.prog.vertex, dim3((uint32_t)P.vblock.size()), dim3(256), &va, x.s));
+    }
+    if (P.prim_total > 0) {
+        SetupArgs sa{d_dd, (const uint32_t*)(arena + P.o_sb), d_tp, b.vout.as<float>(), b.recs.as<TriSetup>(),
+                     b.counts.as<uint32_t>(), P.stride, P.tile_shift};
+        hipLaunchKernelGGL(setup_kernel, dim3((uint32_t)P.sblock.size()), dim3(256), 0, x.s, sa);
+        HIP_TRY(hipGetLastError());
+    }
+    return scan_u32(x, b.counts.as<uint32_t>(), P.prim_total, b.offsets.as<uint32_t>(), true);
+}
+
+// Emit, sort, tile ranges, scheduling and the raster kernel, with the pair
+// arrays holding at most pair_cap pairs.
+int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap)
+{
+    int rc = ensure_back(x, P, pair_cap);
+    if (rc) return rc;
+    Buffers& b = *x.b;
+    uint32_t* stats = b.stats.as<uint32_t>();
+    const DrawDesc* d_dd = (const DrawDesc*)(arena + P.o_dd);
+    const FragmentState* d_fs = (const FragmentState*)(arena + P.o_fs);
+    const TargetParams* d_tp = (const TargetParams*)(arena + P.o_tp);
+    const uint32_t cap = std::max(pair_cap, 1u);
+    hipLaunchKernelGGL(clamp_pairs_kernel, dim3(1), dim3(64), 0, x.s, (const uint32_t*)b.offsets.as<uint32_t>() + P.prim_total,
+                       cap, stats + kStatPairs, stats + kStatOverflow);
+    // emit pairs in primitive order
+    if (P.prim_total) {
+        fill_u32(x.s, b.big.p, 0u, 1);
+        EmitArgs ea{b.recs.as<TriSetup>(), b.counts.as<uint32_t>(), b.offsets.as<uint32_t>(), d_dd, d_tp, b.keys[0].as<uint32_t>(),
+                    b.vals[0].as<uint32_t>(), b.big.as<uint32_t>(), P.prim_total, P.tile_shift, cap};
+        const uint32_t blocks = std::min<uint32_t>((P.prim_total + 255) / 256, 8192);
+        hipLaunchKernelGGL(emit_kernel, dim3(blocks), dim3(256), 0, x.s, ea);
+        hipLaunchKernelGGL(emit_big_kernel, dim3(1024), dim3(256), 0, x.s, ea);
+        HIP_TRY(hipGetLastError());
+    }
+    // stable radix sort of the pairs by tile
+    int src = 0;
+    if (P.tiles > 1) {
+        int bits = 0;
+        while ((1u << bits) < P.tiles) bits++;
+        rc = radix_sort(x, b.keys, b.vals, stats + kStatPairs, bits, &src);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3(cap / 256 + 1), dim3(256), 0, x.s, (const uint32_t*)b.keys[src].p,
+                       (const uint32_t*)(stats + kStatPairs), P.tiles, b.tile_start.as<uint32_t>());
+    HIP_TRY(hipGetLastError());
+
+    // scheduling
+    const int sched = effective_scheduler(x);
+    const uint32_t* order = nullptr;
+    if (needs_features(x) && P.tiles) {
+        FeatureArgs fa{b.recs.as<TriSetup>(), b.vals[src].as<uint32_t>(), b.tile_start.as<uint32_t>(), d_tp, d_dd, d_fs,
+                       b.features.as<float>(), P.tiles, (uint32_t)P.targets.size(), (int32_t)x.cfg.tile_size,
+                       (int32_t)x.cfg.waves_per_tile, P.hiz_op >= 0 ? 1 : 0, 0};
+        hipLaunchKernelGGL(tile_features_kernel, dim3((P.tiles + 3) / 4), dim3(256), 0, x.s, fa);
+        HIP_TRY(hipGetLastError());
+    }
+    if (sched == CPV_SCHED_PREDICTED && P.tiles) {
+        rc = mlp_enqueue_predict(x.predictor, b.features.as<float>(), P.tiles, b.pred.as<float>(), x.s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(order_keys_kernel, dim3((P.tiles + 255) / 256), dim3(256), 0, x.s, (const float*)b.pred.p, P.tiles,
+                           b.okeys[0].as<uint32_t>(), b.ovals[0].as<uint32_t>());
+        fill_u32(x.s, stats + kStatTiles, P.tiles, 1);
+        int osrc = 0;
+        rc = radix_sort(x, b.okeys, b.ovals, stats + kStatTiles, 16, &osrc);
+        if (rc) return rc;
+        order = b.ovals[osrc].as<uint32_t>();
+    }
+
+    // raster
+    const int ci = tile_config_index(x.cfg.tile_size, x.cfg.waves_per_tile);
+    const KernelRef& rk = P.prog.raster[ci];
+    const int block = 64 * (int)x.cfg.waves_per_tile;
+    RasterArgs ra{};
+    ra.recs = b.recs.as<TriSetup>();
+    ra.tile_prims = b.vals[src].as<uint32_t>();
+    ra.tile_start = b.tile_start.as<uint32_t>();
+    ra.targets = d_tp;
+    ra.draws = d_dd;
+    ra.fstates = d_fs;
+    ra.vout = b.vout.as<float>();
+    ra.order = order;
+    ra.queue = stats + kStatQueue;
+    ra.fragments = (unsigned long long*)(stats + kStatFragments);
+    ra.trace_cycles = x.cfg.record_costs ? b.traces.as<float>() : nullptr;
+    ra.vout_stride = P.stride;
+    ra.tiles = P.tiles;
+    ra.target_count = (uint32_t)P.targets.size();
+    ra.persistent = sched == CPV_SCHED_GRID ? 0u : 1u;
+    ra.hiz_op = P.hiz_op;
+    ra.hiz_depth_kind = P.hiz_kind;
+    uint32_t grid = P.tiles;
+    if (ra.persistent) {
+        const int resident = resident_workgroups(x, rk, block);
+        if (resident <= 0) return CPV_ERROR_DEVICE;
+        grid = std::min<uint32_t>(P.tiles, (uint32_t)resident);
+        fill_u32(x.s, stats + kStatQueue, 0u, 1);
+    }
+    if (grid == 0) return CPV_OK;
+    if (x.r0) HIP_TRY(hipEventRecord(x.r0, x.s));
+    HIP_TRY(launch(rk, dim3(grid), dim3(block), &ra, x.s));
+    if (x.r1) HIP_TRY(hipEventRecord(x.r1, x.s));
+    return CPV_OK;
+}
+
+// Uploads staged descriptors through the context's pinned buffer.
+int upload_staging(cpv_context* c, DevBuf& arena, const Staging& st)
+{
+    int rc = ensure_pinned(c, st.bytes.size());
+    if (rc) return rc;
+    // The previous launch may still read the pinned staging: wait for it.
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(c->pinned, st.bytes.data(), st.bytes.size());
+    HIP_TRY(hipMemcpyAsync(arena.p, c->pinned, st.bytes.size(), hipMemcpyHostToDevice, c->stream));
+    return CPV_OK;
+}
+
+Exec context_exec(cpv_context* c, Buffers* b)
+{
+    Exec x;
+    x.s = c->stream;
+    x.b = b;
+    x.cfg = c->cfg;
+    x.predictor = c->predictor;
+    x.ctx = c;
+    return x;
+}
+
+// One immediate launch on the context: plan, upload, front half, pair-count
+// readback, back half.
+int run_launch(cpv_context* c, const Program& prog, const std::vector<TargetParams>& targets,
+               const std::vector<LaunchDraw>& lds, const std::vector<cpv_pass_desc>& passes,
+               const std::vector<uint32_t>& draw_target)
+{
+    LaunchPlan P;
+    int rc = plan_launch(c->cfg, prog, targets, lds, passes, draw_target, &P);
+    if (rc) return rc;
+    Staging st;
+    stage_plan(P, st);
+    if (c->buf.arena.ensure(st.bytes.size())) return CPV_ERROR_OUT_OF_MEMORY;
+    uint8_t* arena = c->buf.arena.as<uint8_t>();
+    patch_plan(P, st, arena);
+    rc = upload_staging(c, c->buf.arena, st);
+    if (rc) return rc;
+    Exec x = context_exec(c, &c->buf);
+    rc = enqueue_front(x, P, arena);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->readback, c->buf.offsets.as<uint32_t>() + P.prim_total, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint32_t pairs = c->readback[0];
+    x.r0 = c->ev_raster0;
+    x.r1 = c->ev_raster1;
+    rc = enqueue_back(x, P, arena, pairs);
+    if (rc) return rc;
+    c->last_tiles = P.tiles;
+    c->last_waves = c->cfg.waves_per_tile;
+    c->last_tile_size = c->cfg.tile_size;
+    c->last_traces = c->cfg.record_costs != 0;
+    c->last_target_base.clear();
+    for (const TargetParams& t : P.targets) c->last_target_base.push_back(t.tile_base);
+    return CPV_OK;
+}
+
+bool same_program(const cpv_pipeline_desc& a, const cpv_pipeline_desc& b)
+{
+    if (a.program || b.program) return a.program == b.program;
+    return a.vs_kind == b.vs_kind && a.fs_kind == b.fs_kind;
+}
+
+// Groups jobs by shader program: one launch per group, every job its own
+// render target (tile ranges laid end to end).
+struct JobGroup {
+    Program prog;
+    std::vector<TargetParams> targets;
+    std::vector<cpv_pass_desc> passes;
+    std::vector<LaunchDraw> lds;
+    std::vector<uint32_t> dt;
+};
+
+int group_jobs(const cpv_job_desc* jobs, uint32_t job_count, int tile, std::vector<JobGroup>* groups)
+{
+    std::vector<bool> done(job_count, false);
+    for (uint32_t g = 0; g < job_count; g++) {
+        if (done[g]) continue;
+        JobGroup G;
+        int rc = resolve_program(jobs[g].pipeline, &G.prog);
+        if (rc) return rc;
+        uint32_t tile_base = 0;
+        for (uint32_t j = g; j < job_count; j++) {
+            if (done[j] || !same_program(jobs[j].pipeline, jobs[g].pipeline)) continue;
+            done[j] = true;
+            TargetParams t;
+            rc = build_target(jobs[j].pass, true, tile, tile_base, &t);
+            if (rc) return rc;
+            if ((uint64_t)tile_base + (uint64_t)t.tiles_x * t.tiles_y >= (1ull << 31))
+                return fail(CPV_ERROR_UNSUPPORTED, "more than 2^31 tiles in one launch");
+            tile_base += t.tiles_x * t.tiles_y;
+            const uint32_t ti = (uint32_t)G.targets.size();
+            G.targets.push_back(t);
+            G.passes.push_back(jobs[j].pass);
+            for (uint32_t inst = 0; inst < jobs[j].draw.instance_count; inst++) {
+                G.lds.push_back({&jobs[j].pipeline, &jobs[j].draw, inst});
+                G.dt.push_back(ti);
+            }
+        }
+        groups->push_back(std::move(G));
+    }
+    return CPV_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ render calls
+
+extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                               const cpv_draw_desc* draws, const uint32_t* draw_pipeline, uint32_t draw_count)
+{
+    if (!c || !pass || (draw_count && (!pipelines || !draws))) return fail(CPV_ERROR_INVALID, "null argument");
+    if (pass->color_count > CPV_MAX_COLOR_ATTACHMENTS) return fail(CPV_ERROR_UNSUPPORTED, "%u colour attachments", pass->color_count);
+    std::lock_guard<std::mutex> guard(c->lock);
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->buf.stats.ensure(kStatWords * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+    fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
+    HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+    const int tile = (int)c->cfg.tile_size;
+    std::vector<cpv_pass_desc> passes(1, *pass);
+    bool first = true;
+    uint32_t i = 0;
+    do {
+        // a run of draws sharing one shader program
+        std::vector<LaunchDraw> lds;
+        std::vector<uint32_t> dt;
+        Program prog;
+        if (draw_count == 0) {
+            // clear-only pass: an empty launch still initialises every tile
+            if (!builtin_program(CPV_VS_PASSTHROUGH, CPV_FS_COLOR, &prog)) return fail(CPV_ERROR_DEVICE, "built-in table");
+        } else {
+            const uint32_t pi0 = draw_pipeline ? draw_pipeline[i] : 0;
+            int rc = resolve_program(pipelines[pi0], &prog);
+            if (rc) return rc;
+            uint32_t j = i;
+            while (j < draw_count) {
+                const uint32_t pj = draw_pipeline ? draw_pipeline[j] : 0;
+                if (!same_program(pipelines[pj], pipelines[pi0])) break;
+                for (uint32_t inst = 0; inst < draws[j].instance_count; inst++) {
+                    lds.push_back({&pipelines[pj], &draws[j], inst});
+                    dt.push_back(0);
+                }
+                j++;
+            }
+            i = j;
+        }
+        std::vector<TargetParams> targets(1);
+        int rc = build_target(*pass, first, tile, 0, &targets[0]);
+        if (rc) return rc;
+        rc = run_launch(c, prog, targets, lds, passes, dt);
+        if (rc) return rc;
+        first = false;
+    } while (i < draw_count);
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    c->timing_valid = true;
+    c->last_count_exact = c->cfg.count_fragments != 0;
+    return CPV_OK;
+}
+
+extern "C" int cpv_render_jobs(cpv_context* c, const cpv_job_desc* jobs, uint32_t job_count)
+{
+    if (!c || (job_count && !jobs)) return fail(CPV_ERROR_INVALID, "null argument");
+    std::lock_guard<std::mutex> guard(c->lock);
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->buf.stats.ensure(kStatWords * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+    fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
+    HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+    std::vector<JobGroup> groups;
+    int rc = group_jobs(jobs, job_count, (int)c->cfg.tile_size, &groups);
+    if (rc) return rc;
+    for (JobGroup& G : groups) {
+        rc = run_launch(c, G.prog, G.targets, G.lds, G.passes, G.dt);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    c->timing_valid = true;
+    c->last_count_exact = c->cfg.count_fragments != 0;
+    return CPV_OK;
+}
+
+extern "C" int cpv_last_fragment_count(cpv_context* c, uint64_t* fragments)
+{
+    if (!c || !fragments) return fail(CPV_ERROR_INVALID, "null argument");
+    if (!c->last_count_exact) return fail(CPV_ERROR_INVALID, "the last render call ran with count_fragments off");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemcpyAsync(c->readback + 16, c->buf.stats.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(fragments, c->readback + 16, 8);
+    return CPV_OK;
+}
+
+extern "C" int cpv_last_timing(cpv_context* c, float* raster_ms, float* total_ms)
+{
+    if (!c) return fail(CPV_ERROR_INVALID, "null context");
+    if (!c->timing_valid) return fail(CPV_ERROR_INVALID, "no render call recorded");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventSynchronize(c->ev_end));
+    if (raster_ms) HIP_TRY(hipEventElapsedTime(raster_ms, c->ev_raster0, c->ev_raster1));
+    if (total_ms) HIP_TRY(hipEventElapsedTime(total_ms, c->ev_begin, c->ev_end));
+    return CPV_OK;
+}
+
+extern "C" int cpv_get_tile_traces(cpv_context* c, cpv_tile_trace* traces, uint32_t capacity, uint32_t* count)
+{
+    if (!c || !count || (capacity && !traces)) return fail(CPV_ERROR_INVALID, "null argument");
+    *count = 0;
+    if (!c->last_traces) return fail(CPV_ERROR_INVALID, "the last launch ran with record_costs off");
+    HIP_TRY(hipSetDevice(c->device));
+    const uint32_t tiles = c->last_tiles, waves = c->last_waves;
+    *count = tiles;
+    const uint32_t n = std::min(tiles, capacity);
+    if (n == 0) return CPV_OK;
+    std::vector<float> feat((size_t)n * kFeatureStride), cyc((size_t)n * waves);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(feat.data(), c->buf.features.p, feat.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cyc.data(), c->buf.traces.p, cyc.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t t = 0; t < n; t++) {
+        cpv_tile_trace& o = traces[t];
+        std::memset(&o, 0, sizeof(o));
+        for (int k = 0; k < kFeatures; k++) o.features[k] = feat[(size_t)t * kFeatureStride + k];
+        float s = 0.0f;
+        for (uint32_t w = 0; w < waves; w++) s += cyc[(size_t)t * waves + w];
+        o.cycles = s;
+        const auto it = std::upper_bound(c->last_target_base.begin(), c->last_target_base.end(), t);
+        const uint32_t job = (uint32_t)(it - c->last_target_base.begin()) - 1u;
+        o.job = job;
+        o.tile = t - c->last_target_base[job];
+    }
+    return CPV_OK;
+}
+
+// -------------------------------------------------------- recorded batches
+
+struct cpv_batch {
+    cpv_context* c = nullptr;
+    cpv_config cfg{};
+    cpvrt::Buffers b;
+    std::vector<LaunchPlan> plans;
+    std::vector<uint32_t> caps;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool submitted = false;
+};
+
+extern "C" void cpv_batch_destroy(cpv_batch* B)
+{
+    if (!B) return;
+    (void)hipSetDevice(B->c->device);
+    (void)hipStreamSynchronize(B->c->stream);
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->e0) (void)hipEventDestroy(B->e0);
+    if (B->e1) (void)hipEventDestroy(B->e1);
+    B->b.release_all();
+    delete B;
+}
+
+static int batch_build(cpv_context* c, cpv_batch* B, const cpv_job_desc* jobs, uint32_t job_count)
+{
+    HIP_TRY(hipEventCreate(&B->e0));
+    HIP_TRY(hipEventCreate(&B->e1));
+    std::vector<JobGroup> groups;
+    int rc = group_jobs(jobs, job_count, (int)B->cfg.tile_size, &groups);
+    if (rc) return rc;
+    B->plans.resize(groups.size());
+    Staging st;
+    for (size_t g = 0; g < groups.size(); g++) {
+        rc = plan_launch(B->cfg, groups[g].prog, groups[g].targets, groups[g].lds, groups[g].passes, groups[g].dt, &B->plans[g]);
+        if (rc) return rc;
+        stage_plan(B->plans[g], st);
+    }
+    if (B->b.arena.ensure(std::max<size_t>(st.bytes.size(), 256))) return CPV_ERROR_OUT_OF_MEMORY;
+    uint8_t* arena = B->b.arena.as<uint8_t>();
+    for (LaunchPlan& P : B->plans) patch_plan(P, st, arena);
+    rc = upload_staging(c, B->b.arena, st);
+    if (rc) return rc;
+    // sizing run: the front half of every launch, to learn its pair count
+    Exec x = context_exec(c, &B->b);
+    x.cfg = B->cfg;
+    B->caps.resize(B->plans.size());
+    for (size_t g = 0; g < B->plans.size(); g++) {
+        rc = enqueue_front(x, B->plans[g], arena);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(c->readback, B->b.offsets.as<uint32_t>() + B->plans[g].prim_total, 4, hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const uint32_t pairs = c->readback[0];
+        B->caps[g] = pairs + pairs / 8 + 4096;   // room for vertex data that moves a little between submits
+        rc = ensure_back(x, B->plans[g], B->caps[g]);
+        if (rc) return rc;
+        const int ci = tile_config_index(B->cfg.tile_size, B->cfg.waves_per_tile);
+        if (resident_workgroups(x, B->plans[g].prog.raster[ci], 64 * (int)B->cfg.waves_per_tile) <= 0) return CPV_ERROR_DEVICE;
+    }
+    if (B->b.stats.ensure(kStatWords * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // record: every kernel of every launch, no allocation, no host sync
+    x.frozen = true;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    rc = CPV_OK;
+    fill_u32(c->stream, B->b.stats.p, 0u, kStatWords);
+    for (size_t g = 0; g < B->plans.size() && rc == CPV_OK; g++) {
+        rc = enqueue_front(x, B->plans[g], arena);
+        if (rc == CPV_OK) rc = enqueue_back(x, B->plans[g], arena, B->caps[g]);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(CPV_ERROR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    B->graph = graph;
+    HIP_TRY(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
+    return CPV_OK;
+}
+
+extern "C" int cpv_batch_create(cpv_context* c, const cpv_job_desc* jobs, uint32_t job_count, cpv_batch** out)
+{
+    if (!c || !out || (job_count && !jobs)) return fail(CPV_ERROR_INVALID, "null argument");
+    if (job_count == 0) return fail(CPV_ERROR_INVALID, "a batch needs at least one job");
+    std::lock_guard<std::mutex> guard(c->lock);
+    HIP_TRY(hipSetDevice(c->device));
+    cpv_batch* B = new cpv_batch();
+    B->c = c;
+    B->cfg = c->cfg;
+    B->cfg.count_fragments = c->cfg.count_fragments;
+    int rc = batch_build(c, B, jobs, job_count);
+    if (rc) {
+        const std::string msg = cpv_last_error();
+        cpv_batch_destroy(B);
+        return fail(rc, "%s", msg.c_str());
+    }
+    *out = B;
+    return CPV_OK;
+}
+
+extern "C" int cpv_batch_submit(cpv_batch* B)
+{
+    if (!B) return fail(CPV_ERROR_INVALID, "null batch");
+    std::lock_guard<std::mutex> guard(B->c->lock);
+    HIP_TRY(hipSetDevice(B->c->device));
+    HIP_TRY(hipEventRecord(B->e0, B->c->stream));
+    HIP_TRY(hipGraphLaunch(B->exec, B->c->stream));
+    HIP_TRY(hipEventRecord(B->e1, B->c->stream));
+    B->submitted = true;
+    return CPV_OK;
+}
+
+extern "C" int cpv_batch_wait(cpv_batch* B, uint64_t* fragments)
+{
+    if (!B) return fail(CPV_ERROR_INVALID, "null batch");
+    if (!B->submitted) return fail(CPV_ERROR_INVALID, "batch not submitted");
+    HIP_TRY(hipSetDevice(B->c->device));
+    HIP_TRY(hipEventSynchronize(B->e1));
+    uint32_t words[kStatWords];
+    HIP_TRY(hipMemcpy(words, B->b.stats.p, sizeof(words), hipMemcpyDeviceToHost));
+    if (words[kStatOverflow])
+        return fail(CPV_ERROR_OVERFLOW, "the batch's (tile, primitive) pairs exceeded the capacity recorded for it; record it again");
+    if (fragments) {
+        if (!B->cfg.count_fragments) return fail(CPV_ERROR_INVALID, "the batch was recorded with count_fragments off");
+        std::memcpy(fragments, words + kStatFragments, 8);
+    }
+    return CPV_OK;
+}
+
+extern "C" int cpv_batch_timing(cpv_batch* B, float* ms)
+{
+    if (!B || !ms) return fail(CPV_ERROR_INVALID, "null argument");
+    if (!B->submitted) return fail(CPV_ERROR_INVALID, "batch not submitted");
+    HIP_TRY(hipSetDevice(B->c->device));
+    HIP_TRY(hipEventSynchronize(B->e1));
+    HIP_TRY(hipEventElapsedTime(ms, B->e0, B->e1));
+    return CPV_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+
+extern "C" int cpv_autotune(cpv_context* c, const cpv_job_desc* jobs, uint32_t job_count, uint32_t repeats,
+                            cpv_autotune_entry* entries, uint32_t capacity, uint32_t* count, cpv_config* best)
+{
+    if (!c || !count || (capacity && !entries)) return fail(CPV_ERROR_INVALID, "null argument");
+    static const uint32_t configs[7][2] = {{8, 1}, {16, 1}, {16, 2}, {16, 4}, {32, 1}, {32, 2}, {32, 4}};
+    const cpv_config saved = c->cfg;
+    cpv_config best_cfg = saved;
+    float best_ms = -1.0f;
+    uint32_t n = 0;
+    repeats = std::max(repeats, 1u);
+    for (const auto& tc : configs) {
+        cpv_config cfg = saved;
+        cfg.tile_size = tc[0];
+        cfg.waves_per_tile = tc[1];
+        cfg.record_costs = 0;
+        c->cfg = cfg;
+        cpv_batch* B = nullptr;
+        int rc = cpv_batch_create(c, jobs, job_count, &B);
+        if (rc) { c->cfg = saved; return rc; }
+        float total = 0.0f;
+        // one untimed replay, then `repeats` timed ones
+        for (uint32_t r = 0; r <= repeats && rc == CPV_OK; r++) {
+            rc = cpv_batch_submit(B);
+            if (rc == CPV_OK) rc = cpv_batch_wait(B, nullptr);
+            float ms = 0.0f;
+            if (rc == CPV_OK) rc = cpv_batch_timing(B, &ms);
+            if (r > 0) total += ms;
+        }
+        cpv_batch_destroy(B);
+        if (rc) { c->cfg = saved; return rc; }
+        const float ms = total / (float)repeats;
+        if (n < capacity) entries[n] = cpv_autotune_entry{tc[0], tc[1], ms, 0u};
+        n++;
+        if (best_ms < 0.0f || ms < best_ms) { best_ms = ms; best_cfg = cfg; }
+    }
+    c->cfg = saved;
+    best_cfg.record_costs = saved.record_costs;
+    *count = n;
+    if (best) *best = best_cfg;
+    return CPV_OK;
+}
+
+extern "C" int cpv_set_predictor(cpv_context* c, cpv_mlp* m)
+{
+    if (!c) return fail(CPV_ERROR_INVALID, "null context");
+    std::lock_guard<std::mutex> guard(c->lock);
+    c->predictor = m;
+    return CPV_OK;
+}

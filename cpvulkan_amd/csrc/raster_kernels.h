@@ -70,7 +70,7 @@ struct RasterArgs {
     const uint32_t* order;          // queue order of tiles (nullptr: index order)
     uint32_t* queue;                // work counter for the persistent schedule
     unsigned long long* fragments;  // covered (primitive, pixel) pairs
-    float* trace_cycles;            // per tile shader cycles (nullptr: not recorded)
+    float* trace_cycles;            // shader cycles per (tile, wave slot) (nullptr: not recorded)
     uint32_t vout_stride;
     uint32_t tiles;                 // total tiles over all targets
     uint32_t target_count;

@@ -331,7 +331,7 @@ __global__ __launch_bounds__(64 * WAVES) void raster_kernel(RasterArgs a)
         // ---- statistics
         for (int off = 32; off > 0; off >>= 1) nfrag += __shfl_down(nfrag, off, 64);
         if (lane == 0 && nfrag) atomicAdd(a.fragments, nfrag);
-        if (a.trace_cycles && lane == 0) a.trace_cycles[unit] = (float)(clock64() - t_begin);
+        if (a.trace_cycles && lane == 0) a.trace_cycles[(uint64_t)tile * WAVES + slot] = (float)(clock64() - t_begin);
 
         if (!a.persistent) break;
     }

@@ -576,3 +576,40 @@ def grid_scene(n: int = 512, width: int = 3840, height: int = 2160, seed: int = 
                     textures={1: noise_texture(tex_size, seed + 100)})
     return Scene(f"grid{n}", width, height, abi.FORMAT["R8G8B8A8_UNORM"], abi.FORMAT["D32_SFLOAT"], pipeline, [draw],
                  clear_color=(0.0, 0.0, 0.0, 0.0), clear_depth=1.0)
+
+
+def zipf_counts(n: int, rng: np.random.Generator, s: float = 1.2, lo: int = 12, hi: int = 200_000) -> np.ndarray:
+    """Triangle counts ~ Zipf(s) truncated to [lo, hi], by inverse CDF of the
+    continuous power law on [lo, hi + 1), rounded down."""
+    u = rng.random(n)
+    a, b = lo ** (1.0 - s), (hi + 1) ** (1.0 - s)
+    k = np.floor((a + u * (b - a)) ** (1.0 / (1.0 - s))).astype(np.int64)
+    return np.clip(k, lo, hi)
+
+
+def job_scenes(n_jobs: int = 256, seed: int = 4, sides=(256, 512, 1024, 2048, 4096), lo: int = 12,
+               hi: int = 200_000) -> list:
+    """independent render jobs of uneven size.  Triangle
+    counts ~ Zipf(1.2) over [lo, hi]; framebuffer side uniform over `sides`
+    (square); R8G8B8A8 + D32; triangles as soup_scene, from per-job seeds, in
+    each job's own pixel units."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    counts = zipf_counts(n_jobs, rng, lo=lo, hi=hi)
+    side = rng.choice(np.asarray(sides), size=n_jobs)
+    jobs = []
+    for j in range(n_jobs):
+        s = soup_scene(int(counts[j]), int(side[j]), int(side[j]), seed=seed * 1_000_003 + j)
+        s.name = f"job{j}-{int(side[j])}-{int(counts[j])}"
+        jobs.append(s)
+    return jobs
+
+
+def job_desc(m: Materialized) -> abi.JobDesc:
+    """The C-ABI job (one pass, one pipeline, one draw) of a materialised scene."""
+    if len(m.draws) != 1 or len(m.pipelines) != 1:
+        raise ValueError("a job holds one pipeline and one draw")
+    j = abi.JobDesc()
+    j.pass_ = m.pass_desc
+    j.pipeline = m.pipelines[0]
+    j.draw = m.draws[0]
+    return j

@@ -56,7 +56,8 @@ enum {
     CPV_ERROR_UNSUPPORTED = -2,  /* state the reference aborts on, or not built here */
     CPV_ERROR_DEVICE = -3,       /* HIP runtime failure (VK_ERROR_DEVICE_LOST) */
     CPV_ERROR_OUT_OF_MEMORY = -4,
-    CPV_ERROR_COMPILE = -5       /* SPIR-V translation or hiprtc failure */
+    CPV_ERROR_COMPILE = -5,      /* SPIR-V translation or hiprtc failure */
+    CPV_ERROR_OVERFLOW = -6      /* a recorded batch outgrew the capacity sized when it was recorded */
 };
 
 /* Built-in shader programs.  Each mirrors a GLSL shader of the benchmark
@@ -255,6 +256,80 @@ int cpv_last_timing(cpv_context* ctx, float* raster_ms, float* total_ms);
  * record_costs set; *count receives the number available. */
 int cpv_get_tile_traces(cpv_context* ctx, cpv_tile_trace* traces, uint32_t capacity, uint32_t* count);
 
+/* ---- recorded batches ----
+ * A batch is a set of render jobs recorded once and replayed many times, the
+ * way a Vulkan command buffer is recorded by vkCmdDraw* and replayed by every
+ * vkQueueSubmit (CPVulkan/CommandBuffer.Draw.cpp:2350-2360, Queue.cpp:52-74).
+ * Recording plans every launch, uploads the descriptors, runs the vertex
+ * and setup stages once to size the (tile, primitive) pair arrays, and
+ * captures every kernel of the frame -- all jobs, one launch per shader
+ * program -- into one hipGraph; it uses the context's configuration at
+ * recording time (tile size, waves per tile, scheduler, predictor).  A
+ * submit replays the graph on the context's stream without host work.
+ * Vertex and index data may change between submits; if a replay needs more
+ * pairs than were recorded, the overflow is detected and cpv_batch_wait
+ * reports CPV_ERROR_OVERFLOW (record the batch again). */
+typedef struct cpv_batch cpv_batch;
+int cpv_batch_create(cpv_context* ctx, const cpv_job_desc* jobs, uint32_t job_count, cpv_batch** out);
+int cpv_batch_submit(cpv_batch* batch);
+/* Waits for the last submit; *fragments (optional) receives its covered
+ * fragment count when the batch was recorded with count_fragments on. */
+int cpv_batch_wait(cpv_batch* batch, uint64_t* fragments);
+/* Device time of the last submit in milliseconds (HIP events around the graph). */
+int cpv_batch_timing(cpv_batch* batch, float* ms);
+void cpv_batch_destroy(cpv_batch* batch);
+
+/* ---- autotuner ----
+ * Records the jobs as a batch under every tile configuration (tile size 8,
+ * 16, 32 x waves per tile 1, 2, 4 where the tile has that many 8x8 blocks),
+ * replays each `repeats` times after one warm-up replay, and returns the
+ * mean device time per configuration and the fastest configuration (the
+ * context's configuration with tile_size / waves_per_tile replaced).  The
+ * context's configuration is left unchanged. */
+typedef struct cpv_autotune_entry {
+    uint32_t tile_size;
+    uint32_t waves_per_tile;
+    float ms;
+    uint32_t reserved;
+} cpv_autotune_entry;
+int cpv_autotune(cpv_context* ctx, const cpv_job_desc* jobs, uint32_t job_count, uint32_t repeats,
+                 cpv_autotune_entry* entries, uint32_t capacity, uint32_t* count, cpv_config* best);
+
+/* ---- tile-cost predictor (CPV_SCHED_PREDICTED) ----
+ * A BF16 MLP (12 tile features -> 64 -> 64 -> 1, ReLU, FP32 accumulate) that
+ * predicts log2 of a tile's raster cycles from its binning statistics.  It
+ * is trained on the device by hand-written MFMA forward and backward
+ * kernels on cost traces the raster kernel records (cpv_get_tile_traces).
+ * The persistent raster kernel takes tiles longest-predicted first; the
+ * prediction changes only the order in which tiles are taken, never a
+ * pixel.
+ *
+ * Parameters, fp32, in this order (CPV_MLP_PARAMS floats):
+ *   W1[12][64], b1[64], W2[64][64], b2[64], w3[64], b3
+ * with h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), y = h2 . w3 + b3. */
+#define CPV_MLP_FEATURES 12
+#define CPV_MLP_HIDDEN 64
+#define CPV_MLP_PARAMS (12 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+typedef struct cpv_mlp cpv_mlp;
+int cpv_mlp_create(cpv_context* ctx, uint64_t seed, cpv_mlp** out);
+void cpv_mlp_destroy(cpv_mlp* mlp);
+int cpv_mlp_get_params(cpv_mlp* mlp, float* params, uint32_t count);
+int cpv_mlp_set_params(cpv_mlp* mlp, const float* params, uint32_t count);
+/* y[rows] = network(x[rows][12]) (host arrays). */
+int cpv_mlp_predict(cpv_mlp* mlp, const float* x, uint32_t rows, float* y);
+/* One forward and backward pass over rows (host arrays): the gradient of
+ * the mean squared error mean((y - target)^2) in the parameter order above,
+ * and the loss. */
+int cpv_mlp_gradients(cpv_mlp* mlp, const float* x, const float* target, uint32_t rows, float* grads, float* loss);
+/* Adam training on the device: `steps` minibatches of `batch` rows drawn
+ * with a seeded generator from (x, target); *loss receives the mean squared
+ * error over all rows after training. */
+int cpv_mlp_train(cpv_mlp* mlp, const float* x, const float* target, uint32_t rows, uint32_t steps, uint32_t batch,
+                  float learning_rate, uint64_t seed, float* loss);
+/* The predictor CPV_SCHED_PREDICTED uses (NULL: the scheduler falls back to
+ * the counter-ordered queue). */
+int cpv_set_predictor(cpv_context* ctx, cpv_mlp* mlp);
+
 /* ---- shader programs ---- */
 int cpv_program_create_spirv(cpv_context* ctx,
                              const uint32_t* vs_words, size_t vs_word_count, const char* vs_entry,
@@ -265,7 +340,7 @@ void cpv_program_destroy(cpv_context* ctx, cpv_program* program);
 const char* cpv_program_source(cpv_program* program);
 
 /* sizeof() of the descriptor structs, in the order: pipeline, draw, pass,
- * attachment, texture, sampler, job, config, tile trace.  Lets bindings in
+ * attachment, texture, sampler, job, config, tile trace, autotune entry.  Lets bindings in
  * other languages check their struct layouts.  Returns the count. */
 int cpv_abi_sizes(uint32_t* out, uint32_t capacity);
 

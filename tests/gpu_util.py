@@ -53,3 +53,20 @@ def compare(scene: Scene, gpu, ref, label: str = "") -> None:
         raise AssertionError(f"{label} depth/stencil differs in {diff.size} bytes (first at {diff[:5]})")
     if gf is not None:
         assert gf == rf, f"{label} fragment count {gf} != oracle {rf}"
+
+
+def materialize_jobs(ctx, scenes):
+    """Device copies of the job scenes and their C-ABI job descriptors."""
+    from cpvulkan_amd.workloads import job_desc
+    mem = DeviceMemory(ctx)
+    ms = [materialize(s, mem) for s in scenes]
+    return mem, ms, [job_desc(m) for m in ms]
+
+
+def oracle_jobs(scenes):
+    return [oracle_lib.render_scene(s) for s in scenes]
+
+
+def compare_jobs(scenes, ms, refs, label=""):
+    for s, m, ref in zip(scenes, ms, refs):
+        compare(s, (m.read_color(), m.read_ds(), None), ref, f"{label} {s.name}")

@@ -1,0 +1,158 @@
+"""Scheduling and batching on the GPU: every scheduler (grid, counter queue,
+predicted queue), the recorded hipGraph batch, replays, the autotuner and
+the pair-capacity overflow check leave every pixel bit-identical to the CPU
+oracle."""
+import copy
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from gpu_util import compare, compare_jobs, materialize_jobs, oracle_jobs, render_gpu
+import cpvulkan_amd
+from cpvulkan_amd import abi
+from cpvulkan_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+SCHED = abi.SCHED
+
+
+def _cfg(ctx, **kw):
+    cfg = ctx.config()
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    ctx.apply_config(cfg)
+
+
+@pytest.fixture
+def ctx(gpu_ctx):
+    saved = gpu_ctx.config()
+    yield gpu_ctx
+    gpu_ctx.apply_config(saved)
+    gpu_ctx.set_predictor(None)
+
+
+@pytest.mark.parametrize("sched", ["COUNTER", "PREDICTED"])
+@pytest.mark.parametrize("tile,waves", [(16, 4), (8, 1), (32, 2)])
+def test_schedulers_bit_exact(ctx, sched, tile, waves):
+    scene = W.skewed_scene(40_000, 640, 360, seed=7)
+    ref = oracle_lib.render_scene(scene)
+    mlp = None
+    if sched == "PREDICTED":
+        mlp = cpvulkan_amd.Mlp(ctx, seed=3)     # untrained: any order must give the same image
+        ctx.set_predictor(mlp)
+    _cfg(ctx, tile_size=tile, waves_per_tile=waves, scheduler=SCHED[sched])
+    print(f"[sched] {sched} tile={tile} waves={waves}", flush=True)
+    try:
+        compare(scene, render_gpu(ctx, scene), ref, f"{sched} {tile}x{waves}")
+        compare(scene, render_gpu(ctx, scene, count=False), ref[:2] + (None,), f"{sched} {tile}x{waves} hiz")
+    finally:
+        ctx.set_predictor(None)
+        if mlp:
+            mlp.close()
+
+
+def test_predicted_without_model_falls_back(ctx):
+    scene = W.soup_scene(5000, 320, 180, seed=9)
+    ref = oracle_lib.render_scene(scene)
+    _cfg(ctx, scheduler=SCHED["PREDICTED"])
+    compare(scene, render_gpu(ctx, scene), ref, "predicted, no model")
+
+
+def test_tile_traces(ctx):
+    scene = W.skewed_scene(30_000, 480, 270, seed=2)
+    _cfg(ctx, tile_size=16, waves_per_tile=4, record_costs=1)
+    render_gpu(ctx, scene)
+    tr = ctx.tile_traces()
+    tiles = ((480 + 15) // 16) * ((270 + 15) // 16)
+    assert tr["features"].shape == (tiles, 12)
+    assert np.all(tr["cycles"] > 0)
+    assert np.array_equal(tr["tile"], np.arange(tiles))
+    f = tr["features"]
+    assert np.all(np.isfinite(f))
+    assert np.allclose(f[:, 7], 0.5) and np.allclose(f[:, 8], 1.0)
+    # busier tiles cost more: rank correlation of primitive count and cycles
+    rc = np.corrcoef(np.argsort(np.argsort(f[:, 0])), np.argsort(np.argsort(tr["cycles"])))[0, 1]
+    print(f"[sched] traces: {tiles} tiles, rank corr(n, cycles) = {rc:.3f}", flush=True)
+    assert rc > 0.5
+
+
+def _jobs(n=10, seed=11):
+    return W.job_scenes(n, seed=seed, sides=(64, 128, 256, 200), lo=12, hi=4000)
+
+
+@pytest.mark.parametrize("sched", ["GRID", "COUNTER"])
+def test_batch_replays_bit_exact(ctx, sched):
+    scenes = _jobs()
+    refs = oracle_jobs(scenes)
+    _cfg(ctx, scheduler=SCHED[sched])
+    mem, ms, jobs = materialize_jobs(ctx, scenes)
+    try:
+        total_ref = sum(r[2] for r in refs)
+        with ctx.batch(jobs) as b:
+            for rep in range(3):
+                for m in ms:
+                    m.reset_attachments()   # replays must not depend on what the last one left
+                b.submit()
+                n = b.wait(count=True)
+                print(f"[batch] {sched} replay {rep}: {n} fragments, {b.timing():.3f} ms", flush=True)
+                assert n == total_ref
+                compare_jobs(scenes, ms, refs, f"batch {sched} replay {rep}")
+    finally:
+        mem.release()
+
+
+def test_render_jobs_immediate(ctx):
+    scenes = _jobs(6, seed=5)
+    refs = oracle_jobs(scenes)
+    mem, ms, jobs = materialize_jobs(ctx, scenes)
+    try:
+        ctx.render_jobs(jobs)
+        ctx.synchronize()
+        assert ctx.fragment_count() == sum(r[2] for r in refs)
+        compare_jobs(scenes, ms, refs, "render_jobs")
+    finally:
+        mem.release()
+
+
+def test_batch_overflow_detected(ctx):
+    """Vertex data that grows after recording must be reported, not drawn
+    out of bounds: the emit kernels drop pairs past the recorded capacity."""
+    scenes = _jobs(4, seed=8)
+    mem, ms, jobs = materialize_jobs(ctx, scenes)
+    try:
+        with ctx.batch(jobs) as b:
+            b.submit()
+            b.wait()
+            # grow every triangle 50x about its centroid: many more (tile, primitive) pairs
+            for s, m in zip(scenes, ms):
+                v = s.draws[0].vertices[0].view(np.float32).reshape(-1, 3, 8).copy()
+                c = v[:, :, 0:2].mean(axis=1, keepdims=True)
+                v[:, :, 0:2] = c + (v[:, :, 0:2] - c) * 50.0
+                mem.upload(m.draws[0].vertex_address[0], v)
+            b.submit()
+            with pytest.raises(cpvulkan_amd.CpvError, match="exceeded the capacity"):
+                b.wait()
+    finally:
+        mem.release()
+
+
+def test_autotune(ctx):
+    scenes = _jobs(8, seed=12)
+    refs = oracle_jobs(scenes)
+    mem, ms, jobs = materialize_jobs(ctx, scenes)
+    try:
+        table, best = ctx.autotune(jobs, repeats=2)
+        print(f"[autotune] {table} -> {best.tile_size}x{best.waves_per_tile}", flush=True)
+        assert len(table) == 7 and all(t[2] > 0 for t in table)
+        assert (best.tile_size, best.waves_per_tile) == min(table, key=lambda t: t[2])[:2]
+        ctx.apply_config(best)
+        with ctx.batch(jobs) as b:
+            for m in ms:
+                m.reset_attachments()
+            b.submit()
+            b.wait()
+        compare_jobs(scenes, ms, refs, "autotuned batch")
+    finally:
+        mem.release()
