@@ -1,6 +1,6 @@
 """Draw-path benchmark: frames/s of one workload on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload soup|cube|grid|skewed]
+    python bench.py [--steps K] [--warmup W] [--workload soup|cube|grid|skewed|jobs]
                     [--cpu-baseline full|sample|none] [--no-check]
 
 A step is one frame: the render pass of the workload (clear, vertex stage,
@@ -8,9 +8,21 @@ setup, binning, raster/fragment) over inputs already resident in HBM.  The
 default workload is the 1920x1080, 1M-triangle alpha-blended soup
 (R8G8B8A8_UNORM + D32_SFLOAT, LESS, SRC_ALPHA/ONE_MINUS_SRC_ALPHA).
 
-Before timing, the GPU frame is compared byte for byte with the CPU oracle's
-frame; the oracle's single-threaded wall time for that frame is the
-reported CPU baseline.  Prints one JSON line.
+  skewed  3840x2160, 4M triangles from 8 Gaussian clusters, alpha blended:
+          the plain single-draw path (hardware-ordered tile grid, immediate
+          render call) against the batched renderer (the frame recorded as
+          one hipGraph, persistent tile queue ordered by the BF16 tile-cost
+          network trained on the device on traces of the soup, grid and job
+          workloads).  `value` is the batched renderer's frames/s.
+  jobs    256 independent render jobs (Zipf triangle counts, 256..4096
+          square targets) recorded as one hipGraph batch after the
+          autotuner picked the tile configuration; a step is one replay of
+          all jobs.
+
+Before timing, the GPU frames are compared byte for byte with the CPU
+oracle's; the oracle's single-threaded wall time is the reported CPU
+baseline.  The dominant kernel's duration comes from HIP events on the
+stream it runs on.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -32,19 +44,6 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_scene(name: str):
-    from cpvulkan_amd import workloads as W
-    if name == "soup":
-        return W.soup_scene(1_000_000, 1920, 1080, seed=2), "soup-1080p-1M-blend"
-    if name == "cube":
-        return W.cube_scene(), "cube-500x500"
-    if name == "grid":
-        return W.grid_scene(512, 3840, 2160, seed=3), "grid-4k-512x512-textured"
-    if name == "skewed":
-        return W.skewed_scene(4_000_000, 3840, 2160, seed=5), "skewed-4k-4M-blend"
-    raise SystemExit(f"unknown workload {name}")
-
-
 def fmt_name(code: int) -> str:
     from cpvulkan_amd import abi
     return next((k for k, v in abi.FORMAT.items() if v == code), str(code))
@@ -58,6 +57,284 @@ def cpu_model() -> str:
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+def make_scene(name: str):
+    from cpvulkan_amd import workloads as W
+    if name == "soup":
+        return W.soup_scene(1_000_000, 1920, 1080, seed=2), "soup-1080p-1M-blend"
+    if name == "cube":
+        return W.cube_scene(), "cube-500x500"
+    if name == "grid":
+        return W.grid_scene(512, 3840, 2160, seed=3), "grid-4k-512x512-textured"
+    if name == "skewed":
+        return W.skewed_scene(4_000_000, 3840, 2160, seed=5), "skewed-4k-4M-blend"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def geometry_bytes(scene) -> int:
+    """Vertex, index, uniform and sampled-texture bytes a frame reads once."""
+    n = 0
+    for d in scene.draws:
+        for v in d.vertices.values():
+            n += v.nbytes
+        if d.indices is not None:
+            n += d.indices.nbytes
+        for u in d.uniforms.values():
+            n += u.nbytes
+        for t in d.textures.values():
+            n += t.width * t.height * 4      # level 0: the only level implicit LOD reads
+    return n
+
+
+def oracle_frames(scenes, mode: str):
+    """Oracle render of each scene: (results or None per scene, seconds).
+    mode "sample" renders the first 50k triangles of each scene and scales
+    the time by the triangle count (no image to compare)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+    from cpvulkan_amd.workloads import HostMemory, materialize
+    out, t_total = [], 0.0
+    for s in scenes:
+        hm = materialize(s, HostMemory())
+        if mode == "sample":
+            n = min(s.primitive_count, 50_000)
+            t0 = time.perf_counter()
+            oracle_lib.render(hm, (0, n))
+            t_total += (time.perf_counter() - t0) * s.primitive_count / max(n, 1)
+            out.append(None)
+        else:
+            t0 = time.perf_counter()
+            frags = oracle_lib.render(hm)
+            t_total += time.perf_counter() - t0
+            out.append((hm.read_color(), hm.read_ds(), frags))
+    return out, t_total
+
+
+def same(a, b) -> bool:
+    import numpy as np
+    return (np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+            and (a[2] is None or b[2] is None or a[2] == b[2]))
+
+
+def roofline(kernel_ms: float, alg_bytes: int) -> dict:
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None)
+
+
+def cpu_entry(dt: float, mode: str, what: str) -> dict:
+    note = f"{what} single-threaded, {dt:.2f} s" if mode == "full" else \
+        f"{what}: first 50000 triangles per frame, time scaled by triangle count"
+    return dict(value=round(1.0 / dt, 5), unit="frames/s", cores=1, kind="port", sample=f"{note}; {cpu_model()}")
+
+
+def bench_single(args, ctx, scene, workload):
+    import numpy as np
+    from cpvulkan_amd.workloads import DeviceMemory, materialize
+    mem = DeviceMemory(ctx)
+    m = materialize(scene, mem)
+    # one frame with exact fragment counting (the count is a statistic; it
+    # forces coverage evaluation of every pixel), then one frame the way the
+    # timed frames run (counting off, hierarchical depth culling on); both
+    # go through the correctness gate
+    ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves)
+    ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
+    ctx.synchronize()
+    fragments = ctx.fragment_count()
+    counted = (m.read_color(), m.read_ds(), fragments)
+    ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, count_fragments=0)
+    m.reset_attachments()
+    ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
+    ctx.synchronize()
+    fast = (m.read_color(), m.read_ds(), None)
+    log(f"gpu frames done: {fragments} fragments")
+    cpu, check = None, "skipped"
+    if args.cpu_baseline != "none" or not args.no_check:
+        mode = "sample" if args.cpu_baseline == "sample" else "full"
+        refs, dt = oracle_frames([scene], mode)
+        cpu = cpu_entry(dt, mode, f"one frame ({scene.primitive_count} triangles)")
+        if refs[0] is not None and not args.no_check:
+            ok = same(counted, refs[0]) and same(fast, refs[0])
+            check = "bit-exact" if ok else "MISMATCH"
+            log(f"oracle frame {dt:.2f} s, check {check}")
+            if not ok:
+                return None, dict(error="GPU frame differs from the CPU oracle")
+    for _ in range(args.warmup):
+        ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
+    ctx.synchronize()
+    raster_ms, total_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
+        r, t = ctx.timing()
+        raster_ms.append(r)
+        total_ms.append(t)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    fps = args.steps / wall
+    raster_avg = float(np.mean(raster_ms))
+    b_fb = scene.color_bytes + scene.ds_bytes
+    out = dict(
+        value=round(fps, 3), ms_per_step=round(wall * 1000.0 / args.steps, 4),
+        config=dict(workload=workload, width=scene.width, height=scene.height, triangles=scene.primitive_count,
+                    tile=args.tile, waves_per_tile=args.waves, color=fmt_name(scene.color_format),
+                    depth=fmt_name(scene.ds_format)),
+        fragments_per_frame=fragments, fragments_per_s=round(fragments * fps, 1),
+        raster_ms=round(raster_avg, 4), frame_gpu_ms=round(float(np.mean(total_ms)), 4),
+        bytes_per_frame=dict(framebuffer=b_fb, geometry=geometry_bytes(scene)),
+        roofline=roofline(raster_avg, b_fb), cpu_baseline=cpu, check=check,
+    )
+    mem.release()
+    return out, None
+
+
+def train_for_skew(ctx):
+    """Traces of the soup, grid and job workloads -> a device-trained predictor."""
+    from cpvulkan_amd import predictor as PR
+    from cpvulkan_amd import workloads as W
+    from cpvulkan_amd.workloads import DeviceMemory, job_desc, materialize
+    mem = DeviceMemory(ctx)
+    soup = materialize(W.soup_scene(1_000_000, 1920, 1080, seed=2), mem)
+    grid = materialize(W.grid_scene(), mem)
+    jobs = [job_desc(materialize(s, mem)) for s in W.job_scenes(64, seed=4)]
+    traces = PR.collect_traces(ctx, passes=[soup, grid], job_sets=[jobs], configs=[(16, 4, 0), (16, 2, 0), (8, 1, 0)])
+    mlp, report = PR.train_predictor(ctx, traces, samples=10_000, steps=3000)
+    mem.release()
+    report["traces"] = int(len(traces["cycles"]))
+    return mlp, report
+
+
+def timed(ctx, fn, steps: int, warmup: int) -> float:
+    for _ in range(warmup):
+        fn()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    ctx.synchronize()
+    return steps / (time.perf_counter() - t0)
+
+
+def bench_skewed(args, ctx, scene, workload):
+    from cpvulkan_amd import abi
+    from cpvulkan_amd.workloads import DeviceMemory, job_desc, materialize
+    mlp, report = train_for_skew(ctx)
+    log(f"predictor trained: {report}")
+    mem = DeviceMemory(ctx)
+    m = materialize(scene, mem)
+    cpu, check, refs = None, "skipped", None
+    if args.cpu_baseline != "none" or not args.no_check:
+        mode = "sample" if args.cpu_baseline == "sample" else "full"
+        refs, dt = oracle_frames([scene], mode)
+        cpu = cpu_entry(dt, mode, f"one frame ({scene.primitive_count} triangles)")
+    # plain single-draw path
+    ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, scheduler=abi.SCHED["GRID"], count_fragments=0)
+    m.reset_attachments()
+    ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
+    ctx.synchronize()
+    plain_img = (m.read_color(), m.read_ds(), None)
+    plain_fps = timed(ctx, lambda: ctx.render_pass(m.pass_desc, m.pipelines, m.draws), args.steps, args.warmup)
+    plain_raster = ctx.timing()[0]
+    # batched renderer: the frame as one hipGraph, predicted persistent queue
+    ctx.set_predictor(mlp)
+    ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, scheduler=abi.SCHED["PREDICTED"], count_fragments=0)
+    batch = ctx.batch([job_desc(m)])
+    m.reset_attachments()
+    batch.submit()
+    batch.wait()
+    sched_img = (m.read_color(), m.read_ds(), None)
+    sched_fps = timed(ctx, batch.submit, args.steps, args.warmup)
+    batch.wait()
+    sched_ms = batch.timing()
+    batch.close()
+    # the counter-ordered queue in the same batch form: the judge of the learned order
+    ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, scheduler=abi.SCHED["COUNTER"], count_fragments=0)
+    cbatch = ctx.batch([job_desc(m)])
+    counter_fps = timed(ctx, cbatch.submit, args.steps, args.warmup)
+    cbatch.wait()
+    cbatch.close()
+    ctx.set_predictor(None)
+    mlp.close()
+    if refs is not None and refs[0] is not None and not args.no_check:
+        ok = same(plain_img, refs[0]) and same(sched_img, refs[0])
+        check = "bit-exact" if ok else "MISMATCH"
+        if not ok:
+            return None, dict(error="GPU frame differs from the CPU oracle")
+    else:
+        check = "predicted-queue frame identical to plain frame" if same(plain_img, sched_img) else "MISMATCH"
+        if check == "MISMATCH":
+            return None, dict(error="predicted-queue frame differs from the plain frame")
+    b_fb = scene.color_bytes + scene.ds_bytes
+    out = dict(
+        value=round(sched_fps, 3), ms_per_step=round(1000.0 / sched_fps, 4),
+        config=dict(workload=workload, width=scene.width, height=scene.height, triangles=scene.primitive_count,
+                    tile=args.tile, waves_per_tile=args.waves, scheduler="predicted", batch="hipgraph"),
+        plain_single_draw_fps=round(plain_fps, 3), counter_queue_batch_fps=round(counter_fps, 3),
+        speedup_vs_plain=round(sched_fps / plain_fps, 4), predictor=report,
+        plain_raster_ms=round(plain_raster, 4), batch_gpu_ms=round(sched_ms, 4),
+        roofline=roofline(plain_raster, b_fb), cpu_baseline=cpu, check=check,
+    )
+    mem.release()
+    return out, None
+
+
+def bench_jobs(args, ctx):
+    import numpy as np
+    from cpvulkan_amd import workloads as W
+    from cpvulkan_amd.workloads import DeviceMemory, job_desc, materialize
+    scenes = W.job_scenes(256, seed=4)
+    mem = DeviceMemory(ctx)
+    ms = [materialize(s, mem) for s in scenes]
+    jobs = [job_desc(m) for m in ms]
+    tris = sum(s.primitive_count for s in scenes)
+    table, best = ctx.autotune(jobs, repeats=2)
+    log(f"autotune: {table} -> {best.tile_size}x{best.waves_per_tile}")
+    best.count_fragments = 1
+    ctx.apply_config(best)
+    batch = ctx.batch(jobs)
+    for m in ms:
+        m.reset_attachments()
+    batch.submit()
+    fragments = batch.wait(count=True)
+    batch.close()
+    cpu, check = None, "skipped"
+    if args.cpu_baseline != "none" or not args.no_check:
+        mode = "sample" if args.cpu_baseline == "sample" else "full"
+        refs, dt = oracle_frames(scenes, mode)
+        cpu = cpu_entry(dt, mode, f"all {len(scenes)} jobs ({tris} triangles)")
+        if refs[0] is not None and not args.no_check:
+            ok = all(same((m.read_color(), m.read_ds(), None), r) for m, r in zip(ms, refs))
+            ok = ok and fragments == sum(r[2] for r in refs)
+            check = "bit-exact" if ok else "MISMATCH"
+            if not ok:
+                return None, dict(error="GPU frame differs from the CPU oracle")
+    best.count_fragments = 0
+    ctx.apply_config(best)
+    batch = ctx.batch(jobs)
+    for _ in range(args.warmup):
+        batch.submit()
+    batch.wait()
+    gpu_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.submit()
+        batch.wait()
+        gpu_ms.append(batch.timing())
+    wall = time.perf_counter() - t0
+    fps = args.steps / wall
+    batch.close()
+    b_fb = sum(s.color_bytes + s.ds_bytes for s in scenes)
+    out = dict(
+        value=round(fps, 3), ms_per_step=round(wall * 1000.0 / args.steps, 4),
+        config=dict(workload="jobs-256-zipf", jobs=len(scenes), triangles=tris, tile=best.tile_size,
+                    waves_per_tile=best.waves_per_tile, batch="hipgraph", autotune=[list(t) for t in table]),
+        fragments_per_frame=fragments, fragments_per_s=round(fragments * fps, 1),
+        batch_gpu_ms=round(float(np.mean(gpu_ms)), 4),
+        roofline=roofline(float(np.mean(gpu_ms)), b_fb), cpu_baseline=cpu, check=check,
+    )
+    mem.release()
+    return out, None
 
 
 def main() -> int:
@@ -76,9 +353,9 @@ def main() -> int:
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the draw path runs on a single GPU (use --gpus 1)", file=sys.stderr)
         return 2
+    import cpvulkan_amd
+    ctx = cpvulkan_amd.Context(0)
     if args.copy_calibration:
-        import cpvulkan_amd
-        ctx = cpvulkan_amd.Context(0)
         n = args.copy_calibration << 20
         src, dst = ctx.alloc(n), ctx.alloc(n)
         for _ in range(3):
@@ -87,104 +364,23 @@ def main() -> int:
         print(json.dumps(dict(copy_bytes=n, launches=3)))
         ctx.close()
         return 0
-
-    import numpy as np
-
-    import cpvulkan_amd
-    from cpvulkan_amd import abi
-    from cpvulkan_amd.workloads import DeviceMemory, HostMemory, materialize
-
-    scene, workload = make_scene(args.workload)
-    log(f"workload {workload}: {scene.primitive_count} triangles, {scene.width}x{scene.height}")
-
-    ctx = cpvulkan_amd.Context(0)
-    ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves)
-    mem = DeviceMemory(ctx)
-    m = materialize(scene, mem)
-
-    # one frame with exact fragment counting (the count is a statistic; it
-    # forces coverage evaluation of every pixel), then one frame the way the
-    # timed frames run (counting off, hierarchical depth culling on); both
-    # go through the correctness gate
-    ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
-    ctx.synchronize()
-    fragments = ctx.fragment_count()
-    gpu_color, gpu_ds = m.read_color(), m.read_ds()
-    ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, count_fragments=0)
-    m.reset_attachments()
-    ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
-    ctx.synchronize()
-    fast_color, fast_ds = m.read_color(), m.read_ds()
-    log(f"gpu frames done: {fragments} fragments")
-
-    cpu = None
-    check = "skipped"
-    if args.cpu_baseline != "none" or not args.no_check:
-        sys.path.insert(0, str(ROOT / "tests"))
-        import oracle_lib
-        hm = materialize(scene, HostMemory())
-        prims = scene.primitive_count
-        if args.cpu_baseline == "sample":
-            n = min(prims, 50_000)
-            t0 = time.perf_counter()
-            oracle_lib.render(hm, (0, n))
-            dt = time.perf_counter() - t0
-            frame_s = dt * prims / n
-            cpu = dict(value=1.0 / frame_s, unit="frames/s", cores=1, kind="port",
-                       sample=f"first {n} of {prims} triangles, frame time scaled by triangle count; "
-                              f"{cpu_model()}")
+    if args.workload == "jobs":
+        out, err = bench_jobs(args, ctx)
+    else:
+        scene, workload = make_scene(args.workload)
+        log(f"workload {workload}: {scene.primitive_count} triangles, {scene.width}x{scene.height}")
+        if args.workload == "skewed":
+            out, err = bench_skewed(args, ctx, scene, workload)
         else:
-            t0 = time.perf_counter()
-            ref_frags = oracle_lib.render(hm)
-            dt = time.perf_counter() - t0
-            cpu = dict(value=1.0 / dt, unit="frames/s", cores=1, kind="port",
-                       sample=f"one full frame ({prims} triangles) single-threaded, {dt:.2f} s; {cpu_model()}")
-            if not args.no_check:
-                ref_color, ref_ds = hm.read_color(), hm.read_ds()
-                same = (np.array_equal(ref_color, gpu_color) and np.array_equal(ref_ds, gpu_ds)
-                        and np.array_equal(ref_color, fast_color) and np.array_equal(ref_ds, fast_ds)
-                        and ref_frags == fragments)
-                check = "bit-exact" if same else "MISMATCH"
-                log(f"oracle frame {dt:.2f} s, check {check}")
-                if not same:
-                    print(json.dumps(dict(metric="frames/s", error="GPU frame differs from the CPU oracle")))
-                    return 1
-
-    # timed frames
-    for _ in range(args.warmup):
-        ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
-    ctx.synchronize()
-    raster_ms = []
-    total_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
-        r, t = ctx.timing()
-        raster_ms.append(r)
-        total_ms.append(t)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1000.0 / args.steps
-    fps = args.steps / wall
-
-    b_fb = scene.color_bytes + scene.ds_bytes
-    raster_avg = float(np.mean(raster_ms))
-    achieved = b_fb / (raster_avg * 1e-3) / 1e9
-    out = dict(
-        metric="frames/s", value=round(fps, 3), unit="frames/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
-        ms_per_step=round(ms_per_step, 4), higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f32", data="synthetic",
-        config=dict(workload=workload, width=scene.width, height=scene.height, triangles=scene.primitive_count,
-                    tile=args.tile, waves_per_tile=args.waves, color=fmt_name(scene.color_format),
-                    depth=fmt_name(scene.ds_format)),
-        fragments_per_frame=fragments, fragments_per_s=round(fragments * fps, 1),
-        raster_ms=round(raster_avg, 4), frame_gpu_ms=round(float(np.mean(total_ms)), 4),
-        roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None),
-        cpu_baseline=cpu, check=check,
-    )
-    print(json.dumps(out), flush=True)
-    mem.release()
+            out, err = bench_single(args, ctx, scene, workload)
+    if err:
+        print(json.dumps(dict(metric="frames/s", **err)), flush=True)
+        return 1
+    line = dict(metric="frames/s", value=out.pop("value"), unit="frames/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=out.pop("ms_per_step"), higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype="f32", data="synthetic", config=out.pop("config"))
+    line.update(out)
+    print(json.dumps(line), flush=True)
     ctx.close()
     return 0
 

@@ -177,13 +177,20 @@ __global__ __launch_bounds__(64 * WAVES) void raster_kernel(RasterArgs a)
             for (int kk = 0; kk < BPW; kk++)
                 if (kk == k) px[kk] = ps;
         };
-        auto drain = [&]() {
+        // Shades the oldest queued entry of every lane that has one and
+        // shifts the queues: called when some lane's queue is full, this
+        // keeps the other lanes' queues filled, so each shading pass runs
+        // with as many lanes as possible.
+        auto drain_one = [&]() {
+            if (qn > 0) shade(q_prim[0] & 0x0FFFFFFFu, (int)(q_prim[0] >> 28), q_w0[0], q_w1[0], q_w2[0]);
 #pragma unroll
-            for (int e = 0; e < Q; e++) {
-                if (!__any(e < qn)) break;
-                if (e < qn) shade(q_prim[e] & 0x0FFFFFFFu, (int)(q_prim[e] >> 28), q_w0[e], q_w1[e], q_w2[e]);
+            for (int e = 0; e + 1 < Q; e++) {
+                q_prim[e] = q_prim[e + 1];
+                q_w0[e] = q_w0[e + 1];
+                q_w1[e] = q_w1[e + 1];
+                q_w2[e] = q_w2[e + 1];
             }
-            qn = 0;
+            qn = qn > 0 ? qn - 1 : 0;
         };
 
         // ---- walk the tile's primitives in submission order
@@ -254,7 +261,7 @@ __global__ __launch_bounds__(64 * WAVES) void raster_kernel(RasterArgs a)
                 for (int k = 0; k < BPW; k++) {
                     // every lane is active here; a lane queues at most one
                     // entry per iteration, so a free slot always exists
-                    if (!FS::kMayDiscard && __any(qn == Q)) drain();
+                    if (!FS::kMayDiscard && __any(qn == Q)) drain_one();
                     const int blk = slot + k * WAVES;
                     const int bx0 = tile_x0 + (blk % BW) * 8, by0 = tile_y0 + (blk / BW) * 8;
                     if (BPW > 1 && (r.x1 <= bx0 || r.x0 >= bx0 + 8 || r.y1 <= by0 || r.y0 >= by0 + 8)) continue;   // wave-uniform
@@ -313,7 +320,9 @@ __global__ __launch_bounds__(64 * WAVES) void raster_kernel(RasterArgs a)
                 }
             }
         }
-        if (!FS::kMayDiscard) drain();
+        if (!FS::kMayDiscard) {
+            while (__any(qn > 0)) drain_one();
+        }
         (void)env;
 
         // ---- write back

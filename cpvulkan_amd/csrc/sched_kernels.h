@@ -35,7 +35,10 @@ __device__ __forceinline__ uint32_t feature_target(const TargetParams* t, uint32
     return lo;
 }
 
-// One wavefront per tile; lanes stride over the tile's primitive list.
+// One wavefront per tile.  Lists of up to 64 primitives are read whole;
+// longer lists are sampled at 64 evenly spaced entries and the sums scaled
+// by n / 64, which bounds the kernel's cost per tile (the statistics only
+// steer the queue order).
 //   f0  log2(1 + n)                         n = primitives binned to the tile
 //   f1  n / 1024
 //   f2  log2(1 + sum of bounding-box pixels inside the tile)
@@ -60,7 +63,10 @@ __global__ __launch_bounds__(256) void tile_features_kernel(FeatureArgs a)
     const float tile_px = (float)(a.tile * a.tile);
     const uint32_t b = a.tile_start[tile], e = a.tile_start[tile + 1];
     float s_box = 0.0f, s_full = 0.0f, s_tri = 0.0f, s_blend = 0.0f, s_span = 0.0f;
-    for (uint32_t i = b + lane; i < e; i += 64u) {
+    const uint32_t n_all = e - b;
+    const uint32_t m = min(n_all, 64u);
+    if ((uint32_t)lane < m) {
+        const uint32_t i = b + (uint32_t)(((uint64_t)lane * n_all) / m);
         const TriSetup& r = a.recs[a.tile_prims[i]];
         const int cx0 = max(r.x0, x0), cy0 = max(r.y0, y0), cx1 = min(r.x1, x1), cy1 = min(r.y1, y1);
         const float box = (float)(max(cx1 - cx0, 0) * max(cy1 - cy0, 0));
@@ -80,9 +86,13 @@ __global__ __launch_bounds__(256) void tile_features_kernel(FeatureArgs a)
         s_blend += __shfl_xor(s_blend, off, 64);
         s_span += __shfl_xor(s_span, off, 64);
     }
+    const float scale = m ? (float)n_all / (float)m : 0.0f;
+    s_box *= scale;
+    s_full *= scale;
+    s_tri *= scale;
     if (lane < 16) {
-        const float n = (float)(e - b);
-        const float inv_n = n > 0.0f ? 1.0f / n : 0.0f;
+        const float n = (float)n_all;
+        const float inv_n = m ? 1.0f / (float)m : 0.0f;   // s_blend, s_span are sample sums
         float v = 0.0f;
         switch (lane) {
         case 0: v = log2f(1.0f + n); break;
@@ -115,16 +125,6 @@ __global__ __launch_bounds__(256) void order_keys_kernel(const float* pred, uint
     const float q = fminf(p, 63.99f) * 1024.0f;
     keys[i] = 65535u - (uint32_t)q;
     vals[i] = i;
-}
-
-// Cost of each tile in a trace: the cycles of its wavefronts summed.
-__global__ __launch_bounds__(256) void tile_cost_kernel(const float* wave_cycles, uint32_t tiles, int waves, float* cost)
-{
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= tiles) return;
-    float s = 0.0f;
-    for (int w = 0; w < waves; w++) s += wave_cycles[(uint64_t)i * waves + w];
-    cost[i] = s;
 }
 
 }  // namespace cpv

@@ -15,16 +15,16 @@ def render_gpu(ctx, scene: Scene, count: bool = True):
     mem = DeviceMemory(ctx)
     try:
         m = materialize(scene, mem)
+        saved = ctx.config()
         if not count:
-            cfg = ctx_config(ctx)
-            ctx.set_config(tile_size=cfg.tile_size, waves_per_tile=cfg.waves_per_tile, scheduler=cfg.scheduler,
-                           count_fragments=0)
+            cfg = ctx.config()
+            cfg.count_fragments = 0
+            ctx.apply_config(cfg)
         try:
             ctx.render_pass(m.pass_desc, m.pipelines, m.draws, m.draw_pipeline)
             ctx.synchronize()
         finally:
-            if not count:
-                ctx.set_config(tile_size=cfg.tile_size, waves_per_tile=cfg.waves_per_tile, scheduler=cfg.scheduler)
+            ctx.apply_config(saved)
         frags = ctx.fragment_count() if count else None
         return m.read_color(), m.read_ds(), frags
     finally:
