@@ -138,11 +138,11 @@ __device__ __forceinline__ float interpolate_unit_w(const float w[3], float v0, 
 
 // Colour work of one fragment that passed the depth/stencil tests:
 // interpolation (SetDatum), the fragment shader and the colour write with
-// blend (PipelineCompiler.cpp:1415-1727).  Kept out of line: it runs for a
-// small fraction of the fragments, and inlining it at every call site would
-// multiply the raster loop's code size.
+// blend (PipelineCompiler.cpp:1415-1727).  The raster kernel calls it from
+// one place (the colour queue's drain), so it is inlined: an out-of-line call
+// would pass the pixel state through scratch memory.
 template <class FS>
-__device__ __noinline__ PixelState shade_fragment(const TriSetup* recs, const DrawDesc* draws, const FragmentState* fstates,
+__device__ __forceinline__ PixelState shade_fragment(const TriSetup* recs, const DrawDesc* draws, const FragmentState* fstates,
                                                   const float* vout, uint32_t stride, uint32_t gprim, float w0, float w1,
                                                   float w2, int x, int y, PixelState ps)
 {
@@ -178,9 +178,9 @@ struct DepthStencilResult {
     uint32_t write;
 };
 
-// depth_stencil_ops out of line (the exact path runs only for fragments the
-// conservative test could not reject).
-__device__ __noinline__ DepthStencilResult depth_stencil_call(const FragmentState* st, PixelState px, float depth, uint32_t front)
+// depth_stencil_ops with the pixel state passed and returned by value (one
+// call site per raster kernel instantiation, inlined).
+__device__ __forceinline__ DepthStencilResult depth_stencil_call(const FragmentState* st, PixelState px, float depth, uint32_t front)
 {
     DepthStencilResult r;
     r.px = px;

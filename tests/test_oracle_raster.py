@@ -239,3 +239,38 @@ def test_cube_frame_structure():
     dv = d.view(np.uint16)
     assert np.all(dv[bg] == 65535) and np.all(dv[~bg] < 65535)
     assert frags > int((~bg).sum())          # hidden faces also produce fragments
+
+
+def _strip_scene(n, pts, topology, cull="NONE"):
+    s = _quad_scene(n, [pts])
+    s.pipeline["topology"] = abi.TOPOLOGY[topology]
+    s.pipeline["cull"] = abi.CULL[cull]
+    s.pipeline["front_face"] = abi.FRONT_FACE["CCW"]
+    return s
+
+
+@pytest.mark.parametrize("n", [8, 32])
+def test_strip_keeps_vertex_order(n):
+    """CalculatePrimitives (:629-645) assembles strip triangle i as
+    (i, i+1, i+2) without the usual odd-triangle reorder, so consecutive
+    strip triangles have opposite windings: the quad strip covers the
+    square (diagonal twice) with culling off, and back-face culling keeps
+    exactly one of its two triangles (the lower-right half, n(n+1)/2
+    pixels including the diagonal)."""
+    pts = [(-1, -1), (1, -1), (-1, 1), (1, 1)]
+    assert oracle_lib.render_scene(_strip_scene(n, pts, "TRIANGLE_STRIP"))[2] == n * n + n
+    one = oracle_lib.render_scene(_strip_scene(n, pts, "TRIANGLE_STRIP", cull="BACK"))[2]
+    other = oracle_lib.render_scene(_strip_scene(n, pts, "TRIANGLE_STRIP", cull="FRONT"))[2]
+    assert one == n * (n + 1) // 2 and other == n * (n + 1) // 2
+
+
+@pytest.mark.parametrize("n", [8, 32])
+def test_fan_shares_first_vertex(n):
+    """Fan triangle i is (0, i+1, i+2) (:646-662): a fan around the centre
+    through the four corners covers the square, every triangle with the
+    same winding (one culling mode keeps all, the other none)."""
+    pts = [(0, 0), (-1, -1), (1, -1), (1, 1), (-1, 1), (-1, -1)]
+    total = oracle_lib.render_scene(_strip_scene(n, pts, "TRIANGLE_FAN"))[2]
+    assert total >= n * n
+    kept = {c: oracle_lib.render_scene(_strip_scene(n, pts, "TRIANGLE_FAN", cull=c))[2] for c in ("FRONT", "BACK")}
+    assert sorted(kept.values()) == [0, total]
