@@ -66,6 +66,10 @@ def _declare(lib: C.CDLL) -> None:
     lib.cpv_mlp_gradients.argtypes = [P, FP, FP, C.c_uint32, FP, FP]
     lib.cpv_mlp_train.argtypes = [P, FP, FP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, FP]
     lib.cpv_set_predictor.argtypes = [P, P]
+    lib.cpv_spirv_translate.argtypes = [P, C.c_size_t, C.c_char_p, P, C.c_size_t, C.c_char_p, C.c_char_p, C.c_size_t,
+                                        C.POINTER(C.c_size_t)]
+    lib.cpv_spirv_translate_error.restype = C.c_char_p
+    lib.cpv_spirv_build.argtypes = [P, C.c_size_t, C.c_char_p, P, C.c_size_t, C.c_char_p, C.POINTER(C.c_size_t)]
 
 
 def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
@@ -210,6 +214,60 @@ class Context:
 
     def set_predictor(self, mlp: "Mlp | None") -> None:
         _check(self.lib, self.lib.cpv_set_predictor(self.handle, mlp.handle if mlp else None), "cpv_set_predictor")
+
+
+def _words(words):
+    import numpy as np
+    return np.ascontiguousarray(np.asarray(words, dtype=np.uint32))
+
+
+def spirv_translate(vs_words, fs_words, vs_entry: str = "main", fs_entry: str = "main") -> str:
+    """HIP source of a SPIR-V (vertex, fragment) pair (no device needed)."""
+    lib = load_library()
+    v, f = _words(vs_words), _words(fs_words)
+    n = C.c_size_t()
+    rc = lib.cpv_spirv_translate(v.ctypes.data, v.size, vs_entry.encode(), f.ctypes.data, f.size, fs_entry.encode(), None,
+                                 0, C.byref(n))
+    if rc != 0:
+        raise CpvError(f"cpv_spirv_translate failed ({rc}): {lib.cpv_spirv_translate_error().decode()}")
+    buf = C.create_string_buffer(n.value + 1)
+    lib.cpv_spirv_translate(v.ctypes.data, v.size, vs_entry.encode(), f.ctypes.data, f.size, fs_entry.encode(), buf,
+                            n.value + 1, C.byref(n))
+    return buf.value.decode()
+
+
+def spirv_build(vs_words, fs_words, vs_entry: str = "main", fs_entry: str = "main") -> int:
+    """Translate and compile with hiprtc (no device needed); returns the code object size."""
+    lib = load_library()
+    v, f = _words(vs_words), _words(fs_words)
+    n = C.c_size_t()
+    rc = lib.cpv_spirv_build(v.ctypes.data, v.size, vs_entry.encode(), f.ctypes.data, f.size, fs_entry.encode(), C.byref(n))
+    if rc != 0:
+        raise CpvError(f"cpv_spirv_build failed ({rc}): {lib.cpv_spirv_translate_error().decode()}")
+    return n.value
+
+
+class Program:
+    """A shader program translated from SPIR-V and compiled with hiprtc
+    (cpv_program_create_spirv); set it as a pipeline's `program`."""
+
+    def __init__(self, ctx: "Context", vs_words, fs_words, vs_entry: str = "main", fs_entry: str = "main"):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        v, f = _words(vs_words), _words(fs_words)
+        self.handle = C.c_void_p()
+        rc = self.lib.cpv_program_create_spirv(ctx.handle, v.ctypes.data, v.size, vs_entry.encode(), f.ctypes.data, f.size,
+                                               fs_entry.encode(), C.byref(self.handle))
+        _check(self.lib, rc, "cpv_program_create_spirv")
+
+    @property
+    def source(self) -> str:
+        return self.lib.cpv_program_source(self.handle).decode()
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.cpv_program_destroy(self.ctx.handle, self.handle)
+            self.handle = C.c_void_p()
 
 
 class Batch:

@@ -33,7 +33,7 @@ LOAD_OP = dict(LOAD=0, CLEAR=1, DONT_CARE=2)
 FILTER = dict(NEAREST=0, LINEAR=1)
 ADDRESS = dict(REPEAT=0, MIRRORED_REPEAT=1, CLAMP_TO_EDGE=2, CLAMP_TO_BORDER=3, MIRROR_CLAMP_TO_EDGE=4)
 VS = dict(PASSTHROUGH=1, MVP_COLOR=2, MVP_LAMBERT=3)
-FS = dict(COLOR=1, TEX_LAMBERT=2)
+FS = dict(COLOR=1, TEX_LAMBERT=2, COLOR_DISCARD=3)
 SCHED = dict(GRID=0, COUNTER=1, PREDICTED=2)
 COLOR_RGBA = 15
 

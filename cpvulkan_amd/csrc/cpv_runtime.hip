@@ -352,6 +352,10 @@ bool builtin_program(uint32_t vs, uint32_t fs, Program* p)
         for (int i = 0; i < 7; i++) p->raster[i].host_fn = RasterTable<FsTexLambert>::get(i);
         p->fs_inputs = FsTexLambert::kInputs;
         break;
+    case CPV_FS_COLOR_DISCARD:
+        for (int i = 0; i < 7; i++) p->raster[i].host_fn = RasterTable<FsColorDiscard>::get(i);
+        p->fs_inputs = FsColorDiscard::kInputs;
+        break;
     default: return false;
     }
     return true;

@@ -9,6 +9,16 @@
 
 #ifndef __HIPCC_RTC__
 #include <stdint.h>
+#else
+// hiprtc declares the fixed-width integer types in its own namespace
+using __hip_internal::int8_t;
+using __hip_internal::int16_t;
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint8_t;
+using __hip_internal::uint16_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
 #endif
 
 namespace cpv {

@@ -60,6 +60,19 @@ struct FsColor {
     }
 };
 
+// if (color.r < 0.25) discard; out = color.
+struct FsColorDiscard {
+    static constexpr int kInputs = 4;
+    static constexpr bool kMayDiscard = true;
+    __device__ static int interp(int) { return 0; }
+    __device__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
+    {
+        if (in[0] < 0.25f) return true;
+        out[0] = in[0]; out[1] = in[1]; out[2] = in[2]; out[3] = in[3];
+        return false;
+    }
+};
+
 // out = texture(tex@1, uv) * max(dot(normalize(n), L@2.xyz), 0).
 struct FsTexLambert {
     static constexpr int kInputs = 5;

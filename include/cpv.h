@@ -72,8 +72,9 @@ enum {
 };
 enum {
     CPV_FS_COLOR = 1,         /* in vec4 color; out vec4 = color (15-draw_cube.cpp:53-61) */
-    CPV_FS_TEX_LAMBERT = 2    /* in vec3 n, in vec2 uv; sampler2D(binding 1); uniform(binding 2) vec4 L;
+    CPV_FS_TEX_LAMBERT = 2,   /* in vec3 n, in vec2 uv; sampler2D(binding 1); uniform(binding 2) vec4 L;
                                  out = texture(tex, uv) * max(dot(normalize(n), L.xyz), 0) */
+    CPV_FS_COLOR_DISCARD = 3  /* in vec4 color; if (color.r < 0.25) discard; out vec4 = color */
 };
 
 /* Scheduling policies of the tile work queue (cpv_config.scheduler). */
@@ -336,6 +337,18 @@ int cpv_program_create_spirv(cpv_context* ctx,
                              const uint32_t* fs_words, size_t fs_word_count, const char* fs_entry,
                              cpv_program** out);
 void cpv_program_destroy(cpv_context* ctx, cpv_program* program);
+/* The HIP source a (vertex, fragment) SPIR-V pair translates to, without
+ * compiling it (no device needed).  Writes at most capacity bytes including
+ * the terminating NUL; *length receives the full length.  On
+ * CPV_ERROR_UNSUPPORTED, cpv_spirv_translate_error() names the construct. */
+int cpv_spirv_translate(const uint32_t* vs_words, size_t vs_word_count, const char* vs_entry,
+                        const uint32_t* fs_words, size_t fs_word_count, const char* fs_entry,
+                        char* source, size_t capacity, size_t* length);
+const char* cpv_spirv_translate_error(void);
+/* Translates and compiles the pair with hiprtc for gfx950 without loading
+ * the code object (no device needed); *code_size receives its size. */
+int cpv_spirv_build(const uint32_t* vs_words, size_t vs_word_count, const char* vs_entry,
+                    const uint32_t* fs_words, size_t fs_word_count, const char* fs_entry, size_t* code_size);
 /* Generated HIP source of a program (for inspection); valid until destroy. */
 const char* cpv_program_source(cpv_program* program);
 

@@ -612,6 +612,7 @@ static int fs_inputs(uint32_t kind)
 {
     switch (kind) {
     case CPV_FS_COLOR: return 4;
+    case CPV_FS_COLOR_DISCARD: return 4;
     case CPV_FS_TEX_LAMBERT: return 5;
     default: return -1;
     }
@@ -708,6 +709,11 @@ static int run_fs(const cpv_pipeline_desc* p, const cpv_draw_desc* d, const floa
 {
     switch (p->fs_kind) {
     case CPV_FS_COLOR:
+        memcpy(out, in, 16);
+        return 0;
+    case CPV_FS_COLOR_DISCARD:
+        /* if (color.r < 0.25) discard: OpFOrdLessThan then OpKill */
+        if (in[0] < 0.25f) return 1;
         memcpy(out, in, 16);
         return 0;
     case CPV_FS_TEX_LAMBERT: {
