@@ -238,6 +238,26 @@ extern "C" int cpv_free(cpv_context* c, uint64_t address)
     return CPV_OK;
 }
 
+extern "C" int cpv_alloc_host(cpv_context* c, uint64_t bytes, uint64_t* address)
+{
+    if (!c || !address) return fail(CPV_ERROR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(CPV_ERROR_OUT_OF_MEMORY, "hipHostMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
+    *address = (uint64_t)(uintptr_t)p;
+    return CPV_OK;
+}
+
+extern "C" int cpv_free_host(cpv_context* c, uint64_t address)
+{
+    if (!c) return fail(CPV_ERROR_INVALID, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipHostFree((void*)(uintptr_t)address));
+    return CPV_OK;
+}
+
 extern "C" int cpv_upload(cpv_context* c, uint64_t dst, const void* src, uint64_t bytes)
 {
     if (!c || (!src && bytes)) return fail(CPV_ERROR_INVALID, "null argument");

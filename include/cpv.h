@@ -228,6 +228,10 @@ int cpv_context_create(int device_ordinal, cpv_context** out);
 void cpv_context_destroy(cpv_context* ctx);
 int cpv_alloc(cpv_context* ctx, uint64_t bytes, uint64_t* address);
 int cpv_free(cpv_context* ctx, uint64_t address);
+/* Pinned host memory the kernels read and write at the same address
+ * (HOST_VISIBLE | HOST_COHERENT Vulkan memory); free with cpv_free_host. */
+int cpv_alloc_host(cpv_context* ctx, uint64_t bytes, uint64_t* address);
+int cpv_free_host(cpv_context* ctx, uint64_t address);
 int cpv_upload(cpv_context* ctx, uint64_t dst, const void* src, uint64_t bytes);
 int cpv_download(cpv_context* ctx, void* dst, uint64_t src, uint64_t bytes);
 int cpv_synchronize(cpv_context* ctx);
