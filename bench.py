@@ -117,10 +117,23 @@ def same(a, b) -> bool:
             and (a[2] is None or b[2] is None or a[2] == b[2]))
 
 
-def roofline(kernel_ms: float, alg_bytes: int) -> dict:
+def measured_traffic(workload: str, tile: int, waves: int):
+    """HBM bytes per raster launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (profiles/traffic.json), when they cover this
+    workload and tile configuration; else None."""
+    try:
+        t = json.loads((ROOT / "profiles" / "traffic.json").read_text()).get(workload)
+    except (OSError, ValueError):
+        return None
+    if not t or t.get("tile") != tile or t.get("waves_per_tile") != waves:
+        return None
+    return int(t["bytes_per_launch"])
+
+
+def roofline(kernel_ms: float, alg_bytes: int, traffic=None) -> dict:
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None)
+                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic)
 
 
 def cpu_entry(dt: float, mode: str, what: str) -> dict:
@@ -183,7 +196,8 @@ def bench_single(args, ctx, scene, workload):
         fragments_per_frame=fragments, fragments_per_s=round(fragments * fps, 1),
         raster_ms=round(raster_avg, 4), frame_gpu_ms=round(float(np.mean(total_ms)), 4),
         bytes_per_frame=dict(framebuffer=b_fb, geometry=geometry_bytes(scene)),
-        roofline=roofline(raster_avg, b_fb), cpu_baseline=cpu, check=check,
+        roofline=roofline(raster_avg, b_fb, measured_traffic(workload, args.tile, args.waves)), cpu_baseline=cpu,
+        check=check,
     )
     mem.release()
     return out, None

@@ -66,8 +66,12 @@ __device__ __forceinline__ void store_pixel(uint8_t* p, int bytes, const uint32_
     }
 }
 
+#ifndef CPV_RASTER_WAVES_PER_EU
+#define CPV_RASTER_WAVES_PER_EU 4
+#endif
+
 template <class FS, int TILE, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void raster_kernel(RasterArgs a)
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(CPV_RASTER_WAVES_PER_EU))) void raster_kernel(RasterArgs a)
 {
     constexpr int BW = TILE / 8;               // 8x8 blocks per tile row
     constexpr int BLOCKS = BW * BW;

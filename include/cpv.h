@@ -22,7 +22,20 @@
  *                           (LLVMRuntime/PipelineCompiler.cpp:1801-1832), called
  *                           from GraphicsPipeline::Create (CPVulkan/Pipeline.cpp:599-714).
  *   cpv_alloc/cpv_free/cpv_upload/cpv_download -> DeviceMemory allocation and
- *                           vkMapMemory access (CPVulkan/Device.cpp:97-200).
+ *                           vkMapMemory access (CPVulkan/Device.cpp:97-200);
+ *                           cpv_alloc_host/cpv_free_host back host-visible
+ *                           memory that vkMapMemory returns directly.
+ *   cpv_device_copy      -> the transfer half of vkCmdCopyBuffer
+ *                           (CPVulkan/CommandBuffer.Copy.cpp:15-60).
+ *   cpv_batch_*, cpv_autotune, cpv_mlp_*, cpv_set_predictor, cpv_get_tile_traces,
+ *   cpv_set_config       -> no reference counterpart: recorded hipGraph
+ *                           batches, the tile-configuration autotuner and the
+ *                           learned tile scheduler only change how the same
+ *                           frames are scheduled.
+ * The Vulkan ICD entry points (vk_icdNegotiateLoaderICDInterfaceVersion,
+ * vk_icdGetInstanceProcAddr, vk_icdGetPhysicalDeviceProcAddr; the reference's
+ * CPVulkan/CPVulkan.cpp:14-104) are declared in cpv_vulkan.h and exported by
+ * libcpvulkan_amd_icd.so, which is built on this C-ABI.
  *
  * Addresses in descriptor structs are device addresses obtained from
  * cpv_alloc (or any HIP device allocation).  The CPU oracle under oracle/
