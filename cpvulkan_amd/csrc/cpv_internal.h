@@ -38,11 +38,14 @@ struct DevBuf {
 // Control words at the start of Buffers::stats.
 enum : uint32_t {
     kStatFragments = 0,   // u64 at words 0-1: covered fragments
-    kStatQueue = 8,       // work counter of the persistent raster kernel
     kStatPairs = 9,       // (tile, primitive) pairs kept, min(total, capacity)
     kStatOverflow = 10,   // set when the pair total exceeded the capacity
     kStatTiles = 11,      // tile count of the launch (length of the order sort)
-    kStatWords = 64,
+    kStatWords = 64,      // words read back after a launch
+    // Work counters of the persistent raster kernel, one per queue, each on
+    // its own 128-byte line, after the read-back words.
+    kStatQueues = kStatWords,
+    kStatAlloc = kStatWords + cpv::kRasterQueues * cpv::kRasterQueueStride,
 };
 
 // Working buffers of the launch sequence.  A context owns one set for its

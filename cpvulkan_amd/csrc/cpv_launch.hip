@@ -289,7 +289,7 @@ int ensure_front(Exec& x, const LaunchPlan& P)
     if (b.vout.ensure((size_t)std::max(P.vout_total, 1u) * P.stride * 4, f) ||
         b.recs.ensure((size_t)std::max(P.prim_total, 1u) * sizeof(TriSetup), f) ||
         b.counts.ensure((size_t)std::max(P.prim_total, 1u) * 4, f) || b.offsets.ensure(((size_t)P.prim_total + 1) * 4, f) ||
-        b.stats.ensure(kStatWords * 4, f))
+        b.stats.ensure(kStatAlloc * 4, f))
         return CPV_ERROR_OUT_OF_MEMORY;
     return CPV_OK;
 }
@@ -407,7 +407,7 @@ int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap
     ra.fstates = d_fs;
     ra.vout = b.vout.as<float>();
     ra.order = order;
-    ra.queue = stats + kStatQueue;
+    ra.queue = stats + kStatQueues;
     ra.fragments = (unsigned long long*)(stats + kStatFragments);
     ra.trace_cycles = x.cfg.record_costs ? b.traces.as<float>() : nullptr;
     ra.vout_stride = P.stride;
@@ -421,7 +421,7 @@ int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap
         const int resident = resident_workgroups(x, rk, block);
         if (resident <= 0) return CPV_ERROR_DEVICE;
         grid = std::min<uint32_t>(P.tiles, (uint32_t)resident);
-        fill_u32(x.s, stats + kStatQueue, 0u, 1);
+        fill_u32(x.s, stats + kStatQueues, 0u, kRasterQueues * kRasterQueueStride);
     }
     if (grid == 0) return CPV_OK;
     if (x.r0) HIP_TRY(hipEventRecord(x.r0, x.s));
@@ -546,7 +546,7 @@ extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const 
     if (pass->color_count > CPV_MAX_COLOR_ATTACHMENTS) return fail(CPV_ERROR_UNSUPPORTED, "%u colour attachments", pass->color_count);
     std::lock_guard<std::mutex> guard(c->lock);
     HIP_TRY(hipSetDevice(c->device));
-    if (c->buf.stats.ensure(kStatWords * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+    if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
     fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
     HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
     const int tile = (int)c->cfg.tile_size;
@@ -595,7 +595,7 @@ extern "C" int cpv_render_jobs(cpv_context* c, const cpv_job_desc* jobs, uint32_
     if (!c || (job_count && !jobs)) return fail(CPV_ERROR_INVALID, "null argument");
     std::lock_guard<std::mutex> guard(c->lock);
     HIP_TRY(hipSetDevice(c->device));
-    if (c->buf.stats.ensure(kStatWords * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+    if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
     fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
     HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
     std::vector<JobGroup> groups;
@@ -725,7 +725,7 @@ static int batch_build(cpv_context* c, cpv_batch* B, const cpv_job_desc* jobs, u
         const int ci = tile_config_index(B->cfg.tile_size, B->cfg.waves_per_tile);
         if (resident_workgroups(x, B->plans[g].prog.raster[ci], 64 * (int)B->cfg.waves_per_tile) <= 0) return CPV_ERROR_DEVICE;
     }
-    if (B->b.stats.ensure(kStatWords * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+    if (B->b.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
     HIP_TRY(hipStreamSynchronize(c->stream));
     // record: every kernel of every launch, no allocation, no host sync
     x.frozen = true;

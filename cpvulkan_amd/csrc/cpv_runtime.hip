@@ -196,7 +196,7 @@ extern "C" int cpv_context_create(int device_ordinal, cpv_context** out)
         delete c;
         return fail(CPV_ERROR_DEVICE, "stream/event/pinned allocation failed");
     }
-    if (c->buf.stats.ensure(kStatWords * 4)) { delete c; return CPV_ERROR_OUT_OF_MEMORY; }
+    if (c->buf.stats.ensure(kStatAlloc * 4)) { delete c; return CPV_ERROR_OUT_OF_MEMORY; }
     *out = c;
     return CPV_OK;
 }

@@ -23,6 +23,16 @@ using __hip_internal::uint64_t;
 
 namespace cpv {
 
+// Work queues of the persistent raster kernel.  Tile i of the launch's
+// queue order belongs to queue i % kRasterQueues, and a wave starts on the
+// queue of the XCD its workgroup was dispatched to (workgroups go to the
+// MI355X's 8 XCDs round-robin by index), so the wave slots of one tile run
+// under one L2 and share its copies of the tile's triangle records.  A
+// wave whose queue is empty takes work from the others in turn.  Counters
+// are kRasterQueueStride words apart, one 128-byte line each.
+constexpr uint32_t kRasterQueues = 8;
+constexpr uint32_t kRasterQueueStride = 32;
+
 // Channel base types of FormatInformation (CPVulkanBase/Formats.h:30-103).
 enum BaseType : int32_t {
     kBaseNone = 0,

@@ -52,6 +52,7 @@ struct VsMvpLambert {
 struct FsColor {
     static constexpr int kInputs = 4;
     static constexpr bool kMayDiscard = false;
+    static constexpr int kWavesPerEU = 6;
     __device__ static int interp(int) { return 0; }
     __device__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
     {
@@ -64,6 +65,7 @@ struct FsColor {
 struct FsColorDiscard {
     static constexpr int kInputs = 4;
     static constexpr bool kMayDiscard = true;
+    static constexpr int kWavesPerEU = 6;
     __device__ static int interp(int) { return 0; }
     __device__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
     {

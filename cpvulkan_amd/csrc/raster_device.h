@@ -277,14 +277,14 @@ CPV_DEV void sample_level(const TextureDesc& t, uint32_t level, float u, float v
     lerp4(a, b, (double)ty, out);
 }
 
-// ImageSampleImplicitLod (CPVulkan/GlslFunctions.cpp:656-671): lambda base 0,
-// then ImageSampleExplicitLod (:598-654) and SampleImage
-// (ImageSampler.cpp:581-673).
-CPV_DEV void sample_implicit(const TextureDesc& t, float u, float v, float out[4])
+// ImageSampleExplicitLod (CPVulkan/GlslFunctions.cpp:598-654) and SampleImage
+// (ImageSampler.cpp:581-673): lambda = clamp(lod + clamp(bias, +-32), minLod,
+// maxLod).
+CPV_DEV void sample_explicit(const TextureDesc& t, float u, float v, float lod, float out[4])
 {
     float bias = t.sampler.mip_lod_bias + 0.0f;
     bias = bias < -32.0f ? -32.0f : (bias > 32.0f ? 32.0f : bias);
-    float lambda = 0.0f + bias;
+    float lambda = lod + bias;
     lambda = lambda < t.sampler.min_lod ? t.sampler.min_lod : (lambda > t.sampler.max_lod ? t.sampler.max_lod : lambda);
     if (lambda <= 0) {
         sample_level(t, 0, u, v, t.sampler.mag_filter, out);
@@ -312,6 +312,23 @@ CPV_DEV void sample_implicit(const TextureDesc& t, float u, float v, float out[4
     if (t.fmt.off[1] < 0) out[1] = 0.0f;
     if (t.fmt.off[2] < 0) out[2] = 0.0f;
     if (t.fmt.off[3] < 0) out[3] = 1.0f;
+}
+
+// ImageSampleImplicitLod (CPVulkan/GlslFunctions.cpp:656-671): lambda base 0.
+CPV_DEV void sample_implicit(const TextureDesc& t, float u, float v, float out[4])
+{
+    sample_explicit(t, u, v, 0.0f, out);
+}
+
+// ImageFetch (CPVulkan/GlslFunctions.cpp:673-734): the texel of the view's
+// base level; coordinates outside it read ReturnType{} = (0, 0, 0, 0).
+CPV_DEV void fetch_texel(const TextureDesc& t, int32_t x, int32_t y, float out[4])
+{
+    if (x < 0 || (uint32_t)x >= t.level_w[0] || y < 0 || (uint32_t)y >= t.level_h[0]) {
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        return;
+    }
+    texel(t, 0, x, y, out);
 }
 
 // ------------------------------------------------------------------ blend

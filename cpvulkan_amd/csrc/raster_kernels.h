@@ -68,7 +68,7 @@ struct RasterArgs {
     const FragmentState* fstates;
     const float* vout;
     const uint32_t* order;          // queue order of tiles (nullptr: index order)
-    uint32_t* queue;                // work counter for the persistent schedule
+    uint32_t* queue;                // kRasterQueues work counters, kRasterQueueStride words apart
     unsigned long long* fragments;  // covered (primitive, pixel) pairs
     float* trace_cycles;            // shader cycles per (tile, wave slot) (nullptr: not recorded)
     uint32_t vout_stride;

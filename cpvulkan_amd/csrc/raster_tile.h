@@ -66,12 +66,32 @@ __device__ __forceinline__ void store_pixel(uint8_t* p, int bytes, const uint32_
     }
 }
 
-#ifndef CPV_RASTER_WAVES_PER_EU
-#define CPV_RASTER_WAVES_PER_EU 4
-#endif
+// Resident wavefronts per SIMD the raster kernel is compiled for: the
+// kernel is latency bound, and more resident waves hide more of it until
+// the register budget (512 / waves VGPRs) forces spills.  A fragment shader
+// may set kWavesPerEU; measured on MI355X: 6 for the colour shaders, 5 for
+// the textured Lambert shader (the default).
+template <class FS, class = void>
+struct RasterWavesPerEU {
+    static constexpr int value = 5;
+};
+template <class FS>
+struct RasterWavesPerEU<FS, decltype((void)FS::kWavesPerEU)> {
+    static constexpr int value = FS::kWavesPerEU;
+};
+
+// The target applies when a lane holds one 8x8 block's pixel (8x8 tiles,
+// or 16x16 tiles with four waves); a lane holding several blocks' pixels
+// needs the registers of the compiler's default budget.
+template <class FS, int TILE, int WAVES>
+constexpr int raster_waves_per_eu()
+{
+    return (TILE / 8) * (TILE / 8) / WAVES == 1 ? RasterWavesPerEU<FS>::value : 1;
+}
 
 template <class FS, int TILE, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(CPV_RASTER_WAVES_PER_EU))) void raster_kernel(RasterArgs a)
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(raster_waves_per_eu<FS, TILE, WAVES>()))) void
+raster_kernel(RasterArgs a)
 {
     constexpr int BW = TILE / 8;               // 8x8 blocks per tile row
     constexpr int BLOCKS = BW * BW;
@@ -80,18 +100,31 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(CPV_
 
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const uint32_t units = a.tiles * (uint32_t)WAVES;
-    uint32_t unit = blockIdx.x * WAVES + wave;
+    const uint32_t home = blockIdx.x % kRasterQueues;
+    uint32_t tried = 0;   // queues found empty, starting at home
 
     for (;;) {
+        uint32_t item = blockIdx.x;
+        int slot = wave;
         if (a.persistent) {
-            uint32_t u = 0;
-            if (lane == 0) u = atomicAdd(a.queue, 1u);
-            unit = __builtin_amdgcn_readfirstlane(u);
+            // Unit u of queue q is wave slot u % WAVES of the queue's
+            // (u / WAVES)-th tile, queue item (u / WAVES) * kRasterQueues + q.
+            // Every wave ends after at most one failed take per queue.
+            for (; tried < kRasterQueues; tried++) {
+                const uint32_t q = (home + tried) % kRasterQueues;
+                const uint32_t per_q = a.tiles > q ? (a.tiles - q + kRasterQueues - 1) / kRasterQueues : 0u;
+                uint32_t u = 0;
+                if (lane == 0) u = atomicAdd(a.queue + q * kRasterQueueStride, 1u);
+                u = __builtin_amdgcn_readfirstlane(u);
+                if (u < per_q * (uint32_t)WAVES) {
+                    item = (u / WAVES) * kRasterQueues + q;
+                    slot = (int)(u % WAVES);
+                    break;
+                }
+            }
+            if (tried == kRasterQueues) break;
         }
-        if (unit >= units) break;
-        const uint32_t item = unit / WAVES;
-        const int slot = (int)(unit % WAVES);
+        if (item >= a.tiles) break;
         const uint32_t tile = a.order ? a.order[item] : item;
         const uint64_t t_begin = clock64();
 

@@ -37,6 +37,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <functional>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -73,11 +74,24 @@ enum : uint32_t {
     OpPhi = 245, OpLoopMerge = 246, OpSelectionMerge = 247, OpLabel = 248, OpBranch = 249,
     OpBranchConditional = 250, OpKill = 252, OpReturn = 253, OpReturnValue = 254, OpSource = 3,
     OpSourceExtension = 4, OpMemberName = 6, OpString = 7, OpLine = 8, OpNoLine = 317, OpModuleProcessed = 330,
+    OpUndef = 1, OpExtension = 10, OpConstantNull = 46, OpSpecConstantTrue = 48, OpSpecConstantFalse = 49,
+    OpSpecConstant = 50, OpSpecConstantComposite = 51, OpVectorExtractDynamic = 77, OpVectorInsertDynamic = 78,
+    OpCompositeInsert = 82, OpCopyObject = 83, OpTranspose = 84, OpSampledImage = 86, OpImageSampleExplicitLod = 88,
+    OpImageFetch = 95, OpImage = 100, OpImageQuerySizeLod = 103, OpImageQuerySize = 104, OpImageQueryLevels = 106,
+    OpUConvert = 113, OpSConvert = 114, OpFConvert = 115, OpQuantizeToF16 = 116, OpUMod = 137, OpSRem = 138,
+    OpSMod = 139, OpFRem = 140, OpFMod = 141, OpMatrixTimesScalar = 143, OpVectorTimesMatrix = 144,
+    OpMatrixTimesMatrix = 146, OpOuterProduct = 147, OpAny = 154, OpAll = 155, OpIsNan = 156, OpIsInf = 157,
+    OpLogicalEqual = 164, OpLogicalNotEqual = 165, OpFUnordEqual = 181, OpFUnordNotEqual = 183,
+    OpFUnordLessThan = 185, OpFUnordGreaterThan = 187, OpFUnordLessThanEqual = 189,
+    OpFUnordGreaterThanEqual = 191, OpShiftRightLogical = 194, OpShiftRightArithmetic = 195,
+    OpShiftLeftLogical = 196, OpBitwiseOr = 197, OpBitwiseXor = 198, OpBitwiseAnd = 199, OpNot = 200,
+    OpBitFieldInsert = 201, OpBitFieldSExtract = 202, OpBitFieldUExtract = 203, OpBitReverse = 204, OpBitCount = 205,
+    OpSwitch = 251, OpUnreachable = 255, OpDecorationGroup = 73, OpGroupDecorate = 74,
 };
 
 enum : uint32_t {
     DecBlock = 2, DecColMajor = 5, DecMatrixStride = 7, DecBuiltIn = 11, DecNoPerspective = 13, DecFlat = 14,
-    DecLocation = 30, DecBinding = 33, DecDescriptorSet = 34, DecOffset = 35,
+    DecLocation = 30, DecBinding = 33, DecDescriptorSet = 34, DecOffset = 35, DecArrayStride = 6,
 };
 enum : uint32_t { BiPosition = 0, BiPointSize = 1, BiClipDistance = 3, BiCullDistance = 4, BiFragCoord = 15,
                   BiFrontFacing = 17, BiVertexIndex = 42, BiInstanceIndex = 43 };
@@ -101,7 +115,7 @@ struct MemberDec {
 };
 
 struct Decor {
-    int location = -1, binding = -1, set = -1, builtin = -1;
+    int location = -1, binding = -1, set = -1, builtin = -1, array_stride = -1;
     bool flat = false, noperspective = false, block = false;
     std::map<uint32_t, MemberDec> members;
 };
@@ -109,6 +123,11 @@ struct Decor {
 struct Inst {
     uint32_t op;
     std::vector<uint32_t> w;   // operands
+};
+
+struct Function {
+    uint32_t ret = 0;                    // result type
+    std::vector<Inst> body;              // parameters, then blocks
 };
 
 struct Module {
@@ -120,9 +139,10 @@ struct Module {
     std::map<uint32_t, Decor> dec;
     std::map<uint32_t, std::pair<uint32_t, std::vector<uint32_t>>> consts;   // id -> (type, words or constituents)
     std::map<uint32_t, bool> const_composite;
+    std::set<uint32_t> const_null;                                           // OpConstantNull / module-level OpUndef
     std::map<uint32_t, std::pair<uint32_t, uint32_t>> globals;               // var id -> (pointer type, storage class)
-    std::vector<Inst> body;                                                  // the entry function's instructions
-    std::map<uint32_t, uint32_t> result_type;                                // id -> type id (body results)
+    std::map<uint32_t, uint32_t> global_init;                                // Private variable -> initializer
+    std::map<uint32_t, Function> functions;                                  // every function, by id
 };
 
 std::string lower(const char* s) { return s ? std::string(s) : std::string("main"); }
@@ -135,8 +155,7 @@ Module parse(const uint32_t* w, size_t n, const char* entry_name)
     bool in_fn = false, found = false;
     const std::string want = lower(entry_name);
     std::vector<Inst> fn_insts;
-    uint32_t fn_id = 0;
-    std::map<uint32_t, std::vector<Inst>> functions;
+    uint32_t fn_id = 0, fn_ret = 0;
     while (i < n) {
         const uint32_t len = w[i] >> 16, op = w[i] & 0xFFFFu;
         if (len == 0 || i + len > n) bad("truncated instruction");
@@ -144,7 +163,7 @@ Module parse(const uint32_t* w, size_t n, const char* entry_name)
         i += len;
         if (in_fn) {
             if (op == OpFunctionEnd) {
-                functions[fn_id] = fn_insts;
+                m.functions[fn_id] = Function{fn_ret, fn_insts};
                 fn_insts.clear();
                 in_fn = false;
             } else {
@@ -155,7 +174,7 @@ Module parse(const uint32_t* w, size_t n, const char* entry_name)
         const std::vector<uint32_t>& o = in.w;
         switch (op) {
         case OpCapability: case OpMemoryModel: case OpName: case OpMemberName: case OpSource: case OpSourceExtension:
-        case OpString: case OpLine: case OpNoLine: case OpModuleProcessed:
+        case OpString: case OpLine: case OpNoLine: case OpModuleProcessed: case OpExtension:
             break;
         case OpExtInstImport: {
             const char* s = (const char*)&o[1];
@@ -186,6 +205,7 @@ Module parse(const uint32_t* w, size_t n, const char* entry_name)
             case DecFlat: d.flat = true; break;
             case DecNoPerspective: d.noperspective = true; break;
             case DecBlock: d.block = true; break;
+            case DecArrayStride: d.array_stride = (int)o[2]; break;
             default: break;
             }
             break;
@@ -216,21 +236,29 @@ Module parse(const uint32_t* w, size_t n, const char* entry_name)
         case OpTypeFunction:
             m.types[o[0]] = Type{op, o[1], 0, {}};
             break;
-        case OpConstant:
+        // Specialization constants take their default values: programs are
+        // created without specialization data.
+        case OpConstant: case OpSpecConstant:
             m.consts[o[1]] = {o[0], {o[2]}};
             break;
-        case OpConstantTrue: case OpConstantFalse:
-            m.consts[o[1]] = {o[0], {op == OpConstantTrue ? 1u : 0u}};
+        case OpConstantTrue: case OpConstantFalse: case OpSpecConstantTrue: case OpSpecConstantFalse:
+            m.consts[o[1]] = {o[0], {(op == OpConstantTrue || op == OpSpecConstantTrue) ? 1u : 0u}};
             break;
-        case OpConstantComposite:
+        case OpConstantComposite: case OpSpecConstantComposite:
             m.consts[o[1]] = {o[0], std::vector<uint32_t>(o.begin() + 2, o.end())};
             m.const_composite[o[1]] = true;
             break;
+        case OpConstantNull: case OpUndef:
+            m.consts[o[1]] = {o[0], {}};
+            m.const_null.insert(o[1]);
+            break;
         case OpVariable:
             m.globals[o[1]] = {o[0], o[2]};
+            if (o.size() > 3) m.global_init[o[1]] = o[3];
             break;
         case OpFunction:
             in_fn = true;
+            fn_ret = o[0];
             fn_id = o[1];
             break;
         default: {
@@ -242,31 +270,58 @@ Module parse(const uint32_t* w, size_t n, const char* entry_name)
     }
     if (!found) bad("entry point '" + want + "' not found");
     if (m.model != 0 && m.model != 4) bad("only vertex and fragment entry points are translated");
-    if (functions.size() != 1) bad("shaders with function calls are not supported (expected only the entry function)");
-    if (!functions.count(m.entry)) bad("entry function body missing");
-    m.body = functions[m.entry];
-    for (const Inst& in : m.body) {
+    if (!m.functions.count(m.entry)) bad("entry function body missing");
+    for (const Inst& in : m.functions[m.entry].body)
         if (in.op == OpFunctionParameter) bad("entry function takes parameters");
-        if (in.op == OpFunctionCall) bad("OpFunctionCall is not supported");
-    }
     return m;
 }
 
 // ------------------------------------------------------------ code generation
 
+struct UPtr {
+    int binding = 0;
+    std::string off;           // byte offset expression from the start of the buffer
+    int mstride = 16;          // MatrixStride of the member the pointer is inside
+};
+
 struct Gen {
     const Module& m;
     bool vertex;
-    std::ostringstream decl, code, pro, epi;
+    std::ostringstream types, decl, fns, pro, epi, code;
+    std::ostringstream* od = &decl;          // declarations of the function being generated
+    std::ostringstream* oc = &code;          // statements of the function being generated
+    bool in_entry = true;
     std::map<uint32_t, std::string> lval;    // pointer id -> lvalue expression
     std::map<uint32_t, uint32_t> ptr_pointee;
+    std::map<uint32_t, UPtr> uptr;           // pointers into uniform buffers
+    std::set<uint32_t> structs_done;
     int out_total = 0;                       // vertex: varying components written
     std::map<int, int> loc_offset;           // location -> first varying component
     std::map<int, int> loc_size;
     bool discard = false;
     std::vector<int> interp;                 // fragment: mode per input component
+    std::map<uint32_t, uint32_t> rtype;      // value id -> type id, every function
 
-    Gen(const Module& mm, bool vs) : m(mm), vertex(vs) {}
+    Gen(const Module& mm, bool vs) : m(mm), vertex(vs)
+    {
+        // result types of every instruction, so a value may be used in a
+        // block that precedes its definition in the module's block order
+        for (const auto& f : m.functions)
+            for (const Inst& in : f.second.body)
+                if (has_result(in.op) && in.w.size() >= 2) rtype[in.w[1]] = in.w[0];
+    }
+
+    static bool has_result(uint32_t op)
+    {
+        switch (op) {
+        case OpLabel: case OpStore: case OpBranch: case OpBranchConditional: case OpSwitch: case OpReturn:
+        case OpReturnValue: case OpKill: case OpUnreachable: case OpSelectionMerge: case OpLoopMerge: case OpLine:
+        case OpNoLine:
+            return false;
+        default:
+            return true;
+        }
+    }
 
     const Type& T(uint32_t id) const
     {
@@ -284,15 +339,20 @@ struct Gen {
     }
 
     bool is_vec(uint32_t t) const { return T(t).op == OpTypeVector; }
+    uint32_t scalar_of(uint32_t t) const { return is_vec(t) ? T(t).a : t; }
+    bool is_float(uint32_t t) const { return T(scalar_of(t)).op == OpTypeFloat; }
+    bool is_int(uint32_t t) const { return T(scalar_of(t)).op == OpTypeInt; }
 
-    std::string ctype(uint32_t t) const
+    std::string ctype(uint32_t t)
     {
         const Type& ty = T(t);
         switch (ty.op) {
         case OpTypeVoid: return "void";
         case OpTypeBool: return "bool";
         case OpTypeFloat: return "float";
-        case OpTypeInt: return ty.b ? "int32_t" : "uint32_t";
+        case OpTypeInt:
+            if (ty.a != 32) bad("only 32-bit integers are supported");
+            return ty.b ? "int32_t" : "uint32_t";
         case OpTypeVector: return "V<" + ctype(ty.a) + ", " + std::to_string(ty.b) + ">";
         case OpTypeMatrix: {
             const Type& col = T(ty.a);
@@ -300,7 +360,17 @@ struct Gen {
             return "M<" + std::to_string(col.b) + ", " + std::to_string(ty.b) + ">";
         }
         case OpTypeArray: return "A<" + ctype(ty.a) + ", " + std::to_string(const_u32(ty.b)) + ">";
-        case OpTypeSampledImage: return "const TextureDesc*";
+        case OpTypeStruct: {
+            const std::string name = "S" + std::to_string(t);
+            if (!structs_done.count(t)) {
+                structs_done.insert(t);
+                std::string body;
+                for (size_t k = 0; k < ty.members.size(); k++) body += "        " + ctype(ty.members[k]) + " m" + std::to_string(k) + ";\n";
+                types << "    struct " << name << " {\n" << body << "    };\n";
+            }
+            return name;
+        }
+        case OpTypeImage: case OpTypeSampledImage: return "const TextureDesc*";
         default: bad("unsupported value type");
         }
     }
@@ -309,24 +379,26 @@ struct Gen {
     {
         auto it = m.consts.find(id);
         if (it == m.consts.end() || m.const_composite.count(id)) bad("expected a scalar constant");
+        if (m.const_null.count(id)) return 0;
         return it->second.second[0];
     }
 
-    std::string val(uint32_t id) const
+    std::string val(uint32_t id)
     {
         auto it = m.consts.find(id);
         if (it != m.consts.end()) return const_expr(id);
         return "r" + std::to_string(id);
     }
 
-    std::string const_expr(uint32_t id) const
+    std::string const_expr(uint32_t id)
     {
         const auto& c = m.consts.at(id);
         const Type& ty = T(c.first);
+        if (m.const_null.count(id)) return ctype(c.first) + "{}";
         if (m.const_composite.count(id)) {
-            std::string s = ctype(c.first) + "{{";
+            std::string s;
             for (size_t i = 0; i < c.second.size(); i++) s += (i ? ", " : "") + const_expr(c.second[i]);
-            return s + "}}";
+            return ty.op == OpTypeStruct ? ctype(c.first) + "{" + s + "}" : ctype(c.first) + "{{" + s + "}}";
         }
         const uint32_t v = c.second[0];
         char buf[64];
@@ -346,13 +418,74 @@ struct Gen {
         if (ty.op == OpTypeVector) { *et = ty.a; return base + ".c[" + idx + "]"; }
         if (ty.op == OpTypeMatrix) { *et = ty.a; return base + ".col[" + idx + "]"; }
         if (ty.op == OpTypeArray) { *et = ty.a; return base + ".e[" + idx + "]"; }
+        if (ty.op == OpTypeStruct) {
+            const uint32_t k = (uint32_t)std::stoul(idx);
+            if (k >= ty.members.size()) bad("struct member index");
+            *et = ty.members[k];
+            return base + ".m" + idx;
+        }
         bad("element access into a non-composite");
+    }
+
+    // element access with an index value id (constant for structs)
+    std::string elem_id(uint32_t t, const std::string& base, uint32_t idx, uint32_t* et)
+    {
+        if (T(t).op == OpTypeStruct) return elem(t, base, std::to_string(const_u32(idx)), et);
+        return elem(t, base, val(idx), et);
+    }
+
+    // a value of type t laid out in uniform buffer `b` at byte offset `off`
+    // (Offset, ArrayStride and MatrixStride decorations)
+    std::string uniform_load(uint32_t t, int b, const std::string& off, int mstride)
+    {
+        const Type& ty = T(t);
+        const std::string base = "env.uniform[" + std::to_string(b) + "]";
+        switch (ty.op) {
+        case OpTypeFloat: case OpTypeInt: case OpTypeVector:
+            if (T(scalar_of(t)).op == OpTypeBool) bad("bool in a uniform block");
+            return "(*(const " + ctype(t) + "*)(" + base + " + (" + off + ")))";
+        case OpTypeBool:
+            return "((*(const uint32_t*)(" + base + " + (" + off + "))) != 0u)";
+        case OpTypeMatrix: {
+            std::string s = ctype(t) + "{{";
+            for (uint32_t c = 0; c < ty.b; c++)
+                s += (c ? ", " : "") + uniform_load(ty.a, b, "(" + off + ") + " + std::to_string(c * mstride), mstride);
+            return s + "}}";
+        }
+        case OpTypeArray: {
+            auto d = m.dec.find(t);
+            if (d == m.dec.end() || d->second.array_stride <= 0) bad("uniform array without an ArrayStride");
+            std::string s = ctype(t) + "{{";
+            for (uint32_t i = 0; i < const_u32(ty.b); i++)
+                s += (i ? ", " : "") + uniform_load(ty.a, b, "(" + off + ") + " + std::to_string(i * (uint32_t)d->second.array_stride), mstride);
+            return s + "}}";
+        }
+        case OpTypeStruct: {
+            std::string s = ctype(t) + "{";
+            for (size_t k = 0; k < ty.members.size(); k++) {
+                const MemberDec md = member_dec(t, (uint32_t)k);
+                s += (k ? ", " : "") + uniform_load(ty.members[k], b, "(" + off + ") + " + std::to_string(md.offset),
+                                                    md.matrix_stride > 0 ? md.matrix_stride : 16);
+            }
+            return s + "}";
+        }
+        default: bad("unsupported type in a uniform block");
+        }
+    }
+
+    MemberDec member_dec(uint32_t st, uint32_t k) const
+    {
+        auto d = m.dec.find(st);
+        if (d == m.dec.end()) bad("uniform struct without member offsets");
+        auto md = d->second.members.find(k);
+        if (md == d->second.members.end() || md->second.offset < 0) bad("uniform member without an Offset");
+        return md->second;
     }
 
     void globals()
     {
         // varying layout: locations in increasing order, components packed
-        std::vector<std::pair<int, uint32_t>> io;   // (location, var id)
+        std::vector<std::pair<int, uint32_t>> io;   // (location * 2 + is_output, var id)
         for (const auto& g : m.globals) {
             const uint32_t var = g.first, ptype = g.second.first, sc = g.second.second;
             const uint32_t pointee = T(ptype).b;
@@ -387,6 +520,7 @@ struct Gen {
                     else if (vertex && sc == ScOutput && d.builtin == BiPosition)
                         epi << "    pos[0] = " << name << ".c[0]; pos[1] = " << name << ".c[1]; pos[2] = " << name << ".c[2]; pos[3] = "
                             << name << ".c[3];\n";
+                    else if (vertex && sc == ScOutput && d.builtin == BiPointSize) { /* points are not rasterised */ }
                     else if (!vertex && sc == ScInput && d.builtin == BiFragCoord)
                         pro << "    " << name << " = V<float, 4>{{fc[0], fc[1], fc[2], fc[3]}};\n";
                     else if (!vertex && sc == ScInput && d.builtin == BiFrontFacing) pro << "    " << name << " = front;\n";
@@ -394,8 +528,10 @@ struct Gen {
                     continue;
                 }
                 if (d.location < 0) bad("interface variable without a location");
-                const uint32_t et = is_vec(pointee) ? T(pointee).a : pointee;
-                if (T(et).op != OpTypeFloat) bad("only float interface variables are supported");
+                const uint32_t et = scalar_of(pointee);
+                if (T(et).op != OpTypeFloat && T(et).op != OpTypeInt) bad("only float and 32-bit integer interface variables are supported");
+                if (T(et).op == OpTypeInt && vertex && sc == ScInput) bad("integer vertex inputs are not supported");
+                if (T(et).op == OpTypeInt && sc == ScInput && !d.flat) bad("integer fragment inputs must be Flat");
                 decl << "    " << ctype(pointee) << " " << name << "{};\n";
                 lval[var] = name;
                 io.push_back({d.location * 2 + (sc == ScOutput ? 1 : 0), var});
@@ -404,7 +540,7 @@ struct Gen {
             if (sc == ScUniform) {
                 if (d.set > 0 || d.binding < 0 || d.binding > 3) bad("uniform blocks must be set 0, binding 0-3");
                 if (T(pointee).op != OpTypeStruct) bad("uniform variable is not a block");
-                lval[var] = "@U" + std::to_string(d.binding) + ":" + std::to_string(pointee);
+                uptr[var] = UPtr{d.binding, "0", 16};
                 continue;
             }
             if (sc == ScUniformConstant) {
@@ -416,6 +552,8 @@ struct Gen {
             }
             if (sc == ScPrivate) {
                 decl << "    " << ctype(pointee) << " " << name << "{};\n";
+                auto init = m.global_init.find(var);
+                if (init != m.global_init.end()) pro << "    " << name << " = " << val(init->second) << ";\n";
                 lval[var] = name;
                 continue;
             }
@@ -429,7 +567,12 @@ struct Gen {
             const int n = comps(pointee);
             const int loc = e.first / 2;
             const bool out = e.first & 1;
+            const bool integer = is_int(pointee);
             const std::string name = "g" + std::to_string(var);
+            auto sub = [&](int k) { return name + (is_vec(pointee) ? ".c[" + std::to_string(k) + "]" : ""); };
+            // integer varyings travel bit for bit in the float varying array
+            // (flat interpolation copies the provoking vertex's words)
+            auto store = [&](int k) { return integer ? "__uint_as_float((uint32_t)" + sub(k) + ")" : sub(k); };
             if (vertex && !out) {
                 // vertex attribute of `n` float components (EmitCopyInput, PipelineCompiler.cpp:821-896)
                 if (is_vec(pointee)) pro << "    fetch_attribute(env, " << loc << ", " << n << ", vid, iid, " << name << ".c);\n";
@@ -437,16 +580,17 @@ struct Gen {
             } else if (vertex && out) {
                 loc_offset[loc] = off;
                 loc_size[loc] = n;
-                for (int k = 0; k < n; k++)
-                    epi << "    out[" << off + k << "] = " << name << (is_vec(pointee) ? ".c[" + std::to_string(k) + "]" : "") << ";\n";
+                for (int k = 0; k < n; k++) epi << "    out[" << off + k << "] = " << store(k) << ";\n";
                 off += n;
             } else if (!vertex && !out) {
                 loc_offset[loc] = -1;
                 loc_size[loc] = n;
             } else {
+                // @setPixelF32 / U32 / I32 by the output's base type (PipelineCompiler.cpp:1528-1653):
+                // integer outputs reach an integer attachment as raw words
                 if (loc != 0) bad("fragment outputs other than location 0 are not supported");
-                if (n != 4) bad("the fragment colour output must be a vec4");
-                for (int k = 0; k < 4; k++) epi << "    out[" << k << "] = " << name << ".c[" << k << "];\n";
+                if (n != 4) bad("the fragment colour output must have four components");
+                for (int k = 0; k < 4; k++) epi << "    out[" << k << "] = " << store(k) << ";\n";
             }
         }
         out_total = off;
@@ -468,8 +612,13 @@ struct Gen {
             if (vs_size.at(d.location) < n) bad("fragment input wider than the vertex output at its location");
             const int off = it->second;
             const std::string name = "g" + std::to_string(var);
-            for (int k = 0; k < n; k++)
-                pro << "    " << name << (is_vec(pointee) ? ".c[" + std::to_string(k) + "]" : "") << " = in[" << off + k << "];\n";
+            const bool integer = is_int(pointee);
+            const std::string et = ctype(scalar_of(pointee));
+            for (int k = 0; k < n; k++) {
+                const std::string src = "in[" + std::to_string(off + k) + "]";
+                pro << "    " << name << (is_vec(pointee) ? ".c[" + std::to_string(k) + "]" : "") << " = "
+                    << (integer ? "(" + et + ")__float_as_uint(" + src + ")" : src) << ";\n";
+            }
             total = std::max(total, off + n);
             if ((int)interp.size() < off + n) interp.resize(off + n, 0);
             for (int k = 0; k < n; k++) interp[off + k] = d.flat ? 2 : d.noperspective ? 1 : 0;
@@ -485,96 +634,164 @@ struct Gen {
         std::string s = it->second;
         uint32_t t = ptr_pointee.at(base);
         size_t i = 0;
-        if (s.size() > 2 && s[0] == '@' && s[1] == 'U') {
-            // uniform block: byte offset arithmetic on env.uniform[binding]
-            const size_t colon = s.find(':');
-            const int binding = std::stoi(s.substr(2, colon - 2));
-            const uint32_t block = (uint32_t)std::stoul(s.substr(colon + 1));
-            if (idx.empty()) bad("load of a whole uniform block");
-            const uint32_t mi = const_u32(idx[0]);
-            const Type& bt = T(block);
-            if (mi >= bt.members.size()) bad("uniform member index");
-            const auto& md = m.dec.at(block).members;
-            auto mdi = md.find(mi);
-            if (mdi == md.end() || mdi->second.offset < 0) bad("uniform member without an offset");
-            std::string off = std::to_string(mdi->second.offset);
-            t = bt.members[mi];
-            if (T(t).op == OpTypeMatrix) {
-                const Type& col = T(T(t).a);
-                if (mdi->second.matrix_stride != (int)col.b * 4) bad("uniform matrix stride must equal 4 * rows");
-            }
-            for (i = 1; i < idx.size(); i++) {
-                const Type& ty = T(t);
-                const std::string ix = val(idx[i]);
-                if (ty.op == OpTypeMatrix) {
-                    off = "(" + off + " + 16 * (" + ix + "))";
-                    t = ty.a;
-                } else if (ty.op == OpTypeVector) {
-                    off = "(" + off + " + 4 * (" + ix + "))";
-                    t = ty.a;
-                } else {
-                    bad("unsupported uniform member type");
-                }
-            }
-            *out_t = t;
-            return "(*(const " + ctype(t) + "*)(env.uniform[" + std::to_string(binding) + "] + " + off + "))";
-        }
         const Type& bt = T(t);
-        if (bt.op == OpTypeStruct) {
-            // gl_PerVertex members
+        if (bt.op == OpTypeStruct && m.dec.count(t) && !m.dec.at(t).members.empty() &&
+            m.dec.at(t).members.begin()->second.builtin >= 0) {
+            // gl_PerVertex members are separate variables
             if (idx.empty()) bad("access to a whole block");
             const uint32_t mi = const_u32(idx[0]);
             s = s + "_m" + std::to_string(mi);
             t = bt.members.at(mi);
             i = 1;
         }
-        for (; i < idx.size(); i++) s = elem(t, s, val(idx[i]), &t);
+        for (; i < idx.size(); i++) s = elem_id(t, s, idx[i], &t);
         *out_t = t;
         return s;
     }
 
-    static bool is_float_op(uint32_t op)
+    // access chain into a uniform buffer: byte offset arithmetic
+    UPtr uchain(const UPtr& base, uint32_t t, const std::vector<uint32_t>& idx, uint32_t* out_t)
     {
-        return op == OpFAdd || op == OpFSub || op == OpFMul || op == OpFDiv || op == OpIAdd || op == OpISub || op == OpIMul ||
-               op == OpUDiv || op == OpSDiv;
-    }
-
-    static const char* binop(uint32_t op)
-    {
-        switch (op) {
-        case OpFAdd: case OpIAdd: return "+";
-        case OpFSub: case OpISub: return "-";
-        case OpFMul: case OpIMul: return "*";
-        case OpFDiv: case OpUDiv: case OpSDiv: return "/";
-        case OpFOrdEqual: case OpIEqual: return "==";
-        case OpFOrdNotEqual: case OpINotEqual: return "!=";
-        case OpFOrdLessThan: case OpULessThan: case OpSLessThan: return "<";
-        case OpFOrdGreaterThan: case OpUGreaterThan: case OpSGreaterThan: return ">";
-        case OpFOrdLessThanEqual: case OpULessThanEqual: case OpSLessThanEqual: return "<=";
-        case OpFOrdGreaterThanEqual: case OpUGreaterThanEqual: case OpSGreaterThanEqual: return ">=";
-        case OpLogicalAnd: return "&&";
-        case OpLogicalOr: return "||";
-        default: return nullptr;
+        UPtr p = base;
+        for (uint32_t ix : idx) {
+            const Type& ty = T(t);
+            if (ty.op == OpTypeStruct) {
+                const uint32_t k = const_u32(ix);
+                if (k >= ty.members.size()) bad("uniform member index");
+                const MemberDec md = member_dec(t, k);
+                p.off = "(" + p.off + " + " + std::to_string(md.offset) + ")";
+                if (md.matrix_stride > 0) p.mstride = md.matrix_stride;
+                t = ty.members[k];
+            } else if (ty.op == OpTypeArray) {
+                auto d = m.dec.find(t);
+                if (d == m.dec.end() || d->second.array_stride <= 0) bad("uniform array without an ArrayStride");
+                p.off = "(" + p.off + " + " + std::to_string(d->second.array_stride) + "u * (uint32_t)(" + val(ix) + "))";
+                t = ty.a;
+            } else if (ty.op == OpTypeMatrix) {
+                p.off = "(" + p.off + " + " + std::to_string(p.mstride) + "u * (uint32_t)(" + val(ix) + "))";
+                t = ty.a;
+            } else if (ty.op == OpTypeVector) {
+                p.off = "(" + p.off + " + 4u * (uint32_t)(" + val(ix) + "))";
+                t = ty.a;
+            } else {
+                bad("access chain into a uniform scalar");
+            }
         }
+        *out_t = t;
+        return p;
     }
 
     // component k of value id (scalar values broadcast)
-    std::string comp(uint32_t id, uint32_t t, int k) const
+    std::string comp(uint32_t id, uint32_t t, int k)
     {
         return is_vec(t) ? val(id) + ".c[" + std::to_string(k) + "]" : val(id);
     }
 
-    void body()
+    std::string res(uint32_t id, uint32_t t, int k) const
     {
+        return "r" + std::to_string(id) + (is_vec(t) ? ".c[" + std::to_string(k) + "]" : "");
+    }
+
+    uint32_t type_of(uint32_t id) const
+    {
+        auto c = m.consts.find(id);
+        if (c != m.consts.end()) return c->second.first;
+        auto r = rtype.find(id);
+        if (r != rtype.end()) return r->second;
+        bad("value " + std::to_string(id) + " has no type");
+    }
+
+    // Binary operations with the reference's LLVM semantics
+    // (SPIRVCompiler.cpp:2620-3091): integer arithmetic wraps (done in
+    // uint32_t), signed / unsigned division, remainder and comparison
+    // reinterpret the operand words, shift amounts are taken mod 32.
+    std::string binexpr(uint32_t op, const std::string& et, const std::string& a, const std::string& b)
+    {
+        auto U = [](const std::string& x) { return "(uint32_t)(" + x + ")"; };
+        auto S = [](const std::string& x) { return "(int32_t)(" + x + ")"; };
+        auto as = [&](const std::string& x) { return "(" + et + ")(" + x + ")"; };
+        switch (op) {
+        case OpFAdd: return a + " + " + b;
+        case OpFSub: return a + " - " + b;
+        case OpFMul: return a + " * " + b;
+        case OpFDiv: return a + " / " + b;
+        case OpFRem: return "fmodf(" + a + ", " + b + ")";
+        case OpFMod: return "spv_fmod(" + a + ", " + b + ")";
+        case OpIAdd: return as(U(a) + " + " + U(b));
+        case OpISub: return as(U(a) + " - " + U(b));
+        case OpIMul: return as(U(a) + " * " + U(b));
+        case OpUDiv: return as(U(a) + " / " + U(b));
+        case OpSDiv: return as(S(a) + " / " + S(b));
+        case OpUMod: return as(U(a) + " % " + U(b));
+        case OpSRem: return as(S(a) + " % " + S(b));
+        case OpSMod: return as(U(a) + " - " + U(b) + " * (uint32_t)(" + S(a) + " / " + S(b) + ")");   // :2696-2704
+        case OpIEqual: return U(a) + " == " + U(b);
+        case OpINotEqual: return U(a) + " != " + U(b);
+        case OpUGreaterThan: return U(a) + " > " + U(b);
+        case OpSGreaterThan: return S(a) + " > " + S(b);
+        case OpUGreaterThanEqual: return U(a) + " >= " + U(b);
+        case OpSGreaterThanEqual: return S(a) + " >= " + S(b);
+        case OpULessThan: return U(a) + " < " + U(b);
+        case OpSLessThan: return S(a) + " < " + S(b);
+        case OpULessThanEqual: return U(a) + " <= " + U(b);
+        case OpSLessThanEqual: return S(a) + " <= " + S(b);
+        case OpFOrdEqual: return a + " == " + b;
+        case OpFOrdNotEqual: return "(" + a + " < " + b + " || " + a + " > " + b + ")";
+        case OpFOrdLessThan: return a + " < " + b;
+        case OpFOrdGreaterThan: return a + " > " + b;
+        case OpFOrdLessThanEqual: return a + " <= " + b;
+        case OpFOrdGreaterThanEqual: return a + " >= " + b;
+        case OpFUnordEqual: return "!(" + a + " < " + b + " || " + a + " > " + b + ")";
+        case OpFUnordNotEqual: return a + " != " + b;
+        case OpFUnordLessThan: return "!(" + a + " >= " + b + ")";
+        case OpFUnordGreaterThan: return "!(" + a + " <= " + b + ")";
+        case OpFUnordLessThanEqual: return "!(" + a + " > " + b + ")";
+        case OpFUnordGreaterThanEqual: return "!(" + a + " < " + b + ")";
+        case OpLogicalAnd: return a + " && " + b;
+        case OpLogicalOr: return a + " || " + b;
+        case OpLogicalEqual: return a + " == " + b;
+        case OpLogicalNotEqual: return a + " != " + b;
+        case OpBitwiseAnd: return as(U(a) + " & " + U(b));
+        case OpBitwiseOr: return as(U(a) + " | " + U(b));
+        case OpBitwiseXor: return as(U(a) + " ^ " + U(b));
+        case OpShiftLeftLogical: return as(U(a) + " << (" + U(b) + " & 31u)");
+        case OpShiftRightLogical: return as(U(a) + " >> (" + U(b) + " & 31u)");
+        case OpShiftRightArithmetic: return as(S(a) + " >> (" + U(b) + " & 31u)");
+        default: return "";
+        }
+    }
+
+    void declare(uint32_t type, uint32_t id) { *od << "    " << ctype(type) << " r" << id << "{};\n"; }
+
+    // one function: the entry point's statements, or a lambda for a callee
+    void function(uint32_t fid, bool entry)
+    {
+        const Function& fn = m.functions.at(fid);
+        std::ostringstream fdecl, fcode;
+        od = entry ? &decl : &fdecl;
+        oc = entry ? &code : &fcode;
+        in_entry = entry;
         // OpPhi: per target block, per incoming block, the (result, value)
         // assignments made on that edge
         std::map<uint32_t, std::map<uint32_t, std::vector<std::pair<uint32_t, uint32_t>>>> phis;
+        std::vector<std::string> params;
         {
             uint32_t blk = 0;
-            for (const Inst& in : m.body) {
+            for (const Inst& in : fn.body) {
                 if (in.op == OpLabel) blk = in.w[0];
                 if (in.op == OpPhi)
                     for (size_t k = 2; k + 1 < in.w.size(); k += 2) phis[blk][in.w[k + 1]].push_back({in.w[1], in.w[k]});
+                if (in.op == OpFunctionParameter) {
+                    const Type& pt = T(in.w[0]);
+                    if (pt.op == OpTypePointer) {
+                        const std::string p = "p" + std::to_string(in.w[1]);
+                        params.push_back(ctype(pt.b) + "& " + p);
+                        lval[in.w[1]] = p;
+                        ptr_pointee[in.w[1]] = pt.b;
+                    } else {
+                        params.push_back(ctype(in.w[0]) + " r" + std::to_string(in.w[1]));
+                    }
+                }
             }
         }
         uint32_t cur = 0;
@@ -588,247 +805,610 @@ struct Gen {
             }
             return s + "goto L" + std::to_string(target) + ";";
         };
-        for (const Inst& in : m.body) {
-            const std::vector<uint32_t>& o = in.w;
-            const uint32_t op = in.op;
-            // declare results
-            auto result = [&](uint32_t type, uint32_t id) {
-                decl << "    " << ctype(type) << " r" << id << "{};\n";
-            };
-            switch (op) {
-            case OpLabel:
-                cur = o[0];
-                code << "L" << o[0] << ":;\n";
-                break;
-            case OpVariable: {
-                if (o[2] != ScFunction) bad("non-Function variable inside a function");
-                const uint32_t pointee = T(o[0]).b;
-                decl << "    " << ctype(pointee) << " v" << o[1] << "{};\n";
-                lval[o[1]] = "v" + std::to_string(o[1]);
-                ptr_pointee[o[1]] = pointee;
-                if (o.size() > 3) code << "    v" << o[1] << " = " << val(o[3]) << ";\n";
+        for (const Inst& in : fn.body) {
+            if (in.op == OpFunctionParameter) continue;
+            if (in.op == OpLabel) cur = in.w[0];
+            instruction(in, edge);
+        }
+        if (!entry) {
+            fns << "    auto f" << fid << " = [&](";
+            for (size_t k = 0; k < params.size(); k++) fns << (k ? ", " : "") << params[k];
+            fns << ") -> " << ctype(fn.ret) << " {\n" << fdecl.str() << fcode.str() << "    };\n";
+        }
+        od = &decl;
+        oc = &code;
+        in_entry = true;
+    }
+
+    // callees before callers, each once
+    void functions()
+    {
+        std::vector<uint32_t> order;
+        std::set<uint32_t> seen, active;
+        std::function<void(uint32_t)> visit = [&](uint32_t f) {
+            if (seen.count(f)) return;
+            if (active.count(f)) bad("recursive function calls");
+            active.insert(f);
+            auto it = m.functions.find(f);
+            if (it == m.functions.end()) bad("call of an undefined function");
+            for (const Inst& in : it->second.body)
+                if (in.op == OpFunctionCall) visit(in.w[2]);
+            active.erase(f);
+            seen.insert(f);
+            order.push_back(f);
+        };
+        visit(m.entry);
+        for (uint32_t f : order) function(f, f == m.entry);
+    }
+
+    template <class Edge>
+    void instruction(const Inst& in, Edge& edge)
+    {
+        const std::vector<uint32_t>& o = in.w;
+        const uint32_t op = in.op;
+        std::ostream& code = *oc;
+        switch (op) {
+        case OpLabel:
+            code << "L" << o[0] << ":;\n";
+            break;
+        case OpVariable: {
+            if (o[2] != ScFunction) bad("non-Function variable inside a function");
+            const uint32_t pointee = T(o[0]).b;
+            *od << "    " << ctype(pointee) << " v" << o[1] << "{};\n";
+            lval[o[1]] = "v" + std::to_string(o[1]);
+            ptr_pointee[o[1]] = pointee;
+            if (o.size() > 3) code << "    v" << o[1] << " = " << val(o[3]) << ";\n";
+            break;
+        }
+        case OpLoad: {
+            declare(o[0], o[1]);
+            auto u = uptr.find(o[2]);
+            if (u != uptr.end()) {
+                code << "    r" << o[1] << " = " << uniform_load(o[0], u->second.binding, u->second.off, u->second.mstride) << ";\n";
                 break;
             }
-            case OpLoad: {
-                auto it = lval.find(o[2]);
-                if (it == lval.end()) bad("load through an unknown pointer");
-                result(o[0], o[1]);
-                code << "    r" << o[1] << " = " << it->second << ";\n";
-                break;
+            auto it = lval.find(o[2]);
+            if (it == lval.end()) bad("load through an unknown pointer");
+            code << "    r" << o[1] << " = " << it->second << ";\n";
+            break;
+        }
+        case OpStore: {
+            auto it = lval.find(o[0]);
+            if (it == lval.end() || uptr.count(o[0])) bad("store through a read-only or unknown pointer");
+            code << "    " << it->second << " = " << val(o[1]) << ";\n";
+            break;
+        }
+        case OpAccessChain: {
+            uint32_t t = 0;
+            const std::vector<uint32_t> idx(o.begin() + 3, o.end());
+            auto u = uptr.find(o[2]);
+            if (u != uptr.end()) {
+                uptr[o[1]] = uchain(u->second, ptr_pointee.at(o[2]), idx, &t);
+            } else {
+                lval[o[1]] = chain(o[2], idx, &t);
             }
-            case OpStore: {
-                auto it = lval.find(o[0]);
-                if (it == lval.end() || it->second[0] == '(') bad("store through a read-only or unknown pointer");
-                code << "    " << it->second << " = " << val(o[1]) << ";\n";
-                break;
-            }
-            case OpAccessChain: {
-                uint32_t t = 0;
-                const std::string s = chain(o[2], std::vector<uint32_t>(o.begin() + 3, o.end()), &t);
-                lval[o[1]] = s;
-                ptr_pointee[o[1]] = t;
-                break;
-            }
-            case OpCompositeExtract: {
-                result(o[0], o[1]);
-                uint32_t t = m.result_type.count(o[2]) ? m.result_type.at(o[2]) : type_of(o[2]);
-                std::string s = val(o[2]);
-                for (size_t k = 3; k < o.size(); k++) s = elem(t, s, std::to_string(o[k]), &t);
-                code << "    r" << o[1] << " = " << s << ";\n";
-                break;
-            }
-            case OpCompositeConstruct: {
-                result(o[0], o[1]);
-                const Type& rt = T(o[0]);
-                if (rt.op == OpTypeVector) {
-                    int k = 0;
-                    for (size_t a = 2; a < o.size(); a++) {
-                        const uint32_t at = type_of(o[a]);
-                        for (int c = 0; c < comps(at); c++)
-                            code << "    r" << o[1] << ".c[" << k++ << "] = " << comp(o[a], at, c) << ";\n";
-                    }
-                } else if (rt.op == OpTypeMatrix) {
-                    for (size_t a = 2; a < o.size(); a++) code << "    r" << o[1] << ".col[" << a - 2 << "] = " << val(o[a]) << ";\n";
-                } else {
-                    bad("composite construct of an unsupported type");
+            ptr_pointee[o[1]] = t;
+            break;
+        }
+        case OpCopyObject:
+            declare(o[0], o[1]);
+            code << "    r" << o[1] << " = " << val(o[2]) << ";\n";
+            break;
+        case OpUndef:
+            declare(o[0], o[1]);
+            break;
+        case OpCompositeExtract: {
+            declare(o[0], o[1]);
+            uint32_t t = type_of(o[2]);
+            std::string s = val(o[2]);
+            for (size_t k = 3; k < o.size(); k++) s = elem(t, s, std::to_string(o[k]), &t);
+            code << "    r" << o[1] << " = " << s << ";\n";
+            break;
+        }
+        case OpCompositeInsert: {
+            declare(o[0], o[1]);
+            uint32_t t = o[0];
+            std::string s = "r" + std::to_string(o[1]);
+            for (size_t k = 4; k < o.size(); k++) s = elem(t, s, std::to_string(o[k]), &t);
+            code << "    r" << o[1] << " = " << val(o[3]) << ";\n    " << s << " = " << val(o[2]) << ";\n";
+            break;
+        }
+        case OpVectorExtractDynamic: {
+            // an index past the end is undefined; it reads the last component
+            declare(o[0], o[1]);
+            const int n = comps(type_of(o[2]));
+            code << "    r" << o[1] << " = " << val(o[2]) << ".c[min((uint32_t)(" << val(o[3]) << "), " << n - 1 << "u)];\n";
+            break;
+        }
+        case OpVectorInsertDynamic: {
+            declare(o[0], o[1]);
+            const int n = comps(o[0]);
+            code << "    r" << o[1] << " = " << val(o[2]) << ";\n    { const uint32_t i_ = (uint32_t)(" << val(o[4]) << "); if (i_ < "
+                 << n << "u) r" << o[1] << ".c[i_] = " << val(o[3]) << "; }\n";
+            break;
+        }
+        case OpCompositeConstruct: {
+            declare(o[0], o[1]);
+            const Type& rt = T(o[0]);
+            if (rt.op == OpTypeVector) {
+                int k = 0;
+                for (size_t a = 2; a < o.size(); a++) {
+                    const uint32_t at = type_of(o[a]);
+                    for (int c = 0; c < comps(at); c++) code << "    r" << o[1] << ".c[" << k++ << "] = " << comp(o[a], at, c) << ";\n";
                 }
-                break;
+            } else if (rt.op == OpTypeMatrix) {
+                for (size_t a = 2; a < o.size(); a++) code << "    r" << o[1] << ".col[" << a - 2 << "] = " << val(o[a]) << ";\n";
+            } else if (rt.op == OpTypeArray) {
+                for (size_t a = 2; a < o.size(); a++) code << "    r" << o[1] << ".e[" << a - 2 << "] = " << val(o[a]) << ";\n";
+            } else if (rt.op == OpTypeStruct) {
+                for (size_t a = 2; a < o.size(); a++) code << "    r" << o[1] << ".m" << a - 2 << " = " << val(o[a]) << ";\n";
+            } else {
+                bad("composite construct of an unsupported type");
             }
-            case OpVectorShuffle: {
-                result(o[0], o[1]);
-                const uint32_t t1 = type_of(o[2]), t2 = type_of(o[3]);
-                const int n1 = comps(t1);
-                for (size_t k = 4; k < o.size(); k++) {
-                    const uint32_t s = o[k];
-                    if (s == 0xFFFFFFFFu) continue;
-                    code << "    r" << o[1] << ".c[" << k - 4 << "] = "
-                         << ((int)s < n1 ? comp(o[2], t1, (int)s) : comp(o[3], t2, (int)s - n1)) << ";\n";
+            break;
+        }
+        case OpVectorShuffle: {
+            declare(o[0], o[1]);
+            const uint32_t t1 = type_of(o[2]), t2 = type_of(o[3]);
+            const int n1 = comps(t1);
+            for (size_t k = 4; k < o.size(); k++) {
+                const uint32_t s = o[k];
+                if (s == 0xFFFFFFFFu) continue;
+                code << "    r" << o[1] << ".c[" << k - 4 << "] = " << ((int)s < n1 ? comp(o[2], t1, (int)s) : comp(o[3], t2, (int)s - n1))
+                     << ";\n";
+            }
+            break;
+        }
+        case OpFNegate: case OpSNegate: case OpNot: case OpLogicalNot: case OpBitReverse: case OpBitCount: case OpIsNan:
+        case OpIsInf: {
+            declare(o[0], o[1]);
+            const uint32_t st = type_of(o[2]);
+            const std::string et = ctype(scalar_of(o[0]));
+            for (int k = 0; k < comps(o[0]); k++) {
+                const std::string a = comp(o[2], st, k);
+                std::string e;
+                switch (op) {
+                case OpFNegate: e = "-" + a; break;
+                case OpSNegate: e = "(" + et + ")(0u - (uint32_t)(" + a + "))"; break;
+                case OpNot: e = "(" + et + ")(~(uint32_t)(" + a + "))"; break;
+                case OpLogicalNot: e = "!" + a; break;
+                case OpBitReverse: e = "(" + et + ")__builtin_bitreverse32((uint32_t)(" + a + "))"; break;
+                case OpBitCount: e = "(" + et + ")__builtin_popcount((uint32_t)(" + a + "))"; break;
+                case OpIsNan: e = "(" + a + " != " + a + ")"; break;
+                default: e = "__builtin_isinf(" + a + ")"; break;
                 }
-                break;
+                code << "    " << res(o[1], o[0], k) << " = " << e << ";\n";
             }
-            case OpFNegate: case OpSNegate: {
-                result(o[0], o[1]);
-                const int n = comps(o[0]);
-                for (int k = 0; k < n; k++)
-                    code << "    r" << o[1] << (is_vec(o[0]) ? ".c[" + std::to_string(k) + "]" : "") << " = -" << comp(o[2], o[0], k) << ";\n";
-                break;
+            break;
+        }
+        case OpAny: case OpAll: {
+            declare(o[0], o[1]);
+            const uint32_t st = type_of(o[2]);
+            code << "    r" << o[1] << " = ";
+            for (int k = 0; k < comps(st); k++) code << (k ? (op == OpAny ? " || " : " && ") : "") << comp(o[2], st, k);
+            code << ";\n";
+            break;
+        }
+        case OpConvertFToS: case OpConvertFToU: case OpConvertSToF: case OpConvertUToF: case OpBitcast: case OpUConvert:
+        case OpSConvert: case OpFConvert: case OpQuantizeToF16: {
+            declare(o[0], o[1]);
+            const int n = comps(o[0]);
+            const uint32_t st = type_of(o[2]);
+            if (comps(st) != n) bad("conversion between vectors of different sizes");
+            const std::string dt = ctype(scalar_of(o[0])), sty = ctype(scalar_of(st));
+            for (int k = 0; k < n; k++) {
+                const std::string src = comp(o[2], st, k);
+                std::string e;
+                if (op == OpConvertSToF) e = "(float)(int32_t)(" + src + ")";
+                else if (op == OpConvertUToF) e = "(float)(uint32_t)(" + src + ")";
+                else if (op == OpQuantizeToF16) e = "(float)(_Float16)(" + src + ")";
+                else if (op == OpBitcast && dt == "float" && sty != "float") e = "__uint_as_float((uint32_t)(" + src + "))";
+                else if (op == OpBitcast && dt != "float" && sty == "float") e = "(" + dt + ")__float_as_uint(" + src + ")";
+                else e = "(" + dt + ")(" + src + ")";
+                code << "    " << res(o[1], o[0], k) << " = " << e << ";\n";
             }
-            case OpLogicalNot: {
-                result(o[0], o[1]);
-                const int n = comps(o[0]);
-                for (int k = 0; k < n; k++)
-                    code << "    r" << o[1] << (is_vec(o[0]) ? ".c[" + std::to_string(k) + "]" : "") << " = !" << comp(o[2], o[0], k) << ";\n";
-                break;
-            }
-            case OpConvertFToS: case OpConvertFToU: case OpConvertSToF: case OpConvertUToF: case OpBitcast: {
-                result(o[0], o[1]);
-                const int n = comps(o[0]);
-                const uint32_t et = is_vec(o[0]) ? T(o[0]).a : o[0];
-                const uint32_t st = type_of(o[2]);
-                for (int k = 0; k < n; k++) {
-                    const std::string src = comp(o[2], st, k);
-                    const std::string dt = ctype(et), sty = ctype(is_vec(st) ? T(st).a : st);
-                    std::string e = "(" + dt + ")(" + src + ")";
-                    if (op == OpBitcast && dt == "float" && sty != "float") e = "__uint_as_float((uint32_t)(" + src + "))";
-                    if (op == OpBitcast && dt != "float" && sty == "float") e = "(" + dt + ")__float_as_uint(" + src + ")";
-                    code << "    r" << o[1] << (is_vec(o[0]) ? ".c[" + std::to_string(k) + "]" : "") << " = " << e << ";\n";
-                }
-                break;
-            }
-            case OpVectorTimesScalar: {
-                result(o[0], o[1]);
-                for (int k = 0; k < comps(o[0]); k++)
-                    code << "    r" << o[1] << ".c[" << k << "] = " << val(o[2]) << ".c[" << k << "] * " << val(o[3]) << ";\n";
-                break;
-            }
-            case OpMatrixTimesVector: {
-                result(o[0], o[1]);
-                const Type& mt = T(type_of(o[2]));
-                if (mt.b != 4 || T(mt.a).b != 4) bad("OpMatrixTimesVector is supported for mat4 * vec4");
+            break;
+        }
+        case OpVectorTimesScalar: {
+            declare(o[0], o[1]);
+            for (int k = 0; k < comps(o[0]); k++)
+                code << "    r" << o[1] << ".c[" << k << "] = " << val(o[2]) << ".c[" << k << "] * " << val(o[3]) << ";\n";
+            break;
+        }
+        case OpMatrixTimesScalar: {
+            // glm m * s, column by column (SpirvFunctions.cpp:10-21)
+            declare(o[0], o[1]);
+            const Type& mt = T(o[0]);
+            for (uint32_t c = 0; c < mt.b; c++)
+                for (uint32_t r = 0; r < T(mt.a).b; r++)
+                    code << "    r" << o[1] << ".col[" << c << "].c[" << r << "] = " << val(o[2]) << ".col[" << c << "].c[" << r << "] * "
+                         << val(o[3]) << ";\n";
+            break;
+        }
+        case OpMatrixTimesVector: {
+            declare(o[0], o[1]);
+            const Type& mt = T(type_of(o[2]));
+            const uint32_t cols = mt.b, rows = T(mt.a).b;
+            if (cols == 4 && rows == 4) {
+                // glm mat4 * vec4: (m0 v0 + m1 v1) + (m2 v2 + m3 v3) (SpirvFunctions.cpp:37-47)
                 code << "    mat4_mul_vec4(" << val(o[2]) << ".col[0].c, " << val(o[3]) << ".c, r" << o[1] << ".c);\n";
-                break;
-            }
-            case OpDot: {
-                result(o[0], o[1]);
-                const int n = comps(type_of(o[2]));
-                code << "    r" << o[1] << " = dot_n(" << val(o[2]) << ".c, " << val(o[3]) << ".c, " << n << ");\n";
-                break;
-            }
-            case OpSelect: {
-                result(o[0], o[1]);
-                const uint32_t ct = type_of(o[2]);
-                for (int k = 0; k < comps(o[0]); k++)
-                    code << "    r" << o[1] << (is_vec(o[0]) ? ".c[" + std::to_string(k) + "]" : "") << " = " << comp(o[2], ct, k)
-                         << " ? " << comp(o[3], o[0], k) << " : " << comp(o[4], o[0], k) << ";\n";
-                break;
-            }
-            case OpImageSampleImplicitLod: {
-                result(o[0], o[1]);
-                if (o.size() > 4) bad("image operands on OpImageSampleImplicitLod are not supported");
-                code << "    sample_implicit(*" << val(o[2]) << ", " << val(o[3]) << ".c[0], " << val(o[3]) << ".c[1], r" << o[1] << ".c);\n";
-                break;
-            }
-            case OpExtInst: {
-                if (o[2] != m.glsl) bad("extended instruction from an unknown set");
-                result(o[0], o[1]);
-                ext(o);
-                break;
-            }
-            case OpPhi:
-                result(o[0], o[1]);
-                break;
-            case OpSelectionMerge: case OpLoopMerge:
-                break;
-            case OpBranch:
-                code << "    " << edge(o[0]) << "\n";
-                break;
-            case OpBranchConditional:
-                code << "    if (" << val(o[0]) << ") { " << edge(o[1]) << " } else { " << edge(o[2]) << " }\n";
-                break;
-            case OpKill:
-                if (vertex) bad("OpKill in a vertex shader");
-                discard = true;
-                code << "    return true;\n";
-                break;
-            case OpReturn:
-                code << "    goto L_end;\n";
-                break;
-            default: {
-                const char* b = binop(op);
-                if (!b) {
-                    std::ostringstream e;
-                    e << "unsupported instruction, opcode " << op;
-                    bad(e.str());
+            } else {
+                // glm mat3 * vec3 and the other shapes: left to right
+                for (uint32_t r = 0; r < rows; r++) {
+                    code << "    r" << o[1] << ".c[" << r << "] = ";
+                    for (uint32_t c = 0; c < cols; c++)
+                        code << (c ? " + " : "") << val(o[2]) << ".col[" << c << "].c[" << r << "] * " << val(o[3]) << ".c[" << c << "]";
+                    code << ";\n";
                 }
-                result(o[0], o[1]);
-                const uint32_t at = type_of(o[2]), bt2 = type_of(o[3]);
-                const int n = std::max(comps(at), comps(bt2));
-                for (int k = 0; k < n; k++)
-                    code << "    r" << o[1] << (is_vec(o[0]) ? ".c[" + std::to_string(k) + "]" : "") << " = " << comp(o[2], at, k) << " "
-                         << b << " " << comp(o[3], bt2, k) << ";\n";
-                break;
             }
+            break;
+        }
+        case OpVectorTimesMatrix: {
+            // glm v * m: row j = m[j][0] v0 + m[j][1] v1 + ..., left to right (SpirvFunctions.cpp:23-34)
+            declare(o[0], o[1]);
+            const Type& mt = T(type_of(o[3]));
+            const uint32_t cols = mt.b, rows = T(mt.a).b;
+            for (uint32_t c = 0; c < cols; c++) {
+                code << "    r" << o[1] << ".c[" << c << "] = ";
+                for (uint32_t r = 0; r < rows; r++)
+                    code << (r ? " + " : "") << val(o[3]) << ".col[" << c << "].c[" << r << "] * " << val(o[2]) << ".c[" << r << "]";
+                code << ";\n";
             }
-            if (op != OpLabel && op != OpVariable && op != OpStore && op != OpAccessChain && op != OpSelectionMerge &&
-                op != OpLoopMerge && op != OpBranch && op != OpBranchConditional && op != OpKill && op != OpReturn && o.size() >= 2) {
-                const_cast<Module&>(m).result_type[o[1]] = o[0];
+            break;
+        }
+        case OpMatrixTimesMatrix: {
+            // glm m1 * m2: column j = m1[0] m2[j][0] + m1[1] m2[j][1] + ..., left to right (SpirvFunctions.cpp:49-60)
+            declare(o[0], o[1]);
+            const Type& at = T(type_of(o[2]));
+            const Type& bt = T(type_of(o[3]));
+            const uint32_t rows = T(at.a).b, inner = at.b, cols = bt.b;
+            for (uint32_t c = 0; c < cols; c++)
+                for (uint32_t r = 0; r < rows; r++) {
+                    code << "    r" << o[1] << ".col[" << c << "].c[" << r << "] = ";
+                    for (uint32_t k = 0; k < inner; k++)
+                        code << (k ? " + " : "") << val(o[2]) << ".col[" << k << "].c[" << r << "] * " << val(o[3]) << ".col[" << c << "].c["
+                             << k << "]";
+                    code << ";\n";
+                }
+            break;
+        }
+        case OpOuterProduct: {
+            declare(o[0], o[1]);
+            const Type& mt = T(o[0]);
+            for (uint32_t c = 0; c < mt.b; c++)
+                for (uint32_t r = 0; r < T(mt.a).b; r++)
+                    code << "    r" << o[1] << ".col[" << c << "].c[" << r << "] = " << val(o[2]) << ".c[" << r << "] * " << val(o[3]) << ".c["
+                         << c << "];\n";
+            break;
+        }
+        case OpTranspose: {
+            declare(o[0], o[1]);
+            const Type& mt = T(o[0]);
+            for (uint32_t c = 0; c < mt.b; c++)
+                for (uint32_t r = 0; r < T(mt.a).b; r++)
+                    code << "    r" << o[1] << ".col[" << c << "].c[" << r << "] = " << val(o[2]) << ".col[" << r << "].c[" << c << "];\n";
+            break;
+        }
+        case OpDot: {
+            declare(o[0], o[1]);
+            const int n = comps(type_of(o[2]));
+            code << "    r" << o[1] << " = dot_n(" << val(o[2]) << ".c, " << val(o[3]) << ".c, " << n << ");\n";
+            break;
+        }
+        case OpSelect: {
+            declare(o[0], o[1]);
+            const uint32_t ct = type_of(o[2]);
+            const Type& rt = T(o[0]);
+            if (rt.op != OpTypeVector || !is_vec(ct)) {
+                code << "    r" << o[1] << " = " << val(o[2]) << " ? " << val(o[3]) << " : " << val(o[4]) << ";\n";
+            } else {
+                for (int k = 0; k < comps(o[0]); k++)
+                    code << "    " << res(o[1], o[0], k) << " = " << comp(o[2], ct, k) << " ? " << comp(o[3], o[0], k) << " : "
+                         << comp(o[4], o[0], k) << ";\n";
             }
+            break;
+        }
+        case OpBitFieldInsert: case OpBitFieldSExtract: case OpBitFieldUExtract: {
+            declare(o[0], o[1]);
+            const uint32_t bt = type_of(o[2]);
+            const std::string et = ctype(scalar_of(o[0]));
+            for (int k = 0; k < comps(o[0]); k++) {
+                if (op == OpBitFieldInsert)
+                    code << "    " << res(o[1], o[0], k) << " = (" << et << ")spv_bfi((uint32_t)(" << comp(o[2], bt, k) << "), (uint32_t)("
+                         << comp(o[3], type_of(o[3]), k) << "), (uint32_t)(" << val(o[4]) << "), (uint32_t)(" << val(o[5]) << "));\n";
+                else
+                    code << "    " << res(o[1], o[0], k) << " = (" << et << ")" << (op == OpBitFieldSExtract ? "spv_bfe_s" : "spv_bfe_u")
+                         << "((uint32_t)(" << comp(o[2], bt, k) << "), (uint32_t)(" << val(o[3]) << "), (uint32_t)(" << val(o[4]) << "));\n";
+            }
+            break;
+        }
+        case OpSampledImage:
+            bad("separate images and samplers are not supported (use combined image samplers)");
+        case OpImage:
+            declare(o[0], o[1]);
+            code << "    r" << o[1] << " = " << val(o[2]) << ";\n";
+            break;
+        case OpImageSampleImplicitLod: {
+            declare(o[0], o[1]);
+            if (o.size() > 4) bad("image operands on OpImageSampleImplicitLod are not supported");
+            code << "    sample_implicit(*" << val(o[2]) << ", " << val(o[3]) << ".c[0], " << val(o[3]) << ".c[1], r" << o[1] << ".c);\n";
+            break;
+        }
+        case OpImageSampleExplicitLod: {
+            // ImageSampleExplicitLod (GlslFunctions.cpp:598-654): Lod operand only
+            declare(o[0], o[1]);
+            if (o.size() != 6 || o[4] != 0x2u) bad("OpImageSampleExplicitLod needs exactly the Lod image operand");
+            code << "    sample_explicit(*" << val(o[2]) << ", " << val(o[3]) << ".c[0], " << val(o[3]) << ".c[1], " << val(o[5]) << ", r"
+                 << o[1] << ".c);\n";
+            break;
+        }
+        case OpImageFetch: {
+            // ImageFetch (GlslFunctions.cpp:673-734): the view's base level, a Lod operand is not used
+            declare(o[0], o[1]);
+            if (!is_float(o[0])) bad("OpImageFetch of a non-float image");
+            code << "    fetch_texel(*" << val(o[2]) << ", (int32_t)" << val(o[3]) << ".c[0], (int32_t)" << val(o[3]) << ".c[1], r" << o[1]
+                 << ".c);\n";
+            break;
+        }
+        case OpImageQuerySize: case OpImageQuerySizeLod: {
+            declare(o[0], o[1]);
+            const std::string lod = op == OpImageQuerySizeLod ? "(uint32_t)(" + val(o[3]) + ")" : "0u";
+            code << "    { const uint32_t l_ = min(" << lod << ", 15u); r" << o[1] << ".c[0] = (int32_t)" << val(o[2]) << "->level_w[l_]; r"
+                 << o[1] << ".c[1] = (int32_t)" << val(o[2]) << "->level_h[l_]; }\n";
+            break;
+        }
+        case OpImageQueryLevels:
+            declare(o[0], o[1]);
+            code << "    r" << o[1] << " = (int32_t)" << val(o[2]) << "->level_count;\n";
+            break;
+        case OpExtInst: {
+            if (o[2] != m.glsl) bad("extended instruction from an unknown set");
+            declare(o[0], o[1]);
+            ext(o);
+            break;
+        }
+        case OpFunctionCall: {
+            if (o[0] != 0 && T(o[0]).op != OpTypeVoid) declare(o[0], o[1]);
+            std::string args;
+            for (size_t k = 3; k < o.size(); k++) {
+                std::string a;
+                if (uptr.count(o[k])) bad("a pointer into a uniform buffer passed to a function");
+                const bool pointer = m.globals.count(o[k]) || (rtype.count(o[k]) && T(rtype.at(o[k])).op == OpTypePointer);
+                if (pointer) {
+                    auto it = lval.find(o[k]);
+                    if (it == lval.end()) bad("function argument through an unknown pointer");
+                    a = it->second;
+                } else {
+                    a = val(o[k]);
+                }
+                args += (k > 3 ? ", " : "") + a;
+            }
+            code << "    " << (T(o[0]).op == OpTypeVoid ? "" : "r" + std::to_string(o[1]) + " = ") << "f" << o[2] << "(" << args << ");\n";
+            break;
+        }
+        case OpPhi:
+            declare(o[0], o[1]);
+            break;
+        case OpSelectionMerge: case OpLoopMerge: case OpLine: case OpNoLine:
+            break;
+        case OpBranch:
+            code << "    " << edge(o[0]) << "\n";
+            break;
+        case OpBranchConditional:
+            code << "    if (" << val(o[0]) << ") { " << edge(o[1]) << " } else { " << edge(o[2]) << " }\n";
+            break;
+        case OpSwitch: {
+            code << "    switch ((uint32_t)(" << val(o[0]) << ")) {\n";
+            for (size_t k = 2; k + 1 < o.size(); k += 2) code << "    case " << o[k] << "u: " << edge(o[k + 1]) << "\n";
+            code << "    default: " << edge(o[1]) << "\n    }\n";
+            break;
+        }
+        case OpKill:
+            if (vertex) bad("OpKill in a vertex shader");
+            if (!in_entry) bad("OpKill inside a called function is not supported");
+            discard = true;
+            code << "    return true;\n";
+            break;
+        case OpReturn:
+            code << (in_entry ? "    goto L_end;\n" : "    return;\n");
+            break;
+        case OpReturnValue:
+            if (in_entry) bad("OpReturnValue in the entry point");
+            code << "    return " << val(o[0]) << ";\n";
+            break;
+        case OpUnreachable:
+            code << (in_entry ? "    goto L_end;\n" : "    __builtin_unreachable();\n");
+            break;
+        default: {
+            if (o.size() < 4 || binexpr(op, "T", "a", "b").empty()) {
+                std::ostringstream e;
+                e << "unsupported instruction, opcode " << op;
+                bad(e.str());
+            }
+            const uint32_t at = type_of(o[2]), bt2 = type_of(o[3]);
+            const std::string et = ctype(scalar_of(o[0]));
+            declare(o[0], o[1]);
+            const int n = std::max(comps(at), comps(bt2));
+            for (int k = 0; k < n; k++)
+                code << "    " << res(o[1], o[0], k) << " = " << binexpr(op, et, comp(o[2], at, k), comp(o[3], bt2, k)) << ";\n";
+            break;
+        }
         }
     }
 
-    uint32_t type_of(uint32_t id) const
-    {
-        auto c = m.consts.find(id);
-        if (c != m.consts.end()) return c->second.first;
-        auto r = m.result_type.find(id);
-        if (r != m.result_type.end()) return r->second;
-        bad("value " + std::to_string(id) + " used before its definition");
-    }
-
+    // GLSL.std.450.  The reference registers host functions for FAbs, SAbs,
+    // SSign, Sin, Cos, Pow, [FUS]Min, [FUS]Max, [FUS]Clamp, FMix, Normalize,
+    // Reflect, FindILsb, FindSMsb, FindUMsb and NMin / NMax / NClamp
+    // (GlslFunctions.cpp:930-1300); those follow its code.  The others
+    // follow the GLSL.std.450 specification (glm 0.9.5.3 where glm defines
+    // them: length, distance, cross, faceforward, refract).
     void ext(const std::vector<uint32_t>& o)
     {
+        std::ostream& code = *oc;
         const uint32_t inst = o[3];
         const uint32_t rt = o[0];
         const std::string r = "r" + std::to_string(o[1]);
-        const int n = comps(rt);
+        const int n = comps_or_one(rt);
         auto c = [&](size_t a, int k) { return comp(o[a], type_of(o[a]), k); };
-        auto each = [&](const std::string& f) {
-            for (int k = 0; k < n; k++) code << "    " << r << (is_vec(rt) ? ".c[" + std::to_string(k) + "]" : "") << " = " << f.substr(0, f.find('#'))
-                                             << c(4, k) << f.substr(f.find('#') + 1) << ";\n";
+        auto out = [&](int k) { return r + (is_vec(rt) ? ".c[" + std::to_string(k) + "]" : ""); };
+        auto each = [&](const std::string& f, int args) {
+            // f with #0, #1, #2 standing for the arguments' components
+            for (int k = 0; k < n; k++) {
+                std::string e = f;
+                for (int a = 0; a < args; a++) {
+                    const std::string tag = "#" + std::to_string(a);
+                    for (size_t p = e.find(tag); p != std::string::npos; p = e.find(tag, p + 1)) e.replace(p, tag.size(), "(" + c(4 + a, k) + ")");
+                }
+                code << "    " << out(k) << " = " << e << ";\n";
+            }
+        };
+        const std::string et = is_vec(rt) || T(rt).op != OpTypeStruct ? ctype(scalar_of(rt)) : "";
+        auto sumsq = [&](uint32_t id) {
+            // x.x * x.x + x.y * x.y + ..., left to right (glm length / normalize)
+            const uint32_t t = type_of(id);
+            std::string s;
+            for (int k = 0; k < comps(t); k++) s += (k ? " + " : "") + comp(id, t, k) + " * " + comp(id, t, k);
+            return s;
         };
         switch (inst) {
-        case 4: each("fabsf(#)"); break;                                                    // FAbs
-        case 8: each("floorf(#)"); break;                                                   // Floor
-        case 10: each("spv_fract(#)"); break;                                               // Fract
-        case 31: each("sqrtf(#)"); break;                                                   // Sqrt
-        case 32: each("1.0f / sqrtf(#)"); break;                                            // InverseSqrt
-        case 37: case 40: case 43:                                                          // FMin FMax FClamp
-            for (int k = 0; k < n; k++) {
-                code << "    " << r << (is_vec(rt) ? ".c[" + std::to_string(k) + "]" : "") << " = "
-                     << (inst == 37 ? "spv_fmin(" : inst == 40 ? "spv_fmax(" : "spv_fclamp(") << c(4, k) << ", " << c(5, k);
-                if (inst == 43) code << ", " << c(6, k);
-                code << ");\n";
-            }
-            break;
-        case 66: {                                                                          // Length
-            const uint32_t at = type_of(o[4]);
-            if (is_vec(at)) code << "    " << r << " = sqrtf(dot_n(" << val(o[4]) << ".c, " << val(o[4]) << ".c, " << comps(at) << "));\n";
-            else code << "    " << r << " = fabsf(" << val(o[4]) << ");\n";
+        case 1: each("roundf(#0)", 1); break;                                               // Round
+        case 2: each("rintf(#0)", 1); break;                                                // RoundEven
+        case 3: each("truncf(#0)", 1); break;                                               // Trunc
+        case 4: each("fabsf(#0)", 1); break;                                                // FAbs
+        case 5: each("(int32_t)((int32_t)#0 < 0 ? 0u - (uint32_t)#0 : (uint32_t)#0)", 1); break;   // SAbs (std::abs)
+        case 6: each("(#0 > 0.0f ? 1.0f : (#0 < 0.0f ? -1.0f : #0))", 1); break;            // FSign
+        case 7: each("(int32_t)(0 < (int32_t)#0) - (int32_t)((int32_t)#0 < 0)", 1); break;   // SSign
+        case 8: each("floorf(#0)", 1); break;                                               // Floor
+        case 9: each("ceilf(#0)", 1); break;                                                // Ceil
+        case 10: each("spv_fract(#0)", 1); break;                                           // Fract
+        case 11: each("#0 * 0.01745329251994329577f", 1); break;                            // Radians
+        case 12: each("#0 * 57.2957795130823208768f", 1); break;                            // Degrees
+        case 13: each("sinf(#0)", 1); break;                                                // Sin
+        case 14: each("cosf(#0)", 1); break;                                                // Cos
+        case 15: each("tanf(#0)", 1); break;                                                // Tan
+        case 16: each("asinf(#0)", 1); break;                                               // Asin
+        case 17: each("acosf(#0)", 1); break;                                               // Acos
+        case 18: each("atanf(#0)", 1); break;                                               // Atan
+        case 19: each("sinhf(#0)", 1); break;                                               // Sinh
+        case 20: each("coshf(#0)", 1); break;                                               // Cosh
+        case 21: each("tanhf(#0)", 1); break;                                               // Tanh
+        case 22: each("asinhf(#0)", 1); break;                                              // Asinh
+        case 23: each("acoshf(#0)", 1); break;                                              // Acosh
+        case 24: each("atanhf(#0)", 1); break;                                              // Atanh
+        case 25: each("atan2f(#0, #1)", 2); break;                                          // Atan2
+        case 26: each("powf(#0, #1)", 2); break;                                            // Pow
+        case 27: each("expf(#0)", 1); break;                                                // Exp
+        case 28: each("logf(#0)", 1); break;                                                // Log
+        case 29: each("exp2f(#0)", 1); break;                                               // Exp2
+        case 30: each("log2f(#0)", 1); break;                                               // Log2
+        case 31: each("sqrtf(#0)", 1); break;                                               // Sqrt
+        case 32: each("1.0f / sqrtf(#0)", 1); break;                                        // InverseSqrt (glm inversesqrt)
+        case 37: each("spv_min(#0, #1)", 2); break;                                         // FMin
+        case 38: each("spv_min((uint32_t)#0, (uint32_t)#1)", 2); break;                     // UMin
+        case 39: each("spv_min((int32_t)#0, (int32_t)#1)", 2); break;                       // SMin
+        case 40: each("spv_max(#0, #1)", 2); break;                                         // FMax
+        case 41: each("spv_max((uint32_t)#0, (uint32_t)#1)", 2); break;                     // UMax
+        case 42: each("spv_max((int32_t)#0, (int32_t)#1)", 2); break;                       // SMax
+        case 43: each("spv_clamp(#0, #1, #2)", 3); break;                                   // FClamp
+        case 44: each("spv_clamp((uint32_t)#0, (uint32_t)#1, (uint32_t)#2)", 3); break;     // UClamp
+        case 45: each("spv_clamp((int32_t)#0, (int32_t)#1, (int32_t)#2)", 3); break;        // SClamp
+        case 46: each("#0 * (1.0f - #2) + #1 * #2", 3); break;                              // FMix (Mix :145-149)
+        case 48: each("(#1 < #0 ? 0.0f : 1.0f)", 2); break;                                 // Step
+        case 49: each("spv_smoothstep(#0, #1, #2)", 3); break;                              // SmoothStep
+        case 50: each("fmaf(#0, #1, #2)", 3); break;                                        // Fma
+        case 53: each("ldexpf(#0, (int32_t)#1)", 2); break;                                 // Ldexp
+        case 54: case 55: case 56: case 57: case 58: {                                      // Pack*
+            static const char* f[] = {"spv_pack_snorm4x8", "spv_pack_unorm4x8", "spv_pack_snorm2x16", "spv_pack_unorm2x16", "spv_pack_half2x16"};
+            code << "    " << r << " = " << f[inst - 54] << "(" << val(o[4]) << ");\n";
             break;
         }
-        case 69: {                                                                          // Normalize
-            if (!is_vec(rt)) bad("Normalize of a scalar");
-            code << "    { const float inv_ = 1.0f / sqrtf(dot_n(" << val(o[4]) << ".c, " << val(o[4]) << ".c, " << n << "));\n";
-            for (int k = 0; k < n; k++) code << "      " << r << ".c[" << k << "] = " << val(o[4]) << ".c[" << k << "] * inv_;\n";
+        case 60: case 61: case 62: case 63: case 64: {                                      // Unpack*
+            static const char* f[] = {"spv_unpack_snorm2x16", "spv_unpack_unorm2x16", "spv_unpack_half2x16", "spv_unpack_snorm4x8",
+                                      "spv_unpack_unorm4x8"};
+            code << "    " << r << " = " << f[inst - 60] << "(" << val(o[4]) << ");\n";
+            break;
+        }
+        case 66:                                                                            // Length
+            code << "    " << r << " = sqrtf(" << sumsq(o[4]) << ");\n";
+            break;
+        case 67: {                                                                          // Distance: length(p0 - p1)
+            const uint32_t t = type_of(o[4]);
+            const int m2 = comps(t);
+            code << "    { " << ctype(t) << " d_{};\n";
+            for (int k = 0; k < m2; k++) code << "      d_" << (is_vec(t) ? ".c[" + std::to_string(k) + "]" : "") << " = " << c(4, k) << " - " << c(5, k) << ";\n";
+            std::string s;
+            for (int k = 0; k < m2; k++) {
+                const std::string dk = "d_" + (is_vec(t) ? ".c[" + std::to_string(k) + "]" : std::string());
+                s += (k ? " + " : "") + dk + " * " + dk;
+            }
+            code << "      " << r << " = sqrtf(" << s << "); }\n";
+            break;
+        }
+        case 68:                                                                            // Cross (glm)
+            code << "    " << r << ".c[0] = " << c(4, 1) << " * " << c(5, 2) << " - " << c(5, 1) << " * " << c(4, 2) << ";\n"
+                 << "    " << r << ".c[1] = " << c(4, 2) << " * " << c(5, 0) << " - " << c(5, 2) << " * " << c(4, 0) << ";\n"
+                 << "    " << r << ".c[2] = " << c(4, 0) << " * " << c(5, 1) << " - " << c(5, 0) << " * " << c(4, 1) << ";\n";
+            break;
+        case 69:                                                                            // Normalize
+            if (!is_vec(rt)) {
+                // the reference's scalar Normalise (GlslFunctions.cpp:151-155)
+                each("(__builtin_signbit(#0) ? 1.0f : -1.0f)", 1);
+            } else {
+                code << "    { const float inv_ = 1.0f / sqrtf(" << sumsq(o[4]) << ");\n";
+                for (int k = 0; k < n; k++) code << "      " << r << ".c[" << k << "] = " << c(4, k) << " * inv_;\n";
+                code << "    }\n";
+            }
+            break;
+        case 70: {                                                                          // FaceForward (glm)
+            const std::string d = dotexpr(o[6], o[5]);
+            code << "    { const bool neg_ = " << d << " < 0.0f;\n";
+            for (int k = 0; k < n; k++) code << "      " << out(k) << " = neg_ ? " << c(4, k) << " : -" << c(4, k) << ";\n";
             code << "    }\n";
             break;
         }
+        case 71: {                                                                          // Reflect: I - N * dot(N, I) * 2 (glm)
+            const std::string d = dotexpr(o[5], o[4]);
+            code << "    { const float d_ = " << d << ";\n";
+            for (int k = 0; k < n; k++) code << "      " << out(k) << " = " << c(4, k) << " - " << c(5, k) << " * d_ * 2.0f;\n";
+            code << "    }\n";
+            break;
+        }
+        case 72: {                                                                          // Refract (glm)
+            const std::string d = dotexpr(o[5], o[4]);
+            const std::string eta = val(o[6]);
+            code << "    { const float d_ = " << d << "; const float k_ = 1.0f - " << eta << " * " << eta << " * (1.0f - d_ * d_);\n";
+            for (int k = 0; k < n; k++)
+                code << "      " << out(k) << " = k_ < 0.0f ? 0.0f : " << eta << " * " << c(4, k) << " - (" << eta << " * d_ + sqrtf(k_)) * "
+                     << c(5, k) << ";\n";
+            code << "    }\n";
+            break;
+        }
+        case 73: each("(" + et + ")spv_find_lsb((uint32_t)#0)", 1); break;                  // FindILsb
+        case 74: each("(" + et + ")spv_find_smsb((int32_t)#0)", 1); break;                  // FindSMsb
+        case 75: each("(" + et + ")spv_find_umsb((uint32_t)#0)", 1); break;                 // FindUMsb
+        case 79: each("spv_nmin(#0, #1)", 2); break;                                        // NMin
+        case 80: each("spv_nmax(#0, #1)", 2); break;                                        // NMax
+        case 81: each("spv_nmin(spv_nmax(#0, #1), #2)", 3); break;                          // NClamp
         default: {
             std::ostringstream e;
             e << "unsupported GLSL.std.450 instruction " << inst;
             bad(e.str());
         }
         }
+    }
+
+    int comps_or_one(uint32_t t) const
+    {
+        const Type& ty = T(t);
+        return ty.op == OpTypeVector ? (int)ty.b : 1;
+    }
+
+    // glm::dot of two values (a scalar "dot" is the product)
+    std::string dotexpr(uint32_t a, uint32_t b)
+    {
+        const uint32_t t = type_of(a);
+        if (!is_vec(t)) return val(a) + " * " + val(b);
+        return "dot_n(" + val(a) + ".c, " + val(b) + ".c, " + std::to_string(comps(t)) + ")";
     }
 };
 
@@ -837,11 +1417,103 @@ namespace cpv {
 template <class T, int N> struct V { T c[N]; };
 template <int R, int C> struct M { V<float, R> col[C]; };
 template <class T, int N> struct A { T e[N]; };
-// std::max / std::min / std::clamp (GlslFunctions.cpp:118-142)
-__device__ __forceinline__ float spv_fmax(float a, float b) { return a < b ? b : a; }
-__device__ __forceinline__ float spv_fmin(float a, float b) { return b < a ? b : a; }
-__device__ __forceinline__ float spv_fclamp(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+// std::min / std::max / std::clamp (GlslFunctions.cpp:118-142)
+template <class T> __device__ __forceinline__ T spv_min(T a, T b) { return b < a ? b : a; }
+template <class T> __device__ __forceinline__ T spv_max(T a, T b) { return a < b ? b : a; }
+template <class T> __device__ __forceinline__ T spv_clamp(T v, T lo, T hi) { return v < lo ? lo : (hi < v ? hi : v); }
+// NMin / NMax (GlslFunctions.cpp:266-300): a NaN first operand yields the second
+__device__ __forceinline__ float spv_nmin(float a, float b) { return a != a ? b : spv_min(a, b); }
+__device__ __forceinline__ float spv_nmax(float a, float b) { return a != a ? b : spv_max(a, b); }
 __device__ __forceinline__ float spv_fract(float v) { return v - floorf(v); }
+// OpFMod (SPIRVCompiler.cpp:2712-2722): frem, then the divisor's sign
+__device__ __forceinline__ float spv_fmod(float a, float b)
+{
+    const float r = fmodf(a, b);
+    const float s = copysignf(r, b);
+    return (r < s || r > s) ? r + b : s;
+}
+// OpBitFieldInsert as the reference builds it (SPIRVCompiler.cpp:3099-3136):
+// the mask selects bits [offset, offset + count) of Insert itself
+__device__ __forceinline__ uint32_t spv_bfi(uint32_t base, uint32_t ins, uint32_t offset, uint32_t count)
+{
+    uint32_t mask = (0xFFFFFFFFu << (count & 31u)) ^ 0xFFFFFFFFu;
+    mask <<= (offset & 31u);
+    return (~mask & base) | (mask & ins);
+}
+__device__ __forceinline__ uint32_t spv_bfe_u(uint32_t base, uint32_t offset, uint32_t count)
+{
+    if (count == 0u) return 0u;
+    const uint32_t v = base >> (offset & 31u);
+    return count >= 32u ? v : v & ((1u << count) - 1u);
+}
+__device__ __forceinline__ int32_t spv_bfe_s(uint32_t base, uint32_t offset, uint32_t count)
+{
+    if (count == 0u) return 0;
+    const uint32_t sh = 32u - (count < 32u ? count : 32u);
+    return (int32_t)(spv_bfe_u(base, offset, count) << sh) >> sh;
+}
+// FindILsb / FindSMsb / FindUMsb (GlslFunctions.cpp:185-250)
+__device__ __forceinline__ int32_t spv_find_lsb(uint32_t v) { return v == 0u ? -1 : __builtin_ctz(v); }
+__device__ __forceinline__ int32_t spv_find_umsb(uint32_t v) { return v == 0u ? -1 : 31 - __builtin_clz(v); }
+__device__ __forceinline__ int32_t spv_find_smsb(int32_t v) { return spv_find_umsb((uint32_t)(v < 0 ? ~v : v)); }
+__device__ __forceinline__ float spv_smoothstep(float e0, float e1, float x)
+{
+    const float t = spv_clamp((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+// GLSL.std.450 packing: round to nearest, halves away from zero
+__device__ __forceinline__ uint32_t spv_pack_unorm4x8(V<float, 4> v)
+{
+    uint32_t r = 0u;
+    for (int i = 0; i < 4; i++) r |= (uint32_t)roundf(spv_clamp(v.c[i], 0.0f, 1.0f) * 255.0f) << (8 * i);
+    return r;
+}
+__device__ __forceinline__ uint32_t spv_pack_snorm4x8(V<float, 4> v)
+{
+    uint32_t r = 0u;
+    for (int i = 0; i < 4; i++) r |= ((uint32_t)(int32_t)roundf(spv_clamp(v.c[i], -1.0f, 1.0f) * 127.0f) & 0xFFu) << (8 * i);
+    return r;
+}
+__device__ __forceinline__ uint32_t spv_pack_unorm2x16(V<float, 2> v)
+{
+    return (uint32_t)roundf(spv_clamp(v.c[0], 0.0f, 1.0f) * 65535.0f) | ((uint32_t)roundf(spv_clamp(v.c[1], 0.0f, 1.0f) * 65535.0f) << 16);
+}
+__device__ __forceinline__ uint32_t spv_pack_snorm2x16(V<float, 2> v)
+{
+    const uint32_t a = (uint32_t)(int32_t)roundf(spv_clamp(v.c[0], -1.0f, 1.0f) * 32767.0f) & 0xFFFFu;
+    const uint32_t b = (uint32_t)(int32_t)roundf(spv_clamp(v.c[1], -1.0f, 1.0f) * 32767.0f) & 0xFFFFu;
+    return a | (b << 16);
+}
+__device__ __forceinline__ uint32_t spv_pack_half2x16(V<float, 2> v)
+{
+    const _Float16 a = (_Float16)v.c[0], b = (_Float16)v.c[1];
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+__device__ __forceinline__ V<float, 4> spv_unpack_unorm4x8(uint32_t p)
+{
+    V<float, 4> r;
+    for (int i = 0; i < 4; i++) r.c[i] = (float)((p >> (8 * i)) & 0xFFu) / 255.0f;
+    return r;
+}
+__device__ __forceinline__ V<float, 4> spv_unpack_snorm4x8(uint32_t p)
+{
+    V<float, 4> r;
+    for (int i = 0; i < 4; i++) r.c[i] = spv_clamp((float)(int8_t)((p >> (8 * i)) & 0xFFu) / 127.0f, -1.0f, 1.0f);
+    return r;
+}
+__device__ __forceinline__ V<float, 2> spv_unpack_unorm2x16(uint32_t p)
+{
+    return V<float, 2>{{(float)(p & 0xFFFFu) / 65535.0f, (float)(p >> 16) / 65535.0f}};
+}
+__device__ __forceinline__ V<float, 2> spv_unpack_snorm2x16(uint32_t p)
+{
+    return V<float, 2>{{spv_clamp((float)(int16_t)(p & 0xFFFFu) / 32767.0f, -1.0f, 1.0f),
+                        spv_clamp((float)(int16_t)(p >> 16) / 32767.0f, -1.0f, 1.0f)}};
+}
+__device__ __forceinline__ V<float, 2> spv_unpack_half2x16(uint32_t p)
+{
+    return V<float, 2>{{(float)__builtin_bit_cast(_Float16, (uint16_t)(p & 0xFFFFu)), (float)__builtin_bit_cast(_Float16, (uint16_t)(p >> 16))}};
+}
 )";
 
 struct Translation {
@@ -857,26 +1529,26 @@ Translation translate(const uint32_t* vs, size_t vs_n, const char* vs_entry, con
     if (mf.model != 4) bad("the second module's entry point is not a fragment shader");
     Gen gv(mv, true), gf(mf, false);
     gv.globals();
-    gv.body();
+    gv.functions();
     gf.globals();
     gf.bind_fragment_inputs(gv.loc_offset, gv.loc_size);
-    gf.body();
+    gf.functions();
     if (!mf.origin_upper_left) bad("fragment shaders must declare OriginUpperLeft (Vulkan requires it)");
     std::ostringstream s;
     s << kPrelude;
-    s << "struct SpvVs {\n    static constexpr int kOutputs = " << gv.out_total << ";\n"
+    s << "struct SpvVs {\n    static constexpr int kOutputs = " << gv.out_total << ";\n" << gv.types.str()
       << "    __device__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)\n    {\n"
-      << gv.decl.str() << gv.pro.str() << gv.code.str() << "L_end:;\n" << gv.epi.str()
+      << gv.decl.str() << gv.fns.str() << gv.pro.str() << gv.code.str() << "L_end:;\n" << gv.epi.str()
       << "    (void)env; (void)vid; (void)iid; (void)out; (void)pos;\n    }\n};\n";
     const int fin = (int)gf.interp.size();
-    s << "struct SpvFs {\n    static constexpr int kInputs = " << fin << ";\n"
+    s << "struct SpvFs {\n    static constexpr int kInputs = " << fin << ";\n" << gf.types.str()
       << "    static constexpr bool kMayDiscard = " << (gf.discard ? "true" : "false") << ";\n"
       << "    __device__ static int interp(int c)\n    {\n";
     for (int c = 0; c < fin; c++)
         if (gf.interp[c]) s << "        if (c == " << c << ") return " << gf.interp[c] << ";\n";
     s << "        (void)c;\n        return 0;\n    }\n"
       << "    __device__ static bool run(const DrawEnv& env, const float* in, const float* fc, bool front, float out[4])\n    {\n"
-      << gf.decl.str() << gf.pro.str() << gf.code.str() << "L_end:;\n" << gf.epi.str()
+      << gf.decl.str() << gf.fns.str() << gf.pro.str() << gf.code.str() << "L_end:;\n" << gf.epi.str()
       << "    (void)env; (void)in; (void)fc; (void)front;\n        return false;\n    }\n};\n";
     s << "}  // namespace cpv\n";
     s << "template __global__ void cpv::vertex_kernel<cpv::SpvVs>(cpv::VertexArgs);\n";
