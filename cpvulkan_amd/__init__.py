@@ -30,10 +30,18 @@ def _declare(lib: C.CDLL) -> None:
     lib.cpv_context_destroy.restype = None
     lib.cpv_alloc.argtypes = [P, C.c_uint64, C.POINTER(C.c_uint64)]
     lib.cpv_free.argtypes = [P, C.c_uint64]
+    lib.cpv_alloc_host.argtypes = [P, C.c_uint64, C.POINTER(C.c_uint64)]
+    lib.cpv_free_host.argtypes = [P, C.c_uint64]
     lib.cpv_upload.argtypes = [P, C.c_uint64, P, C.c_uint64]
     lib.cpv_download.argtypes = [P, P, C.c_uint64, C.c_uint64]
     lib.cpv_synchronize.argtypes = [P]
     lib.cpv_device_copy.argtypes = [P, C.c_uint64, C.c_uint64, C.c_uint64]
+    lib.cpv_copy_regions.argtypes = [P, C.POINTER(abi.CopyRegion), C.c_uint32]
+    lib.cpv_fill_buffer.argtypes = [P, C.c_uint64, C.c_uint64, C.c_uint32]
+    lib.cpv_update_buffer.argtypes = [P, C.c_uint64, P, C.c_uint64]
+    lib.cpv_clear_rects.argtypes = [P, C.POINTER(abi.ImageLevel), C.POINTER(C.c_uint32), C.POINTER(abi.Rect), C.c_uint32]
+    lib.cpv_blit_image.argtypes = [P, C.POINTER(abi.ImageLevel), C.POINTER(abi.ImageLevel), C.POINTER(abi.BlitRegion), C.c_uint32,
+                                   C.c_uint32]
     lib.cpv_set_config.argtypes = [P, C.POINTER(abi.Config)]
     lib.cpv_get_config.argtypes = [P, C.POINTER(abi.Config)]
     lib.cpv_render_pass.argtypes = [P, C.POINTER(abi.PassDesc), C.POINTER(abi.PipelineDesc), C.POINTER(abi.DrawDesc),
@@ -147,6 +155,29 @@ class Context:
 
     def device_copy(self, dst: int, src: int, nbytes: int) -> None:
         _check(self.lib, self.lib.cpv_device_copy(self.handle, dst, src, nbytes), "cpv_device_copy")
+
+    # transfers (enqueued on the context stream, in order with render passes)
+    def copy_regions(self, regions) -> None:
+        arr = (abi.CopyRegion * max(len(regions), 1))(*regions)
+        _check(self.lib, self.lib.cpv_copy_regions(self.handle, arr, len(regions)), "cpv_copy_regions")
+
+    def fill_buffer(self, dst: int, size: int, data: int) -> None:
+        _check(self.lib, self.lib.cpv_fill_buffer(self.handle, dst, size, data & 0xFFFFFFFF), "cpv_fill_buffer")
+
+    def update_buffer(self, dst: int, data) -> None:
+        import numpy as np
+        a = np.ascontiguousarray(data)
+        _check(self.lib, self.lib.cpv_update_buffer(self.handle, dst, a.ctypes.data, a.nbytes), "cpv_update_buffer")
+
+    def clear_rects(self, level: abi.ImageLevel, value_words, rects) -> None:
+        v = (C.c_uint32 * 4)(*[int(w) & 0xFFFFFFFF for w in value_words])
+        arr = (abi.Rect * max(len(rects), 1))(*rects)
+        _check(self.lib, self.lib.cpv_clear_rects(self.handle, C.byref(level), v, arr, len(rects)), "cpv_clear_rects")
+
+    def blit_image(self, src: abi.ImageLevel, dst: abi.ImageLevel, regions, filter_: int) -> None:
+        arr = (abi.BlitRegion * max(len(regions), 1))(*regions)
+        _check(self.lib, self.lib.cpv_blit_image(self.handle, C.byref(src), C.byref(dst), arr, len(regions), filter_),
+               "cpv_blit_image")
 
     # configuration
     def set_config(self, tile_size=16, waves_per_tile=4, scheduler=0, split_threshold=0, record_costs=0,

@@ -19,6 +19,8 @@ FORMAT = dict(
     B8G8R8A8_UNORM=44, R16G16B16A16_UNORM=91, R32_SFLOAT=100, R32G32_SFLOAT=103, R32G32B32_SFLOAT=106,
     R32G32B32A32_SFLOAT=109, D16_UNORM=124, X8_D24_UNORM_PACK32=125, D32_SFLOAT=126, S8_UINT=127,
     D16_UNORM_S8_UINT=128, D24_UNORM_S8_UINT=129, D32_SFLOAT_S8_UINT=130,
+    R16_UNORM=70, R16_SINT=75, R16G16_UINT=81, R16G16_SINT=82, R32_UINT=98, R32_SINT=99, R32G32_UINT=101,
+    R32G32B32A32_UINT=107, R32G32B32A32_SINT=108,
 )
 COMPARE = dict(NEVER=0, LESS=1, EQUAL=2, LESS_OR_EQUAL=3, GREATER=4, NOT_EQUAL=5, GREATER_OR_EQUAL=6, ALWAYS=7)
 STENCIL = dict(KEEP=0, ZERO=1, REPLACE=2, INC_CLAMP=3, DEC_CLAMP=4, INVERT=5, INC_WRAP=6, DEC_WRAP=7)
@@ -38,7 +40,8 @@ SCHED = dict(GRID=0, COUNTER=1, PREDICTED=2)
 COLOR_RGBA = 15
 
 FORMAT_BYTES = {9: 1, 37: 4, 38: 4, 41: 4, 42: 4, 44: 4, 91: 8, 100: 4, 103: 8, 106: 12, 109: 16,
-                124: 2, 125: 4, 126: 4, 127: 1, 128: 3, 129: 4, 130: 8}
+                124: 2, 125: 4, 126: 4, 127: 1, 128: 3, 129: 4, 130: 8,
+                70: 2, 75: 2, 81: 4, 82: 4, 98: 4, 99: 4, 101: 8, 107: 16, 108: 16}
 
 u32 = C.c_uint32
 i32 = C.c_int32
@@ -143,12 +146,29 @@ class AutotuneEntry(C.Structure):
     _fields_ = [("tile_size", u32), ("waves_per_tile", u32), ("ms", f32), ("reserved", u32)]
 
 
+class CopyRegion(C.Structure):
+    _fields_ = [("src", u64), ("dst", u64), ("src_pitch", u64), ("dst_pitch", u64), ("row_bytes", u64), ("rows", u32),
+                ("pad", u32)]
+
+
+class ImageLevel(C.Structure):
+    _fields_ = [("address", u64), ("format", u32), ("width", u32), ("height", u32), ("pad", u32)]
+
+
+class Rect(C.Structure):
+    _fields_ = [("x", i32), ("y", i32), ("width", u32), ("height", u32)]
+
+
+class BlitRegion(C.Structure):
+    _fields_ = [("src", i32 * 4), ("dst", i32 * 4), ("aspect", u32), ("pad", u32)]
+
+
 MLP_FEATURES = 12
 MLP_HIDDEN = 64
 MLP_PARAMS = 12 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 
 STRUCT_ORDER = [PipelineDesc, DrawDesc, PassDesc, AttachmentDesc, TextureDesc, SamplerDesc, JobDesc, Config, TileTrace,
-                AutotuneEntry]
+                AutotuneEntry, CopyRegion, ImageLevel, Rect, BlitRegion]
 
 
 def check_layout(lib: C.CDLL) -> None:

@@ -571,7 +571,8 @@ extern "C" int cpv_abi_sizes(uint32_t* out, uint32_t n)
     const uint32_t sizes[] = {(uint32_t)sizeof(cpv_pipeline_desc), (uint32_t)sizeof(cpv_draw_desc), (uint32_t)sizeof(cpv_pass_desc),
                               (uint32_t)sizeof(cpv_attachment_desc), (uint32_t)sizeof(cpv_texture_desc), (uint32_t)sizeof(cpv_sampler_desc),
                               (uint32_t)sizeof(cpv_job_desc), (uint32_t)sizeof(cpv_config), (uint32_t)sizeof(cpv_tile_trace),
-                              (uint32_t)sizeof(cpv_autotune_entry)};
+                              (uint32_t)sizeof(cpv_autotune_entry), (uint32_t)sizeof(cpv_copy_region), (uint32_t)sizeof(cpv_image_level),
+                              (uint32_t)sizeof(cpv_rect), (uint32_t)sizeof(cpv_blit_region)};
     const uint32_t m = (uint32_t)(sizeof(sizes) / sizeof(sizes[0]));
     for (uint32_t i = 0; i < n && i < m; i++) out[i] = sizes[i];
     return (int)m;

@@ -168,7 +168,11 @@ struct Pipeline {
 
 struct CommandPool {};
 
-enum class Op { BeginPass, EndPass, BindPipeline, BindVertex, BindIndex, BindSets, SetViewport, Draw, DrawIndexed };
+enum class Op {
+    BeginPass, EndPass, BindPipeline, BindVertex, BindIndex, BindSets, SetViewport, Draw, DrawIndexed, DrawIndirect,
+    DrawIndexedIndirect, CopyBuffer, CopyImage, CopyBufferToImage, CopyImageToBuffer, FillBuffer, UpdateBuffer, ClearColorImage,
+    ClearDepthStencilImage, ClearAttachments, BlitImage
+};
 
 struct Cmd {
     Op op;
@@ -184,6 +188,23 @@ struct Cmd {
     VkViewport viewport{};
     uint32_t count = 0, instances = 0, first_vertex = 0, first_instance = 0;
     int32_t vertex_offset = 0;
+    // transfers, clears and indirect draws
+    Buffer* src_buffer = nullptr;
+    Buffer* dst_buffer = nullptr;
+    Image* src_image = nullptr;
+    Image* dst_image = nullptr;
+    std::vector<VkBufferCopy> buffer_copies;
+    std::vector<VkImageCopy> image_copies;
+    std::vector<VkBufferImageCopy> buffer_image_copies;
+    std::vector<VkImageBlit> blits;
+    std::vector<VkClearAttachment> attachments;
+    std::vector<VkClearRect> rects;
+    std::vector<VkImageSubresourceRange> ranges;
+    VkClearValue value{};
+    VkDeviceSize offset = 0, size = 0;
+    uint32_t data = 0, stride = 0;
+    int32_t filter = 0;
+    std::vector<uint8_t> bytes;
 };
 
 struct CommandBuffer : Dispatchable {
@@ -884,6 +905,121 @@ void CmdDrawIndexed(VkCommandBuffer cb, uint32_t indexCount, uint32_t instanceCo
     CB(cb)->cmds.push_back(c);
 }
 
+void CmdDrawIndirect(VkCommandBuffer cb, VkBuffer buffer, VkDeviceSize offset, uint32_t drawCount, uint32_t stride)
+{
+    Cmd c{Op::DrawIndirect};
+    c.src_buffer = reinterpret_cast<Buffer*>(buffer);
+    c.offset = offset;
+    c.count = drawCount;
+    c.stride = stride;
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdDrawIndexedIndirect(VkCommandBuffer cb, VkBuffer buffer, VkDeviceSize offset, uint32_t drawCount, uint32_t stride)
+{
+    Cmd c{Op::DrawIndexedIndirect};
+    c.src_buffer = reinterpret_cast<Buffer*>(buffer);
+    c.offset = offset;
+    c.count = drawCount;
+    c.stride = stride;
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdCopyBuffer(VkCommandBuffer cb, VkBuffer src, VkBuffer dst, uint32_t n, const VkBufferCopy* regions)
+{
+    Cmd c{Op::CopyBuffer};
+    c.src_buffer = reinterpret_cast<Buffer*>(src);
+    c.dst_buffer = reinterpret_cast<Buffer*>(dst);
+    c.buffer_copies.assign(regions, regions + n);
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdCopyImage(VkCommandBuffer cb, VkImage src, int32_t, VkImage dst, int32_t, uint32_t n, const VkImageCopy* regions)
+{
+    Cmd c{Op::CopyImage};
+    c.src_image = reinterpret_cast<Image*>(src);
+    c.dst_image = reinterpret_cast<Image*>(dst);
+    c.image_copies.assign(regions, regions + n);
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdCopyBufferToImage(VkCommandBuffer cb, VkBuffer src, VkImage dst, int32_t, uint32_t n, const VkBufferImageCopy* regions)
+{
+    Cmd c{Op::CopyBufferToImage};
+    c.src_buffer = reinterpret_cast<Buffer*>(src);
+    c.dst_image = reinterpret_cast<Image*>(dst);
+    c.buffer_image_copies.assign(regions, regions + n);
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdCopyImageToBuffer(VkCommandBuffer cb, VkImage src, int32_t, VkBuffer dst, uint32_t n, const VkBufferImageCopy* regions)
+{
+    Cmd c{Op::CopyImageToBuffer};
+    c.src_image = reinterpret_cast<Image*>(src);
+    c.dst_buffer = reinterpret_cast<Buffer*>(dst);
+    c.buffer_image_copies.assign(regions, regions + n);
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdFillBuffer(VkCommandBuffer cb, VkBuffer dst, VkDeviceSize offset, VkDeviceSize size, uint32_t data)
+{
+    Cmd c{Op::FillBuffer};
+    c.dst_buffer = reinterpret_cast<Buffer*>(dst);
+    c.offset = offset;
+    // VK_WHOLE_SIZE: to the end of the buffer (FillBufferCommand, CommandBuffer.cpp:280-289)
+    c.size = size == VK_WHOLE_SIZE ? c.dst_buffer->size - offset : size;
+    c.data = data;
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdUpdateBuffer(VkCommandBuffer cb, VkBuffer dst, VkDeviceSize offset, VkDeviceSize size, const void* data)
+{
+    Cmd c{Op::UpdateBuffer};
+    c.dst_buffer = reinterpret_cast<Buffer*>(dst);
+    c.offset = offset;
+    // the data is copied when the command is recorded (UpdateBufferCommand, CommandBuffer.cpp:241-276)
+    c.bytes.assign((const uint8_t*)data, (const uint8_t*)data + size);
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdClearColorImage(VkCommandBuffer cb, VkImage image, int32_t, const VkClearColorValue* color, uint32_t n,
+                        const VkImageSubresourceRange* ranges)
+{
+    Cmd c{Op::ClearColorImage};
+    c.dst_image = reinterpret_cast<Image*>(image);
+    c.value.color = *color;
+    c.ranges.assign(ranges, ranges + n);
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdClearDepthStencilImage(VkCommandBuffer cb, VkImage image, int32_t, const VkClearDepthStencilValue* ds, uint32_t n,
+                               const VkImageSubresourceRange* ranges)
+{
+    Cmd c{Op::ClearDepthStencilImage};
+    c.dst_image = reinterpret_cast<Image*>(image);
+    c.value.depthStencil = *ds;
+    c.ranges.assign(ranges, ranges + n);
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdClearAttachments(VkCommandBuffer cb, uint32_t n, const VkClearAttachment* atts, uint32_t nr, const VkClearRect* rects)
+{
+    Cmd c{Op::ClearAttachments};
+    c.attachments.assign(atts, atts + n);
+    c.rects.assign(rects, rects + nr);
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdBlitImage(VkCommandBuffer cb, VkImage src, int32_t, VkImage dst, int32_t, uint32_t n, const VkImageBlit* regions, int32_t filter)
+{
+    Cmd c{Op::BlitImage};
+    c.src_image = reinterpret_cast<Image*>(src);
+    c.dst_image = reinterpret_cast<Image*>(dst);
+    c.blits.assign(regions, regions + n);
+    c.filter = filter;
+    CB(cb)->cmds.push_back(c);
+}
+
 // ---------------------------------------------------------------- replay
 
 struct ReplayState {
@@ -898,6 +1034,7 @@ struct ReplayState {
     int32_t ib_type = 0;
     DescriptorSet* set0 = nullptr;
     VkViewport viewport{};
+    bool pass_split = false;   // a vkCmdClearAttachments ended an earlier part of the current pass
     std::vector<cpv_pipeline_desc> pipelines;
     std::vector<Pipeline*> pipeline_objs;
     std::vector<cpv_draw_desc> draws;
@@ -951,6 +1088,7 @@ cpv_draw_desc make_draw(const ReplayState& s, const Cmd& c)
 
 VkResult end_pass(Device* dev, ReplayState& s)
 {
+    const bool first_segment = !s.pass_split;
     cpv_pass_desc p;
     std::memset(&p, 0, sizeof(p));
     p.width = s.fb->width;
@@ -960,7 +1098,8 @@ VkResult end_pass(Device* dev, ReplayState& s)
         const VkAttachmentDescription& ad = s.pass->attachments[(size_t)idx];
         a->address = v->image->address() + v->image->level_offset[v->base_level];
         a->format = (uint32_t)v->format;
-        a->load_op = (uint32_t)ad.loadOp;
+        // a pass split by vkCmdClearAttachments continues with what its first part stored
+        a->load_op = first_segment ? (uint32_t)ad.loadOp : (uint32_t)VK_ATTACHMENT_LOAD_OP_LOAD;
         if ((size_t)idx < s.clears.size()) std::memcpy(&a->clear, &s.clears[(size_t)idx], sizeof(VkClearValue));
     };
     if (s.pass->color >= 0) {
@@ -979,6 +1118,245 @@ VkResult end_pass(Device* dev, ReplayState& s)
     return device_error(rc);
 }
 
+// ------------------------------------------------------ transfer replay
+
+uint32_t level_width(const Image* im, uint32_t l) { return std::max(im->width >> l, 1u); }
+uint32_t level_height(const Image* im, uint32_t l) { return std::max(im->height >> l, 1u); }
+
+cpv_image_level image_level(const Image* im, uint32_t l)
+{
+    cpv_image_level v;
+    std::memset(&v, 0, sizeof(v));
+    v.address = im->address() + im->level_offset.at(l);
+    v.format = (uint32_t)im->format;
+    v.width = level_width(im, l);
+    v.height = level_height(im, l);
+    return v;
+}
+
+bool single_layer(const VkImageSubresourceLayers& r) { return r.baseArrayLayer == 0 && r.layerCount == 1; }
+
+// Formats copied as plain pixel rows: colour formats and depth-only formats
+// (ProcessNormal / the depth branch of ProcessDepthStencil,
+// CommandBuffer.Copy.cpp:501-544, 546-620); combined depth/stencil formats
+// take per-aspect repacking there and are not supported here.
+bool row_copyable(int32_t f) { return !(f == CPV_FORMAT_D16_UNORM_S8_UINT || f == CPV_FORMAT_D24_UNORM_S8_UINT || f == CPV_FORMAT_D32_SFLOAT_S8_UINT || f == CPV_FORMAT_S8_UINT); }
+
+// The rows of a VkBufferImageCopy region (CopyBufferToImageCommand::ProcessNormal,
+// CommandBuffer.Copy.cpp:501-544, and CopyImageToBufferCommand::ProcessNormal :811-855):
+// buffer row y of the region starts at bufferOffset + y * rowLength * pixel,
+// image row y at level.Offset + (y + offset.y) * level.Stride + offset.x * pixel.
+VkResult buffer_image_rows(const Buffer* buf, const Image* im, const VkBufferImageCopy& r, bool to_image, cpv_copy_region* out)
+{
+    std::memset(out, 0, sizeof(*out));   // zero rows: nothing to copy
+    if (!single_layer(r.imageSubresource) || r.imageOffset.z != 0 || r.imageExtent.depth != 1 || !row_copyable(im->format) ||
+        r.imageSubresource.mipLevel >= im->levels)
+        return VK_ERROR_FEATURE_NOT_PRESENT;
+    const uint64_t px = format_bytes(im->format);
+    const uint32_t row_length = r.bufferRowLength ? r.bufferRowLength : r.imageExtent.width;
+    const uint32_t l = r.imageSubresource.mipLevel;
+    if (!r.imageExtent.width || !r.imageExtent.height) return VK_SUCCESS;
+    if (r.imageOffset.x < 0 || r.imageOffset.y < 0 || (uint64_t)r.imageOffset.x + r.imageExtent.width > level_width(im, l) ||
+        (uint64_t)r.imageOffset.y + r.imageExtent.height > level_height(im, l) || row_length < r.imageExtent.width ||
+        r.bufferOffset + ((uint64_t)(r.imageExtent.height - 1) * row_length + r.imageExtent.width) * px > buf->size)
+        return VK_ERROR_DEVICE_LOST;   // region outside the buffer or the image level
+    const uint64_t stride = px * level_width(im, l);
+    const uint64_t buf_at = buf->address() + r.bufferOffset;
+    const uint64_t img_at = im->address() + im->level_offset[l] + (uint64_t)r.imageOffset.y * stride + (uint64_t)r.imageOffset.x * px;
+    out->row_bytes = px * r.imageExtent.width;
+    out->rows = r.imageExtent.height;
+    out->src = to_image ? buf_at : img_at;
+    out->dst = to_image ? img_at : buf_at;
+    out->src_pitch = to_image ? px * row_length : stride;
+    out->dst_pitch = to_image ? stride : px * row_length;
+    return VK_SUCCESS;
+}
+
+// CopyImageCommand::Process for colour formats (CommandBuffer.Copy.cpp:77-196):
+// rows of pixelSize * extent.width bytes at each level's Stride.
+VkResult image_rows(const Image* src, const Image* dst, const VkImageCopy& r, cpv_copy_region* out)
+{
+    if (!single_layer(r.srcSubresource) || !single_layer(r.dstSubresource) || r.extent.depth != 1 || r.srcOffset.z || r.dstOffset.z ||
+        !row_copyable(src->format) || format_bytes(src->format) != format_bytes(dst->format) ||
+        r.srcSubresource.mipLevel >= src->levels || r.dstSubresource.mipLevel >= dst->levels)
+        return VK_ERROR_FEATURE_NOT_PRESENT;
+    const uint64_t px = format_bytes(src->format);
+    const uint32_t sl = r.srcSubresource.mipLevel, dl = r.dstSubresource.mipLevel;
+    if (r.srcOffset.x < 0 || r.srcOffset.y < 0 || r.dstOffset.x < 0 || r.dstOffset.y < 0 ||
+        (uint64_t)r.srcOffset.x + r.extent.width > level_width(src, sl) || (uint64_t)r.srcOffset.y + r.extent.height > level_height(src, sl) ||
+        (uint64_t)r.dstOffset.x + r.extent.width > level_width(dst, dl) || (uint64_t)r.dstOffset.y + r.extent.height > level_height(dst, dl))
+        return VK_ERROR_DEVICE_LOST;   // region outside an image level
+    std::memset(out, 0, sizeof(*out));
+    out->src_pitch = px * level_width(src, sl);
+    out->dst_pitch = px * level_width(dst, dl);
+    out->src = src->address() + src->level_offset[sl] + (uint64_t)r.srcOffset.y * out->src_pitch + (uint64_t)r.srcOffset.x * px;
+    out->dst = dst->address() + dst->level_offset[dl] + (uint64_t)r.dstOffset.y * out->dst_pitch + (uint64_t)r.dstOffset.x * px;
+    out->row_bytes = px * r.extent.width;
+    out->rows = r.extent.height;
+    return VK_SUCCESS;
+}
+
+VkResult run_transfer(Device* dev, const Cmd& c)
+{
+    cpv_context* ctx = dev->ctx;
+    std::vector<cpv_copy_region> rows;
+    switch (c.op) {
+    case Op::CopyBuffer:
+        // CopyBufferCommand::Process (CommandBuffer.Copy.cpp:38-46): one memcpy per region
+        for (const VkBufferCopy& r : c.buffer_copies) {
+            if (r.srcOffset + r.size > c.src_buffer->size || r.dstOffset + r.size > c.dst_buffer->size) return VK_ERROR_DEVICE_LOST;
+            cpv_copy_region g;
+            std::memset(&g, 0, sizeof(g));
+            g.src = c.src_buffer->address() + r.srcOffset;
+            g.dst = c.dst_buffer->address() + r.dstOffset;
+            g.src_pitch = g.dst_pitch = g.row_bytes = r.size;
+            g.rows = 1;
+            rows.push_back(g);
+        }
+        break;
+    case Op::CopyImage:
+        for (const VkImageCopy& r : c.image_copies) {
+            cpv_copy_region g;
+            if (VkResult e = image_rows(c.src_image, c.dst_image, r, &g)) return e;
+            rows.push_back(g);
+        }
+        break;
+    case Op::CopyBufferToImage: case Op::CopyImageToBuffer: {
+        const bool to_image = c.op == Op::CopyBufferToImage;
+        for (const VkBufferImageCopy& r : c.buffer_image_copies) {
+            cpv_copy_region g;
+            if (VkResult e = buffer_image_rows(to_image ? c.src_buffer : c.dst_buffer, to_image ? c.dst_image : c.src_image, r, to_image, &g))
+                return e;
+            rows.push_back(g);
+        }
+        break;
+    }
+    case Op::FillBuffer:
+        if (c.offset + c.size > c.dst_buffer->size) return VK_ERROR_DEVICE_LOST;
+        return device_error(cpv_fill_buffer(ctx, c.dst_buffer->address() + c.offset, c.size, c.data));
+    case Op::UpdateBuffer:
+        if (c.offset + c.bytes.size() > c.dst_buffer->size) return VK_ERROR_DEVICE_LOST;
+        return device_error(cpv_update_buffer(ctx, c.dst_buffer->address() + c.offset, c.bytes.data(), c.bytes.size()));
+    case Op::ClearColorImage: case Op::ClearDepthStencilImage: {
+        // ClearImage over every level of each range (CommandBuffer.Draw.cpp:2398-2419, 117-149)
+        uint32_t words[4];
+        std::memcpy(words, &c.value, 16);
+        for (const VkImageSubresourceRange& r : c.ranges) {
+            const uint32_t levels = r.levelCount == VK_REMAINING_MIP_LEVELS ? c.dst_image->levels - r.baseMipLevel : r.levelCount;
+            for (uint32_t l = r.baseMipLevel; l < r.baseMipLevel + levels && l < c.dst_image->levels; l++) {
+                const cpv_image_level lv = image_level(c.dst_image, l);
+                const cpv_rect rect{0, 0, lv.width, lv.height};
+                if (VkResult e = device_error(cpv_clear_rects(ctx, &lv, words, &rect, 1))) return e;
+            }
+        }
+        return VK_SUCCESS;
+    }
+    case Op::BlitImage: {
+        for (const VkImageBlit& r : c.blits) {
+            if (!single_layer(r.srcSubresource) || !single_layer(r.dstSubresource) || r.srcSubresource.mipLevel >= c.src_image->levels ||
+                r.dstSubresource.mipLevel >= c.dst_image->levels)
+                return VK_ERROR_FEATURE_NOT_PRESENT;
+            const cpv_image_level sl = image_level(c.src_image, r.srcSubresource.mipLevel);
+            const cpv_image_level dl = image_level(c.dst_image, r.dstSubresource.mipLevel);
+            cpv_blit_region g;
+            std::memset(&g, 0, sizeof(g));
+            g.src[0] = r.srcOffsets[0].x; g.src[1] = r.srcOffsets[0].y; g.src[2] = r.srcOffsets[1].x; g.src[3] = r.srcOffsets[1].y;
+            g.dst[0] = r.dstOffsets[0].x; g.dst[1] = r.dstOffsets[0].y; g.dst[2] = r.dstOffsets[1].x; g.dst[3] = r.dstOffsets[1].y;
+            g.aspect = r.srcSubresource.aspectMask;
+            if (VkResult e = device_error(cpv_blit_image(ctx, &sl, &dl, &g, 1, (uint32_t)c.filter))) return e;
+        }
+        return VK_SUCCESS;
+    }
+    default:
+        return VK_ERROR_DEVICE_LOST;
+    }
+    return device_error(cpv_copy_regions(ctx, rows.data(), (uint32_t)rows.size()));
+}
+
+// ClearAttachmentsCommand::Process (CommandBuffer.Draw.cpp:2160-2224) inside a
+// render pass: the draws recorded so far are issued first, then each
+// rectangle of the attachment's first view level is set to the clear value
+// (colour, or depth and stencil together, as the reference writes both).
+VkResult clear_attachments(Device* dev, ReplayState& s, const Cmd& c)
+{
+    if (!s.pass || !s.fb) return VK_ERROR_DEVICE_LOST;
+    if (VkResult e = end_pass(dev, s)) return e;
+    s.pass_split = true;
+    for (const VkClearAttachment& a : c.attachments) {
+        const int32_t idx = (a.aspectMask & VK_IMAGE_ASPECT_COLOR_BIT) ? s.pass->color : s.pass->depth;
+        if (idx < 0) continue;
+        const ImageView* v = s.fb->views.at((size_t)idx);
+        const cpv_image_level lv = image_level(v->image, v->base_level);
+        std::vector<cpv_rect> rects;
+        for (const VkClearRect& r : c.rects) {
+            if (r.baseArrayLayer != 0 || r.layerCount != 1) return VK_ERROR_FEATURE_NOT_PRESENT;
+            rects.push_back(cpv_rect{r.rect.offset.x, r.rect.offset.y, r.rect.extent.width, r.rect.extent.height});
+        }
+        uint32_t words[4];
+        std::memcpy(words, &a.clearValue, 16);
+        if (VkResult e = device_error(cpv_clear_rects(dev->ctx, &lv, words, rects.data(), (uint32_t)rects.size()))) return e;
+    }
+    return VK_SUCCESS;
+}
+
+// DrawIndirectCommand / DrawIndexedIndirectCommand (CommandBuffer.Draw.cpp:1874-1996)
+// read their VkDraw(Indexed)IndirectCommand records when the command runs;
+// here they are read when the command buffer is replayed, after the stream
+// has finished the work recorded before them, and become direct draws.
+VkResult read_indirect(Device* dev, const Cmd& c, std::vector<Cmd>* out)
+{
+    const bool indexed = c.op == Op::DrawIndexedIndirect;
+    const uint32_t rec = indexed ? (uint32_t)sizeof(VkDrawIndexedIndirectCommand) : (uint32_t)sizeof(VkDrawIndirectCommand);
+    const uint32_t stride = c.count > 1 ? c.stride : rec;
+    if (!c.count) return VK_SUCCESS;
+    const uint64_t span = (uint64_t)stride * (c.count - 1) + rec;
+    if (c.offset + span > c.src_buffer->size) return VK_ERROR_DEVICE_LOST;
+    std::vector<uint8_t> host(span);
+    if (c.src_buffer->memory && c.src_buffer->memory->host) {
+        if (VkResult e = device_error(cpv_synchronize(dev->ctx))) return e;
+        std::memcpy(host.data(), (const void*)(uintptr_t)(c.src_buffer->address() + c.offset), span);
+    } else {
+        if (VkResult e = device_error(cpv_download(dev->ctx, host.data(), c.src_buffer->address() + c.offset, span))) return e;
+    }
+    for (uint32_t j = 0; j < c.count; j++) {
+        Cmd d{indexed ? Op::DrawIndexed : Op::Draw};
+        if (indexed) {
+            VkDrawIndexedIndirectCommand r;
+            std::memcpy(&r, host.data() + (uint64_t)j * stride, sizeof(r));
+            d.count = r.indexCount;
+            d.instances = r.instanceCount;
+            d.first_vertex = r.firstIndex;
+            d.vertex_offset = r.vertexOffset;
+            d.first_instance = r.firstInstance;
+        } else {
+            VkDrawIndirectCommand r;
+            std::memcpy(&r, host.data() + (uint64_t)j * stride, sizeof(r));
+            d.count = r.vertexCount;
+            d.instances = r.instanceCount;
+            d.first_vertex = r.firstVertex;
+            d.first_instance = r.firstInstance;
+        }
+        out->push_back(d);
+    }
+    return VK_SUCCESS;
+}
+
+VkResult record_draw(ReplayState& s, const Cmd& c)
+{
+    if (!s.pass || !s.pipeline) return VK_ERROR_DEVICE_LOST;
+    if (!c.count || !c.instances) return VK_SUCCESS;
+    uint32_t pi = 0;
+    for (; pi < s.pipeline_objs.size(); pi++)
+        if (s.pipeline_objs[pi] == s.pipeline) break;
+    if (pi == s.pipeline_objs.size()) {
+        s.pipeline_objs.push_back(s.pipeline);
+        s.pipelines.push_back(s.pipeline->desc);
+    }
+    s.draws.push_back(make_draw(s, c));
+    s.draw_pipeline.push_back(pi);
+    return VK_SUCCESS;
+}
+
 VkResult replay(Device* dev, const CommandBuffer* cb)
 {
     ReplayState s;
@@ -988,6 +1366,7 @@ VkResult replay(Device* dev, const CommandBuffer* cb)
             s.pass = c.pass;
             s.fb = c.fb;
             s.clears = c.clears;
+            s.pass_split = false;
             break;
         case Op::EndPass: {
             const VkResult r = end_pass(dev, s);
@@ -1011,19 +1390,24 @@ VkResult replay(Device* dev, const CommandBuffer* cb)
             if (c.first == 0 && !c.sets.empty()) s.set0 = c.sets[0];
             break;
         case Op::SetViewport: s.viewport = c.viewport; break;
-        case Op::Draw: case Op::DrawIndexed: {
+        case Op::Draw: case Op::DrawIndexed:
             if (!s.pass || !s.pipeline) return VK_ERROR_DEVICE_LOST;
-            uint32_t pi = 0;
-            for (; pi < s.pipeline_objs.size(); pi++)
-                if (s.pipeline_objs[pi] == s.pipeline) break;
-            if (pi == s.pipeline_objs.size()) {
-                s.pipeline_objs.push_back(s.pipeline);
-                s.pipelines.push_back(s.pipeline->desc);
-            }
-            s.draws.push_back(make_draw(s, c));
-            s.draw_pipeline.push_back(pi);
+            if (VkResult e = record_draw(s, c)) return e;
+            break;
+        case Op::DrawIndirect: case Op::DrawIndexedIndirect: {
+            std::vector<Cmd> direct;
+            if (VkResult e = read_indirect(dev, c, &direct)) return e;
+            for (const Cmd& d : direct)
+                if (VkResult e = record_draw(s, d)) return e;
             break;
         }
+        case Op::ClearAttachments:
+            if (VkResult e = clear_attachments(dev, s, c)) return e;
+            break;
+        default:
+            if (s.pass) return VK_ERROR_DEVICE_LOST;   // transfers are recorded outside render passes
+            if (VkResult e = run_transfer(dev, c)) return e;
+            break;
         }
     }
     return VK_SUCCESS;
@@ -1114,6 +1498,9 @@ const Entry kDevice[] = {
     E(DestroyCommandPool), E(AllocateCommandBuffers), E(FreeCommandBuffers), E(BeginCommandBuffer), E(EndCommandBuffer),
     E(ResetCommandBuffer), E(CmdBeginRenderPass), E(CmdEndRenderPass), E(CmdBindPipeline), E(CmdBindVertexBuffers),
     E(CmdBindIndexBuffer), E(CmdBindDescriptorSets), E(CmdSetViewport), E(CmdSetScissor), E(CmdDraw), E(CmdDrawIndexed),
+    E(CmdDrawIndirect), E(CmdDrawIndexedIndirect), E(CmdCopyBuffer), E(CmdCopyImage), E(CmdCopyBufferToImage),
+    E(CmdCopyImageToBuffer), E(CmdFillBuffer), E(CmdUpdateBuffer), E(CmdClearColorImage), E(CmdClearDepthStencilImage),
+    E(CmdClearAttachments), E(CmdBlitImage),
     E(QueueSubmit), E(QueueWaitIdle), E(CreateFence), E(DestroyFence), E(WaitForFences), E(GetFenceStatus), E(ResetFences),
 };
 #undef E

@@ -177,6 +177,13 @@ raster_kernel(RasterArgs a)
                 load_pixel((const uint8_t*)tp.ds_addr + pix * tp.ds_bytes, tp.ds_bytes, w4);
                 px[k].ds0 = w4[0];
                 px[k].ds1 = w4[1];
+            } else if (inside[k] && tp.has_ds && tp.ds_bytes == 8) {
+                // D32_SFLOAT_S8_UINT cleared: SetPixelDepthStencil writes the
+                // depth and the stencil byte and keeps the three padding bytes
+                // (ImageCompiler.cpp:895-945)
+                uint32_t w4[4];
+                load_pixel((const uint8_t*)tp.ds_addr + pix * tp.ds_bytes, tp.ds_bytes, w4);
+                px[k].ds1 = (w4[1] & 0xFFFFFF00u) | (clear.ds1 & 0xFFu);
             }
         }
 

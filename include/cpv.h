@@ -27,6 +27,16 @@
  *                           memory that vkMapMemory returns directly.
  *   cpv_device_copy      -> the transfer half of vkCmdCopyBuffer
  *                           (CPVulkan/CommandBuffer.Copy.cpp:15-60).
+ *   cpv_copy_regions     -> the byte copies of CopyBufferCommand, CopyImageCommand,
+ *                           CopyBufferToImageCommand and CopyImageToBufferCommand
+ *                           (CPVulkan/CommandBuffer.Copy.cpp:38-1080), whose
+ *                           region arithmetic the caller turns into rows.
+ *   cpv_fill_buffer / cpv_update_buffer -> FillBufferCommand / UpdateBufferCommand
+ *                           (CPVulkan/CommandBuffer.cpp:241-333).
+ *   cpv_clear_rects      -> ClearImage for vkCmdClearColorImage and
+ *                           vkCmdClearDepthStencilImage, and ClearAttachmentsCommand
+ *                           (CPVulkan/CommandBuffer.Draw.cpp:117-149, 2032-2224, 2398-2419).
+ *   cpv_blit_image       -> BlitImageCommand (CPVulkan/CommandBuffer.cpp:32-238).
  *   cpv_batch_*, cpv_autotune, cpv_mlp_*, cpv_set_predictor, cpv_get_tile_traces,
  *   cpv_set_config       -> no reference counterpart: recorded hipGraph
  *                           batches, the tile-configuration autotuner and the
@@ -252,6 +262,52 @@ int cpv_synchronize(cpv_context* ctx);
  * and size); the transfer half of vkCmdCopyBuffer
  * (CPVulkan/CommandBuffer.Copy.cpp:15-60) for GPU-resident buffers. */
 int cpv_device_copy(cpv_context* ctx, uint64_t dst, uint64_t src, uint64_t bytes);
+
+/* ---------------------------------------------------------------- transfers
+ * Each is enqueued on the context's stream, in order with render passes;
+ * addresses are device addresses or host-visible (cpv_alloc_host) memory. */
+
+/* rows x row_bytes bytes: row r goes from src + r * src_pitch to dst + r * dst_pitch. */
+typedef struct cpv_copy_region {
+    uint64_t src, dst;
+    uint64_t src_pitch, dst_pitch;
+    uint64_t row_bytes;
+    uint32_t rows, pad;
+} cpv_copy_region;
+
+/* One mip level of one array layer of a linear image: rows of width x
+ * pixel-size bytes with no padding (GetNormalImageSize, CPVulkanBase/Formats.cpp:455-482). */
+typedef struct cpv_image_level {
+    uint64_t address;
+    uint32_t format, width, height, pad;
+} cpv_image_level;
+
+typedef struct cpv_rect {
+    int32_t x, y;
+    uint32_t width, height;
+} cpv_rect;
+
+/* VkImageBlit of one layer: corners (x0, y0, x1, y1) of the source and destination rectangles. */
+typedef struct cpv_blit_region {
+    int32_t src[4];
+    int32_t dst[4];
+    uint32_t aspect;   /* VkImageAspectFlags: COLOR 1, DEPTH 2, STENCIL 4 */
+    uint32_t pad;
+} cpv_blit_region;
+
+int cpv_copy_regions(cpv_context* ctx, const cpv_copy_region* regions, uint32_t count);
+/* vkCmdFillBuffer semantics of the reference, including its tail-byte placement for size % 4. */
+int cpv_fill_buffer(cpv_context* ctx, uint64_t dst, uint64_t size, uint32_t data);
+/* Copies `size` bytes of host data (staged at the call) to dst. */
+int cpv_update_buffer(cpv_context* ctx, uint64_t dst, const void* data, uint64_t size);
+/* Stores the packed clear value into every pixel of the rectangles.  value:
+ * VkClearColorValue words for colour formats (float, int or uint by the
+ * format's base type); depth float bits and stencil for depth/stencil formats. */
+int cpv_clear_rects(cpv_context* ctx, const cpv_image_level* image, const uint32_t value[4],
+                    const cpv_rect* rects, uint32_t count);
+/* filter: VK_FILTER_NEAREST (0) or VK_FILTER_LINEAR (1); depth/stencil and integer formats take NEAREST. */
+int cpv_blit_image(cpv_context* ctx, const cpv_image_level* src, const cpv_image_level* dst,
+                   const cpv_blit_region* regions, uint32_t count, uint32_t filter);
 int cpv_set_config(cpv_context* ctx, const cpv_config* config);
 int cpv_get_config(cpv_context* ctx, cpv_config* config);
 

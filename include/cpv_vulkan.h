@@ -38,6 +38,7 @@ extern "C" {
 #define VK_MAX_MEMORY_TYPES 32
 #define VK_MAX_MEMORY_HEAPS 16
 #define VK_WHOLE_SIZE (~0ULL)
+#define VK_REMAINING_MIP_LEVELS (~0U)
 #define VK_ATTACHMENT_UNUSED (~0U)
 #define VK_TRUE 1
 #define VK_FALSE 0
@@ -120,6 +121,8 @@ enum { VK_IMAGE_TYPE_2D = 1 };
 enum { VK_DESCRIPTOR_TYPE_COMBINED_IMAGE_SAMPLER = 1, VK_DESCRIPTOR_TYPE_UNIFORM_BUFFER = 6 };
 enum { VK_DYNAMIC_STATE_VIEWPORT = 0, VK_DYNAMIC_STATE_SCISSOR = 1 };
 enum { VK_INDEX_TYPE_UINT16 = 0, VK_INDEX_TYPE_UINT32 = 1 };
+enum { VK_IMAGE_ASPECT_COLOR_BIT = 1, VK_IMAGE_ASPECT_DEPTH_BIT = 2, VK_IMAGE_ASPECT_STENCIL_BIT = 4 };
+enum { VK_FILTER_NEAREST = 0, VK_FILTER_LINEAR = 1 };
 enum { VK_SHADER_STAGE_VERTEX_BIT = 1, VK_SHADER_STAGE_FRAGMENT_BIT = 0x10 };
 enum { VK_PIPELINE_BIND_POINT_GRAPHICS = 0 };
 enum { VK_ATTACHMENT_LOAD_OP_LOAD = 0, VK_ATTACHMENT_LOAD_OP_CLEAR = 1, VK_ATTACHMENT_LOAD_OP_DONT_CARE = 2 };
@@ -400,6 +403,28 @@ typedef struct VkSubmitInfo {
 } VkSubmitInfo;
 typedef struct VkFenceCreateInfo { VkStructureType sType; const void* pNext; VkFlags flags; } VkFenceCreateInfo;
 
+/* Transfer, clear and indirect-draw commands. */
+typedef struct VkOffset3D { int32_t x, y, z; } VkOffset3D;
+typedef struct VkImageSubresourceLayers { VkFlags aspectMask; uint32_t mipLevel, baseArrayLayer, layerCount; } VkImageSubresourceLayers;
+typedef struct VkBufferCopy { VkDeviceSize srcOffset, dstOffset, size; } VkBufferCopy;
+typedef struct VkImageCopy {
+    VkImageSubresourceLayers srcSubresource; VkOffset3D srcOffset; VkImageSubresourceLayers dstSubresource; VkOffset3D dstOffset;
+    VkExtent3D extent;
+} VkImageCopy;
+typedef struct VkBufferImageCopy {
+    VkDeviceSize bufferOffset; uint32_t bufferRowLength, bufferImageHeight; VkImageSubresourceLayers imageSubresource;
+    VkOffset3D imageOffset; VkExtent3D imageExtent;
+} VkBufferImageCopy;
+typedef struct VkImageBlit {
+    VkImageSubresourceLayers srcSubresource; VkOffset3D srcOffsets[2]; VkImageSubresourceLayers dstSubresource; VkOffset3D dstOffsets[2];
+} VkImageBlit;
+typedef struct VkClearAttachment { VkFlags aspectMask; uint32_t colorAttachment; VkClearValue clearValue; } VkClearAttachment;
+typedef struct VkClearRect { VkRect2D rect; uint32_t baseArrayLayer, layerCount; } VkClearRect;
+typedef struct VkDrawIndirectCommand { uint32_t vertexCount, instanceCount, firstVertex, firstInstance; } VkDrawIndirectCommand;
+typedef struct VkDrawIndexedIndirectCommand {
+    uint32_t indexCount, instanceCount, firstIndex; int32_t vertexOffset; uint32_t firstInstance;
+} VkDrawIndexedIndirectCommand;
+
 /* Layout checks: sizes from the specification's member lists (64-bit). */
 CPV_VK_STATIC_ASSERT(sizeof(VkApplicationInfo) == 48, "VkApplicationInfo");
 CPV_VK_STATIC_ASSERT(sizeof(VkInstanceCreateInfo) == 64, "VkInstanceCreateInfo");
@@ -447,6 +472,12 @@ CPV_VK_STATIC_ASSERT(sizeof(VkRenderPassBeginInfo) == 64, "VkRenderPassBeginInfo
 CPV_VK_STATIC_ASSERT(sizeof(VkSubmitInfo) == 72, "VkSubmitInfo");
 CPV_VK_STATIC_ASSERT(sizeof(VkFenceCreateInfo) == 24, "VkFenceCreateInfo");
 CPV_VK_STATIC_ASSERT(offsetof(VkGraphicsPipelineCreateInfo, layout) == 104, "VkGraphicsPipelineCreateInfo.layout");
+CPV_VK_STATIC_ASSERT(sizeof(VkImageCopy) == 68, "VkImageCopy");
+CPV_VK_STATIC_ASSERT(sizeof(VkBufferImageCopy) == 56, "VkBufferImageCopy");
+CPV_VK_STATIC_ASSERT(sizeof(VkImageBlit) == 80, "VkImageBlit");
+CPV_VK_STATIC_ASSERT(sizeof(VkClearAttachment) == 24, "VkClearAttachment");
+CPV_VK_STATIC_ASSERT(sizeof(VkClearRect) == 24, "VkClearRect");
+CPV_VK_STATIC_ASSERT(sizeof(VkDrawIndexedIndirectCommand) == 20, "VkDrawIndexedIndirectCommand");
 CPV_VK_STATIC_ASSERT(offsetof(VkPhysicalDeviceProperties, limits) == 296, "VkPhysicalDeviceProperties.limits");
 
 /* Entry points of the ICD interface (CPVulkan/CPVulkan.cpp:14-104 is the

@@ -430,16 +430,16 @@ static void get_level(const cpv_texture_desc* t, uint32_t view_level, level_t* l
     lv->h = h;
 }
 
-/* ImageSampleImplicitLod -> ImageSampleExplicitLod with lambda base 0
- * (GlslFunctions.cpp:598-671) -> SampleImage (ImageSampler.cpp:581-673). */
-int cpvref_sample_implicit(const cpv_texture_desc* t, float u, float v, float out[4])
+/* ImageSampleExplicitLod (GlslFunctions.cpp:598-654) -> SampleImage
+ * (ImageSampler.cpp:581-673): lambda = clamp(lod + clamp(bias, +-32), minLod, maxLod). */
+int cpvref_sample_explicit(const cpv_texture_desc* t, float u, float v, float lod, float out[4])
 {
     const cpv_sampler_desc* s = &t->sampler;
     fmt_t f;
     if (!fmt_get(t->format, &f) || f.type != FT_NORMAL) return CPV_ERROR_UNSUPPORTED;
     float bias = s->mip_lod_bias + 0.0f;
     bias = bias < -32.0f ? -32.0f : bias > 32.0f ? 32.0f : bias;   /* MAX_SAMPLER_LOD_BIAS */
-    float lambda = 0.0f + bias;
+    float lambda = lod + bias;
     lambda = lambda < s->min_lod ? s->min_lod : lambda > s->max_lod ? s->max_lod : lambda;
     uint32_t levels = t->level_count;
     level_t lv;
@@ -473,6 +473,25 @@ int cpvref_sample_implicit(const cpv_texture_desc* t, float u, float v, float ou
     if (f.off[1] < 0) out[1] = 0;
     if (f.off[2] < 0) out[2] = 0;
     if (f.off[3] < 0) out[3] = 1;
+    return CPV_OK;
+}
+
+/* ImageSampleImplicitLod: lambda base 0 (GlslFunctions.cpp:656-671). */
+int cpvref_sample_implicit(const cpv_texture_desc* t, float u, float v, float out[4])
+{
+    return cpvref_sample_explicit(t, u, v, 0.0f, out);
+}
+
+/* ImageFetch (GlslFunctions.cpp:673-734): the texel of the view's base
+ * level; outside it, ReturnType{} = (0, 0, 0, 0). */
+int cpvref_fetch(const cpv_texture_desc* t, int32_t x, int32_t y, float out[4])
+{
+    fmt_t f;
+    if (!fmt_get(t->format, &f) || f.type != FT_NORMAL) return CPV_ERROR_UNSUPPORTED;
+    level_t lv;
+    get_level(t, 0, &lv);
+    static const float zero[4] = {0, 0, 0, 0};
+    texel(t->format, &lv, x, y, zero, out);
     return CPV_OK;
 }
 
@@ -1106,10 +1125,10 @@ static int clear_pass(const cpv_pass_desc* pass)
         const cpv_attachment_desc* at = &pass->depth_stencil;
         fmt_t f;
         if (!fmt_get(at->format, &f) || f.type != FT_DEPTH_STENCIL) return CPV_ERROR_UNSUPPORTED;
-        uint8_t px[8] = {0};
-        cpvref_pack_depth_stencil(at->format, at->clear.ds.depth, at->clear.ds.stencil, px);
+        /* SetPixelDepthStencil per pixel, in place: the depth and stencil
+         * bytes only (D32_SFLOAT_S8_UINT keeps its padding, ImageCompiler.cpp:895-945) */
         uint8_t* dst = (uint8_t*)(uintptr_t)at->address;
-        for (uint64_t i = 0; i < npx; i++) memcpy(dst + i * f.total, px, f.total);
+        for (uint64_t i = 0; i < npx; i++) cpvref_pack_depth_stencil(at->format, at->clear.ds.depth, at->clear.ds.stencil, dst + i * f.total);
     }
     return CPV_OK;
 }
@@ -1180,4 +1199,123 @@ int cpvref_pixel_coverage(const float positions[12], uint32_t cull_mode, uint32_
     weights[0] = w0; weights[1] = w1; weights[2] = w2;
     *depth = P[0][2] * w0 + P[1][2] * w1 + P[2][2] * w2;
     return 1;
+}
+
+
+/* ------------------------------------------------------------------ */
+/* Transfers (CPVulkan/CommandBuffer.cpp, CommandBuffer.Draw.cpp)      */
+/* ------------------------------------------------------------------ */
+
+/* FillBufferCommand::Process (CommandBuffer.cpp:307-326): whole words,
+ * then the size % 4 tail bytes at byte size / 4 + i. */
+void cpvref_fill_buffer(uint8_t* dst, uint64_t size, uint32_t data)
+{
+    uint64_t words = size / 4;
+    for (uint64_t i = 0; i < words; i++) memcpy(dst + 4 * i, &data, 4);
+    const uint8_t* bytes = (const uint8_t*)&data;
+    for (uint64_t i = 0; i < size % 4; i++) dst[size / 4 + i] = bytes[i];
+}
+
+/* SetPixel of a VkClearColorValue / VkClearDepthStencilValue
+ * (ImageSampler.cpp:723-800), chosen by the format's base type. */
+static int pack_clear(uint32_t format, const uint32_t value[4], uint8_t* px)
+{
+    fmt_t f;
+    if (!fmt_get(format, &f)) return CPV_ERROR_UNSUPPORTED;
+    if (f.type == FT_DEPTH_STENCIL) {
+        float depth;
+        memcpy(&depth, &value[0], 4);
+        return cpvref_pack_depth_stencil(format, depth, value[1], px);
+    }
+    if (f.base == BT_UINT) return cpvref_pack_u32(format, value, px);
+    if (f.base == BT_SINT) return cpvref_pack_i32(format, (const int32_t*)value, px);
+    float c[4];
+    memcpy(c, value, 16);
+    return cpvref_pack_f32(format, c, px);
+}
+
+/* ClearImage / ClearAttachmentsCommand (CommandBuffer.Draw.cpp:117-149,
+ * 2176-2220): SetPixel of the clear value at every pixel of each rect. */
+int cpvref_clear_rects(const cpv_image_level* image, const uint32_t value[4], const cpv_rect* rects, uint32_t count)
+{
+    uint32_t bytes = cpvref_format_size(image->format);
+    uint8_t* base = (uint8_t*)(uintptr_t)image->address;
+    /* packed in place: a depth/stencil store writes only its depth and
+     * stencil bytes (D32_SFLOAT_S8_UINT keeps its padding) */
+    for (uint32_t i = 0; i < count; i++)
+        for (uint32_t y = 0; y < rects[i].height; y++)
+            for (uint32_t x = 0; x < rects[i].width; x++) {
+                int rc = pack_clear(image->format, value,
+                                    base + ((uint64_t)(rects[i].y + (int32_t)y) * image->width + (uint64_t)(rects[i].x + (int32_t)x)) * bytes);
+                if (rc) return rc;
+            }
+    return CPV_OK;
+}
+
+/* BlitImageCommand::Process (CommandBuffer.cpp:57-238) for one layer:
+ * colour formats sample the source level at (u / W, v / H) with the blit
+ * filter and CLAMP_TO_EDGE (SampleImage, ImageSampler.cpp:676-681), integer
+ * formats take the nearest texel's channels, depth/stencil formats read the
+ * texel at the truncated (u, v); each destination pixel is written with SetPixel. */
+int cpvref_blit_image(const cpv_image_level* src, const cpv_image_level* dst, const cpv_blit_region* regions, uint32_t count,
+                      uint32_t filter)
+{
+    fmt_t sf, df;
+    if (!fmt_get(src->format, &sf) || !fmt_get(dst->format, &df)) return CPV_ERROR_UNSUPPORTED;
+    const uint32_t sbytes = cpvref_format_size(src->format), dbytes = cpvref_format_size(dst->format);
+    const uint8_t* sbase = (const uint8_t*)(uintptr_t)src->address;
+    uint8_t* dbase = (uint8_t*)(uintptr_t)dst->address;
+    level_t lv = {sbase, src->width, src->height};
+    cpv_sampler_desc s;
+    memset(&s, 0, sizeof(s));
+    s.address_u = s.address_v = CPV_ADDRESS_CLAMP_TO_EDGE;
+    for (uint32_t r = 0; r < count; r++) {
+        const cpv_blit_region* g = &regions[r];
+        int32_t w = g->dst[2] - g->dst[0], h = g->dst[3] - g->dst[1];
+        const int negx = w < 0, negy = h < 0;
+        if (negx) w = -w;
+        if (negy) h = -h;
+        const float rx = (float)(g->src[2] - g->src[0]) / (float)(g->dst[2] - g->dst[0]);
+        const float ry = (float)(g->src[3] - g->src[1]) / (float)(g->dst[3] - g->dst[1]);
+        for (int32_t y = 0; y < h; y++) {
+            for (int32_t x = 0; x < w; x++) {
+                const int32_t dx = negx ? x + g->dst[2] : x + g->dst[0];
+                const int32_t dy = negy ? y + g->dst[3] : y + g->dst[1];
+                const float u = ((float)dx + 0.5f - (float)g->dst[0]) * rx + (float)g->src[0];
+                const float v = ((float)dy + 0.5f - (float)g->dst[1]) * ry + (float)g->src[1];
+                uint8_t* out = dbase + ((uint64_t)dy * dst->width + (uint64_t)dx) * dbytes;
+                if (sf.type == FT_DEPTH_STENCIL) {
+                    uint32_t sx = (uint32_t)u, sy = (uint32_t)v;
+                    const uint8_t* in = sbase + ((uint64_t)sy * src->width + sx) * sbytes;
+                    float depth = df.depth_off >= 0 ? cpvref_unpack_depth(dst->format, out) : 0.0f;
+                    uint32_t stencil = df.stencil_off >= 0 ? cpvref_unpack_stencil(dst->format, out) : 0u;
+                    if (g->aspect & 2u) depth = cpvref_unpack_depth(src->format, in);
+                    if (g->aspect & 4u) stencil = sf.stencil_off >= 0 ? cpvref_unpack_stencil(src->format, in) : 0u;
+                    cpvref_pack_depth_stencil(dst->format, depth, stencil, out);
+                } else if (sf.base == BT_UINT || sf.base == BT_SINT) {
+                    int32_t cx = (int32_t)floorf(u / (float)src->width * (float)src->width);
+                    int32_t cy = (int32_t)floorf(v / (float)src->height * (float)src->height);
+                    cx = wrap_coord(cx, (int32_t)src->width, CPV_ADDRESS_CLAMP_TO_EDGE);
+                    cy = wrap_coord(cy, (int32_t)src->height, CPV_ADDRESS_CLAMP_TO_EDGE);
+                    const uint8_t* in = sbase + ((uint64_t)cy * src->width + (uint64_t)cx) * sbytes;
+                    uint32_t ch[4] = {0, 0, 0, 1};
+                    for (int k = 0; k < 4; k++) {
+                        if (sf.off[k] < 0) continue;
+                        ch[k] = sf.base == BT_SINT ? (uint32_t)load_sint(in + sf.off[k], sf.element) : load_uint(in + sf.off[k], sf.element);
+                    }
+                    if (sf.base == BT_UINT) cpvref_pack_u32(dst->format, ch, out);
+                    else cpvref_pack_i32(dst->format, (const int32_t*)ch, out);
+                } else {
+                    float c[4];
+                    sample_level(src->format, &lv, u / (float)src->width, v / (float)src->height, filter, &s, c);
+                    if (sf.off[0] < 0) c[0] = 0;
+                    if (sf.off[1] < 0) c[1] = 0;
+                    if (sf.off[2] < 0) c[2] = 0;
+                    if (sf.off[3] < 0) c[3] = 1;
+                    cpvref_pack_f32(dst->format, c, out);
+                }
+            }
+        }
+    }
+    return CPV_OK;
 }

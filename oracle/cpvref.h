@@ -51,8 +51,17 @@ uint32_t cpvref_format_size(uint32_t format);
 uint64_t cpvref_image_layout(uint32_t format, uint32_t width, uint32_t height, uint32_t levels,
                              uint64_t* level_offsets);
 
-/* Texture sampling through ImageSampleImplicitLod (lambda = 0). */
+/* Texture sampling through ImageSampleImplicitLod (lambda = 0),
+ * ImageSampleExplicitLod (lambda base = lod) and ImageFetch. */
 int cpvref_sample_implicit(const cpv_texture_desc* texture, float u, float v, float rgba[4]);
+int cpvref_sample_explicit(const cpv_texture_desc* texture, float u, float v, float lod, float rgba[4]);
+int cpvref_fetch(const cpv_texture_desc* texture, int32_t x, int32_t y, float rgba[4]);
+
+/* Transfers, with host addresses in the same structs as the C-ABI. */
+void cpvref_fill_buffer(uint8_t* dst, uint64_t size, uint32_t data);
+int cpvref_clear_rects(const cpv_image_level* image, const uint32_t value[4], const cpv_rect* rects, uint32_t count);
+int cpvref_blit_image(const cpv_image_level* src, const cpv_image_level* dst, const cpv_blit_region* regions, uint32_t count,
+                      uint32_t filter);
 
 /* Blend of one fragment (ApplyBlend, dead code in the reference). */
 void cpvref_apply_blend(const float src[4], const float dst[4], const float constant[4],

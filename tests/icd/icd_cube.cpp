@@ -2,7 +2,13 @@
 // frame through the driver's own entry points (as the loader would reach
 // them) and writes the colour and depth images.
 //
-//   icd_cube <icd.so> <dir> <width> <height>
+//   icd_cube <icd.so> <dir> <width> <height> [draw|indirect|midclear]
+//
+// draw: vkCmdDraw(36, 1, 0, 0); indirect: the same vertices as two
+// VkDrawIndirectCommand records read by vkCmdDrawIndirect; midclear:
+// vkCmdDraw, then vkCmdClearAttachments of a colour and a depth rectangle
+// (colour (1, 0, 0.5, 1), depth 0.5) over [W/4, W/4 + W/2) x [H/4, H/4 + H/3),
+// then vkCmdDraw again, all in one render pass.
 //
 // <dir> holds vs.spv, fs.spv (SPIR-V), vertices.bin (36 x {vec4 pos, vec4
 // colour}) and mvp.bin (mat4); the program writes color.bin (B8G8R8A8) and
@@ -60,10 +66,11 @@ template <class F> F load(const char* name)
 
 int main(int argc, char** argv)
 {
-    if (argc != 5) {
-        std::fprintf(stderr, "usage: icd_cube <icd.so> <dir> <width> <height>\n");
+    if (argc != 5 && argc != 6) {
+        std::fprintf(stderr, "usage: icd_cube <icd.so> <dir> <width> <height> [draw|indirect|midclear]\n");
         return 2;
     }
+    const std::string mode = argc == 6 ? argv[5] : "draw";
     const std::string dir = argv[2];
     const uint32_t W = (uint32_t)std::atoi(argv[3]), H = (uint32_t)std::atoi(argv[4]);
     void* lib = dlopen(argv[1], RTLD_NOW | RTLD_LOCAL);
@@ -121,6 +128,8 @@ int main(int argc, char** argv)
     VFN(CmdSetViewport, VkCommandBuffer, uint32_t, uint32_t, const VkViewport*);
     VFN(CmdSetScissor, VkCommandBuffer, uint32_t, uint32_t, const VkRect2D*);
     VFN(CmdDraw, VkCommandBuffer, uint32_t, uint32_t, uint32_t, uint32_t);
+    VFN(CmdDrawIndirect, VkCommandBuffer, VkBuffer, VkDeviceSize, uint32_t, uint32_t);
+    VFN(CmdClearAttachments, VkCommandBuffer, uint32_t, const VkClearAttachment*, uint32_t, const VkClearRect*);
     FN(QueueSubmit, VkQueue, uint32_t, const VkSubmitInfo*, VkFence);
     FN(CreateFence, VkDevice, const VkFenceCreateInfo*, const VkAllocationCallbacks*, VkFence*);
     FN(WaitForFences, VkDevice, uint32_t, const VkFence*, VkBool32, uint64_t);
@@ -275,7 +284,30 @@ int main(int argc, char** argv)
     vkCmdSetScissor(cmd, 0, 1, &sc);
     const VkDeviceSize off = 0;
     vkCmdBindVertexBuffers(cmd, 0, 1, &vbuf, &off);
-    vkCmdDraw(cmd, 36, 1, 0, 0);   // 15-draw_cube.cpp:149
+    VkBuffer ibuf = nullptr;
+    if (mode == "indirect") {
+        const VkDrawIndirectCommand recs[2] = {{18, 1, 0, 0}, {18, 1, 18, 0}};
+        std::vector<char> bytes((const char*)recs, (const char*)recs + sizeof(recs));
+        buffer(bytes, &ibuf);
+        vkCmdDrawIndirect(cmd, ibuf, 0, 2, sizeof(VkDrawIndirectCommand));
+    } else {
+        vkCmdDraw(cmd, 36, 1, 0, 0);   // 15-draw_cube.cpp:149
+    }
+    if (mode == "midclear") {
+        VkClearAttachment ca[2];
+        ca[0].aspectMask = VK_IMAGE_ASPECT_COLOR_BIT;
+        ca[0].colorAttachment = 0;
+        ca[0].clearValue.color.float32[0] = 1.0f;
+        ca[0].clearValue.color.float32[1] = 0.0f;
+        ca[0].clearValue.color.float32[2] = 0.5f;
+        ca[0].clearValue.color.float32[3] = 1.0f;
+        ca[1].aspectMask = VK_IMAGE_ASPECT_DEPTH_BIT;
+        ca[1].colorAttachment = 0;
+        ca[1].clearValue.depthStencil = VkClearDepthStencilValue{0.5f, 0};
+        VkClearRect cr{{{(int32_t)(W / 4), (int32_t)(H / 4)}, {W / 2, H / 3}}, 0, 1};
+        vkCmdClearAttachments(cmd, 2, ca, 1, &cr);
+        vkCmdDraw(cmd, 36, 1, 0, 0);
+    }
     vkCmdEndRenderPass(cmd);
     CHECK(vkEndCommandBuffer(cmd));
 

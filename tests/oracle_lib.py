@@ -45,6 +45,13 @@ def lib() -> C.CDLL:
         L.cpvref_image_layout.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
         L.cpvref_image_layout.restype = C.c_uint64
         L.cpvref_sample_implicit.argtypes = [C.POINTER(abi.TextureDesc), C.c_float, C.c_float, C.POINTER(C.c_float)]
+        L.cpvref_sample_explicit.argtypes = [C.POINTER(abi.TextureDesc), C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)]
+        L.cpvref_fetch.argtypes = [C.POINTER(abi.TextureDesc), C.c_int32, C.c_int32, C.POINTER(C.c_float)]
+        L.cpvref_fill_buffer.argtypes = [P, C.c_uint64, C.c_uint32]
+        L.cpvref_fill_buffer.restype = None
+        L.cpvref_clear_rects.argtypes = [C.POINTER(abi.ImageLevel), C.POINTER(C.c_uint32), C.POINTER(abi.Rect), C.c_uint32]
+        L.cpvref_blit_image.argtypes = [C.POINTER(abi.ImageLevel), C.POINTER(abi.ImageLevel), C.POINTER(abi.BlitRegion),
+                                        C.c_uint32, C.c_uint32]
         L.cpvref_apply_blend.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                          C.POINTER(abi.BlendAttachment), C.POINTER(C.c_float)]
         L.cpvref_apply_blend.restype = None
@@ -78,3 +85,28 @@ def render_scene(scene: Scene) -> tuple[np.ndarray, np.ndarray, int]:
     m = materialize(scene, HostMemory())
     frags = render(m)
     return m.read_color(), m.read_ds(), frags
+
+
+def fill_buffer(buf: np.ndarray, size: int, data: int, offset: int = 0) -> None:
+    """FillBuffer on a host copy (uint8 array) at byte offset."""
+    assert buf.dtype == np.uint8 and buf.flags.c_contiguous
+    lib().cpvref_fill_buffer(buf.ctypes.data + offset, size, data & 0xFFFFFFFF)
+
+
+def _level(img: np.ndarray, fmt: int, width: int, height: int) -> abi.ImageLevel:
+    assert img.dtype == np.uint8 and img.flags.c_contiguous
+    return abi.ImageLevel(img.ctypes.data, fmt, width, height, 0)
+
+
+def clear_rects(img: np.ndarray, fmt: int, width: int, height: int, value_words, rects) -> None:
+    """ClearImage / ClearAttachments on a host image (uint8 array of one level)."""
+    v = (C.c_uint32 * 4)(*[int(w) & 0xFFFFFFFF for w in value_words])
+    arr = (abi.Rect * max(len(rects), 1))(*rects)
+    assert lib().cpvref_clear_rects(C.byref(_level(img, fmt, width, height)), v, arr, len(rects)) == 0
+
+
+def blit_image(src: np.ndarray, sfmt: int, sw: int, sh: int, dst: np.ndarray, dfmt: int, dw: int, dh: int, regions,
+               filter_: int) -> None:
+    arr = (abi.BlitRegion * max(len(regions), 1))(*regions)
+    assert lib().cpvref_blit_image(C.byref(_level(src, sfmt, sw, sh)), C.byref(_level(dst, dfmt, dw, dh)), arr, len(regions),
+                                   filter_) == 0

@@ -60,7 +60,10 @@ def test_instance_and_physical_device(icd):
     assert create(C.addressof(info), None, C.byref(inst)) == 0
     # dispatchable handles start with the loader magic
     assert C.cast(inst, C.POINTER(C.c_uint64))[0] == 0x01CDC0DE
-    for name in (b"vkCmdDraw", b"vkCmdDrawIndexed", b"vkQueueSubmit", b"vkCreateGraphicsPipelines"):
+    for name in (b"vkCmdDraw", b"vkCmdDrawIndexed", b"vkQueueSubmit", b"vkCreateGraphicsPipelines", b"vkCmdDrawIndirect",
+                 b"vkCmdDrawIndexedIndirect", b"vkCmdCopyBuffer", b"vkCmdCopyImage", b"vkCmdCopyBufferToImage",
+                 b"vkCmdCopyImageToBuffer", b"vkCmdFillBuffer", b"vkCmdUpdateBuffer", b"vkCmdClearColorImage",
+                 b"vkCmdClearDepthStencilImage", b"vkCmdClearAttachments", b"vkCmdBlitImage"):
         assert icd.vk_icdGetInstanceProcAddr(inst, name), name
     enum = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p))(
         icd.vk_icdGetInstanceProcAddr(inst, b"vkEnumeratePhysicalDevices"))
