@@ -34,8 +34,8 @@ TOPOLOGY = dict(POINT_LIST=0, LINE_LIST=1, LINE_STRIP=2, TRIANGLE_LIST=3, TRIANG
 LOAD_OP = dict(LOAD=0, CLEAR=1, DONT_CARE=2)
 FILTER = dict(NEAREST=0, LINEAR=1)
 ADDRESS = dict(REPEAT=0, MIRRORED_REPEAT=1, CLAMP_TO_EDGE=2, CLAMP_TO_BORDER=3, MIRROR_CLAMP_TO_EDGE=4)
-VS = dict(PASSTHROUGH=1, MVP_COLOR=2, MVP_LAMBERT=3)
-FS = dict(COLOR=1, TEX_LAMBERT=2, COLOR_DISCARD=3)
+VS = dict(PASSTHROUGH=1, MVP_COLOR=2, MVP_LAMBERT=3, PASSTHROUGH_POINT=4)
+FS = dict(COLOR=1, TEX_LAMBERT=2, COLOR_DISCARD=3, COLOR_FLAT=4, COLOR_NOPERSPECTIVE=5)
 SCHED = dict(GRID=0, COUNTER=1, PREDICTED=2)
 COLOR_RGBA = 15
 
@@ -83,6 +83,7 @@ class PipelineDesc(C.Structure):
         ("color_formats", u32 * MAX_COLOR_ATTACHMENTS),
         ("depth_stencil_format", u32),
         ("blend", BlendAttachment * MAX_COLOR_ATTACHMENTS),
+        ("line_width", f32), ("reserved", u32),
     ]
 
 

@@ -21,6 +21,7 @@ struct SetupArgs {
     const TargetParams* targets;
     const float* vout;
     TriSetup* recs;
+    LineExt* ext;                 // per-primitive line extension (launches with lines)
     uint32_t* counts;             // tiles overlapped per primitive (0 = dropped)
     uint32_t stride;
     int32_t tile_shift;
@@ -77,6 +78,185 @@ __device__ __forceinline__ void depth_bounds(TriSetup& r, const float P[3][4], c
     r.dt_hi = fmaxf(tl, th);
 }
 
+// Vertex-output slot of assembled vertex `raw` of a draw (false: outside the
+// bound vertex buffer).
+__device__ __forceinline__ bool vertex_slot(const DrawDesc& d, uint32_t raw, uint32_t* slot)
+{
+    int64_t s = raw;
+    if (d.vmode == kSlotVertexId) {
+        uint32_t idx;
+        if (d.index_size == 1) idx = ((const uint8_t*)d.index_addr)[d.first + raw];
+        else if (d.index_size == 2) idx = ((const uint16_t*)d.index_addr)[d.first + raw];
+        else idx = ((const uint32_t*)d.index_addr)[d.first + raw];
+        s = (int64_t)d.vertex_offset + (int64_t)idx;
+    }
+    *slot = d.vout_base + (uint32_t)s;
+    return s >= 0 && s < (int64_t)d.vout_count;
+}
+
+__device__ __forceinline__ void clip_to_ndc(const float* vout, uint32_t stride, uint32_t slot, float P[4])
+{
+    const float4 q = *(const float4*)(vout + (uint64_t)slot * stride);
+    P[0] = q.x / q.w;
+    P[1] = q.y / q.w;
+    P[2] = q.z / q.w;
+    P[3] = q.w;
+}
+
+__device__ __forceinline__ void store_record(TriSetup* dst_rec, const TriSetup& r)
+{
+    uint4* dst = (uint4*)dst_rec;
+    const uint4* src = (const uint4*)&r;
+#pragma unroll
+    for (int q = 0; q < 8; q++) dst[q] = src[q];
+}
+
+__device__ __forceinline__ uint32_t tiles_of(int32_t x0, int32_t y0, int32_t x1, int32_t y1, int sh)
+{
+    const uint32_t tx0 = (uint32_t)x0 >> sh, tx1 = (uint32_t)(x1 - 1) >> sh;
+    const uint32_t ty0 = (uint32_t)y0 >> sh, ty1 = (uint32_t)(y1 - 1) >> sh;
+    return (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+}
+
+// 32-bit two's-complement sum, the int32 arithmetic of the reference's
+// point bounds as its compiled code performs it.
+__device__ __forceinline__ int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+__device__ __forceinline__ int32_t wrap_sub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+
+// ProcessPoints (CommandBuffer.Draw.cpp:1315-1383): centre
+// (int)((p.x + 1) * 0.5f * (viewport.width - 1)), half extent
+// (int)ceil(pointSize / 2), square [max(0, c - h), min(width, c + h + 1)).
+__device__ void setup_point(const SetupArgs& a, const DrawDesc& d, uint32_t di, uint32_t p, uint32_t gp)
+{
+    uint32_t slot;
+    if (!vertex_slot(d, p, &slot)) { a.counts[gp] = 0; return; }
+    float P[4];
+    clip_to_ndc(a.vout, a.stride, slot, P);
+    const float size = d.psize >= 0 ? a.vout[(uint64_t)slot * a.stride + 4 + (uint32_t)d.psize] : 0.0f;
+    const int32_t cx = trunc_to_i32((P[0] + 1.0f) * 0.5f * (d.vp_w - 1.0f));
+    const int32_t cy = trunc_to_i32((P[1] + 1.0f) * 0.5f * (d.vp_h - 1.0f));
+    const int32_t h = trunc_to_i32(ceilf(size / 2.0f));
+    const int32_t x0 = max(0, wrap_sub(cx, h)), y0 = max(0, wrap_sub(cy, h));
+    const int32_t x1 = min(d.limit_x, wrap_add(wrap_add(cx, h), 1)), y1 = min(d.limit_y, wrap_add(wrap_add(cy, h), 1));
+    if (x0 >= x1 || y0 >= y1) { a.counts[gp] = 0; return; }
+    TriSetup r;
+    r.ax[0] = __int_as_float(cx);
+    r.ay[0] = __int_as_float(cy);
+    r.dx[0] = size;
+    r.ax[1] = r.ax[2] = r.ay[1] = r.ay[2] = r.dx[1] = r.dx[2] = 0.0f;
+    r.dy[0] = r.dy[1] = r.dy[2] = 0.0f;
+    r.x0 = x0; r.y0 = y0; r.x1 = x1; r.y1 = y1;
+    r.dt_lo = -__builtin_inff();
+    r.dt_hi = __builtin_inff();
+    r.area = 1.0f;
+    r.draw = di;
+    r.z[0] = P[2]; r.z[1] = 0.0f; r.z[2] = 0.0f;
+    r.w[0] = r.w[1] = r.w[2] = P[3];
+    r.v[0] = r.v[1] = r.v[2] = slot;
+    r.flags = kPrimFront | kPrimPoint;   // DrawPixel(..., front = true, ...)
+    r.provoking = slot;
+    r.pad = 0;
+    store_record(&a.recs[gp], r);
+    a.counts[gp] = tiles_of(x0, y0, x1, y1, a.tile_shift);
+}
+
+// Pixel box that holds every pixel whose centre can pass the four edge
+// tests of a line quad with corners c[4] (NDC), or false when the quad is
+// too degenerate for the bound below (the caller then scans the viewport,
+// as the reference does for every line).
+//
+// The bound: in pixel space (x + 0.5 = (ndc + 1) * extent / 2) let the quad
+// have sides of at least 0.5 px and an angle with sine >= 0.1, corners
+// within 4 of the origin in NDC and viewport extents <= 32768.  A pixel
+// centre 3.5 px or more outside the corners' box is then at least
+// 3.5 * sin(angle / 2) >= 0.16 px outside some edge, i.e. >= 9.6e-6 NDC.
+// The float evaluation (x - ax) * dy - (y - ay) * dx of that edge errs by
+// at most 8 * 2^-24 * (|x - ax| + |y - ay|) * |d| <= 4.8e-6 * |d|, so the
+// edge value stays negative and the pixel fails.
+__device__ bool line_box(const DrawDesc& d, const float* c[4], int32_t* x0, int32_t* y0, int32_t* x1, int32_t* y1)
+{
+    if (!(d.vp_w <= 32768.0f) || !(d.vp_h <= 32768.0f)) return false;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (!(fabsf(c[k][0]) <= 4.0f) || !(fabsf(c[k][1]) <= 4.0f)) return false;
+    const double sx = 0.5 * (double)d.vp_w, sy = 0.5 * (double)d.vp_h;
+    // c = {p00, p01, p10, p11}: long side p00 -> p10, short side p00 -> p01
+    const double ux = ((double)c[2][0] - (double)c[0][0]) * sx, uy = ((double)c[2][1] - (double)c[0][1]) * sy;
+    const double vx = ((double)c[1][0] - (double)c[0][0]) * sx, vy = ((double)c[1][1] - (double)c[0][1]) * sy;
+    const double lu = sqrt(ux * ux + uy * uy), lv = sqrt(vx * vx + vy * vy);
+    if (!(lu >= 0.5) || !(lv >= 0.5) || !(fabs(ux * vy - uy * vx) >= 0.1 * lu * lv)) return false;
+    double mnx = 1e30, mxx = -1e30, mny = 1e30, mxy = -1e30;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double px = ((double)c[k][0] + 1.0) * sx, py = ((double)c[k][1] + 1.0) * sy;
+        mnx = fmin(mnx, px); mxx = fmax(mxx, px);
+        mny = fmin(mny, py); mxy = fmax(mxy, py);
+    }
+    *x0 = max(*x0, (int32_t)floor(mnx) - 4);
+    *y0 = max(*y0, (int32_t)floor(mny) - 4);
+    *x1 = min(*x1, (int32_t)floor(mxx) + 5);
+    *y1 = min(*y1, (int32_t)floor(mxy) + 5);
+    return true;
+}
+
+// ProcessLines (CommandBuffer.Draw.cpp:1385-1508): the quad around p0p1
+// with half widths LineWidth / (width, height) along (dir.y, -dir.x),
+// dir = normalize(p1 - p0) over all four components; a pixel is drawn when
+// its centre has E(p00, p01) >= 0, E(p11, p10) >= 0, E(p10, p00) >= 0 and
+// E(p01, p11) >= 0.  The reference scans the whole viewport; the record's
+// box is line_box's when it applies.
+__device__ void setup_line(const SetupArgs& a, const DrawDesc& d, uint32_t di, uint32_t p, uint32_t gp)
+{
+    const uint32_t r0 = d.topology == 1 ? 2 * p : p;   // LINE_LIST or LINE_STRIP (CalculatePrimitives :584-611)
+    uint32_t s0, s1;
+    if (!vertex_slot(d, r0, &s0) || !vertex_slot(d, r0 + 1, &s1)) { a.counts[gp] = 0; return; }
+    float P0[4], P1[4];
+    clip_to_ndc(a.vout, a.stride, s0, P0);
+    clip_to_ndc(a.vout, a.stride, s1, P1);
+    const float lwx = d.line_width / d.vp_w, lwy = d.line_width / d.vp_h;
+    const float vx = P1[0] - P0[0], vy = P1[1] - P0[1], vz = P1[2] - P0[2], vw = P1[3] - P0[3];
+    const float inv = 1.0f / sqrtf(((vx * vx + vy * vy) + vz * vz) + vw * vw);   // glm::normalize (vec4)
+    const float px = vy * inv, py = -(vx * inv);
+    const float p00[2] = {P0[0] + px * lwx, P0[1] + py * lwy}, p01[2] = {P0[0] - px * lwx, P0[1] - py * lwy};
+    const float p10[2] = {P1[0] + px * lwx, P1[1] + py * lwy}, p11[2] = {P1[0] - px * lwx, P1[1] - py * lwy};
+    TriSetup r;
+    put_edge(r, 0, p00, p01);
+    put_edge(r, 1, p11, p10);
+    put_edge(r, 2, p10, p00);
+    LineExt L;
+    L.ax3 = p01[0];
+    L.ay3 = p01[1];
+    L.dy3 = p11[1] - p01[1];
+    L.dx3 = p11[0] - p01[0];
+    L.p0x = P0[0];
+    L.p0y = P0[1];
+    L.ddx = vx;
+    L.ddy = vy;
+    const float len = sqrtf(vx * vx + vy * vy);   // glm::length(p1.xy - p0.xy)
+    r.area = len * len;
+    // ProcessLines loops while (float)x < viewport.width: ceil(width) columns
+    const TargetParams& t = a.targets[d.target];
+    int32_t x0 = 0, y0 = 0;
+    int32_t x1 = d.vp_w >= (float)t.width ? (int32_t)t.width : (int32_t)ceilf(d.vp_w);
+    int32_t y1 = d.vp_h >= (float)t.height ? (int32_t)t.height : (int32_t)ceilf(d.vp_h);
+    const float* cs[4] = {p00, p01, p10, p11};
+    line_box(d, cs, &x0, &y0, &x1, &y1);
+    if (x0 >= x1 || y0 >= y1) { a.counts[gp] = 0; return; }
+    r.x0 = x0; r.y0 = y0; r.x1 = x1; r.y1 = y1;
+    r.dt_lo = -__builtin_inff();
+    r.dt_hi = __builtin_inff();
+    r.draw = di;
+    r.z[0] = P0[2]; r.z[1] = P1[2]; r.z[2] = 0.0f;
+    r.w[0] = P0[3]; r.w[1] = P1[3]; r.w[2] = P1[3];
+    r.v[0] = s0; r.v[1] = s1; r.v[2] = s1;
+    r.flags = kPrimFront | kPrimLine;
+    r.provoking = s0;
+    r.pad = 0;
+    store_record(&a.recs[gp], r);
+    a.ext[gp] = L;
+    a.counts[gp] = tiles_of(x0, y0, x1, y1, a.tile_shift);
+}
+
 // Per-primitive part of ProcessTriangles (CommandBuffer.Draw.cpp:1526-1569)
 // and the per-triangle constants of GetFragmentInput (:879-898, :900 cull).
 __global__ __launch_bounds__(256) void setup_kernel(SetupArgs a)
@@ -86,6 +266,8 @@ __global__ __launch_bounds__(256) void setup_kernel(SetupArgs a)
     const uint32_t p = (blockIdx.x - d.sblock_base) * 256u + threadIdx.x;
     if (p >= d.prim_count) return;
     const uint32_t gp = d.prim_base + p;
+    if (d.topology == 0) { setup_point(a, d, di, p, gp); return; }
+    if (d.topology <= 2) { setup_line(a, d, di, p, gp); return; }
 
     // CalculatePrimitives (:567-673)
     uint32_t raw[3], prov;

@@ -53,14 +53,15 @@ enum : uint32_t {
 // it never reallocates.
 struct Buffers {
     DevBuf vout, recs, counts, offsets, keys[2], vals[2], tile_start, hist, hist_base, sums, big;
-    DevBuf arena, stats, traces, features, pred, okeys[2], ovals[2];
+    DevBuf arena, stats, traces, features, pred, okeys[2], ovals[2], line_ext;
     void release_all();
 };
 
 struct Program {
     KernelRef vertex;
-    KernelRef raster[7];
+    KernelRef raster[cpv::kRasterVariants];
     int vs_outputs = 0;
+    int psize = -1;       // vertex output component holding gl_PointSize, -1: none
     int fs_inputs = 0;
 };
 

@@ -61,6 +61,8 @@ struct LaunchPlan {
     std::vector<uint32_t> vblock, sblock;
     uint32_t vout_total = 0, prim_total = 0, tiles = 0, stride = 0;
     int tile_shift = 4;
+    int raster_variant = 3;          // index into Program::raster
+    bool lines = false;              // some draw rasterises lines (LineExt records)
     int32_t hiz_op = -1, hiz_kind = 32;
     size_t o_env = 0, o_dd = 0, o_fs = 0, o_tp = 0, o_vb = 0, o_sb = 0;   // offsets in the arena
 };
@@ -86,8 +88,11 @@ int plan_launch(const cpv_config& cfg, const Program& prog, const std::vector<Ta
         if (rc) return rc;
         rc = build_fragment_state(p, passes[draw_target[i]], &P->fstates[i]);
         if (rc) return rc;
-        if (p.topology < CPV_TOPOLOGY_TRIANGLE_LIST || p.topology > CPV_TOPOLOGY_TRIANGLE_FAN)
-            return fail(CPV_ERROR_UNSUPPORTED, "topology %u (lines and points are not on this path)", p.topology);
+        if (p.topology > CPV_TOPOLOGY_TRIANGLE_FAN)
+            return fail(CPV_ERROR_UNSUPPORTED, "topology %u (adjacency and patch lists)", p.topology);
+        if (p.topology < CPV_TOPOLOGY_TRIANGLE_LIST && (cfg.tile_size != 16 || cfg.waves_per_tile != 4))
+            return fail(CPV_ERROR_INVALID, "points and lines raster with 16x16 tiles and 4 waves");
+        if (p.topology != CPV_TOPOLOGY_POINT_LIST && p.topology < CPV_TOPOLOGY_TRIANGLE_LIST) P->lines = true;
         if (d.indexed && d.index_size != 1 && d.index_size != 2 && d.index_size != 4)
             return fail(CPV_ERROR_INVALID, "index size %u", d.index_size);
         if (d.indexed && p.primitive_restart) return fail(CPV_ERROR_UNSUPPORTED, "primitive restart");
@@ -117,6 +122,8 @@ int plan_launch(const cpv_config& cfg, const Program& prog, const std::vector<Ta
         o.vp_max_depth = d.viewport[5];
         o.half_x = 1.0f / d.viewport[2] * 0.5f;
         o.half_y = 1.0f / d.viewport[3] * 0.5f;
+        o.line_width = p.line_width;
+        o.psize = prog.psize;
         const float vw = d.viewport[2], vh = d.viewport[3];
         const int32_t lx = (vw >= -2147483648.0f && vw < 2147483648.0f) ? (int32_t)vw : INT32_MIN;
         const int32_t ly = (vh >= -2147483648.0f && vh < 2147483648.0f) ? (int32_t)vh : INT32_MIN;
@@ -158,6 +165,9 @@ int plan_launch(const cpv_config& cfg, const Program& prog, const std::vector<Ta
     if (prim_total >= (1u << 28)) return fail(CPV_ERROR_UNSUPPORTED, "more than 2^28 primitives in one launch");
     P->vout_total = vout_total;
     P->prim_total = prim_total;
+    bool any_prim = false;
+    for (const DrawDesc& o : P->dd) any_prim = any_prim || o.topology < CPV_TOPOLOGY_TRIANGLE_LIST;
+    P->raster_variant = any_prim ? kRasterAnyPrim : tile_config_index(cfg.tile_size, cfg.waves_per_tile);
     uint32_t tiles = 0;
     for (const TargetParams& t : targets) tiles = std::max(tiles, t.tile_base + t.tiles_x * t.tiles_y);
     P->tiles = tiles;
@@ -291,6 +301,7 @@ int ensure_front(Exec& x, const LaunchPlan& P)
         b.counts.ensure((size_t)std::max(P.prim_total, 1u) * 4, f) || b.offsets.ensure(((size_t)P.prim_total + 1) * 4, f) ||
         b.stats.ensure(kStatAlloc * 4, f))
         return CPV_ERROR_OUT_OF_MEMORY;
+    if (P.lines && b.line_ext.ensure((size_t)std::max(P.prim_total, 1u) * sizeof(LineExt), f)) return CPV_ERROR_OUT_OF_MEMORY;
     return CPV_OK;
 }
 
@@ -329,7 +340,7 @@ int enqueue_front(Exec& x, const LaunchPlan& P, uint8_t* arena)
     }
     if (P.prim_total > 0) {
         SetupArgs sa{d_dd, (const uint32_t*)(arena + P.o_sb), d_tp, b.vout.as<float>(), b.recs.as<TriSetup>(),
-                     b.counts.as<uint32_t>(), P.stride, P.tile_shift};
+                     P.lines ? b.line_ext.as<LineExt>() : nullptr, b.counts.as<uint32_t>(), P.stride, P.tile_shift};
         hipLaunchKernelGGL(setup_kernel, dim3((uint32_t)P.sblock.size()), dim3(256), 0, x.s, sa);
         HIP_TRY(hipGetLastError());
     }
@@ -395,8 +406,7 @@ int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap
     }
 
     // raster
-    const int ci = tile_config_index(x.cfg.tile_size, x.cfg.waves_per_tile);
-    const KernelRef& rk = P.prog.raster[ci];
+    const KernelRef& rk = P.prog.raster[P.raster_variant];
     const int block = 64 * (int)x.cfg.waves_per_tile;
     RasterArgs ra{};
     ra.recs = b.recs.as<TriSetup>();
@@ -406,6 +416,7 @@ int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap
     ra.draws = d_dd;
     ra.fstates = d_fs;
     ra.vout = b.vout.as<float>();
+    ra.line_ext = P.lines ? b.line_ext.as<LineExt>() : nullptr;
     ra.order = order;
     ra.queue = stats + kStatQueues;
     ra.fragments = (unsigned long long*)(stats + kStatFragments);
@@ -455,12 +466,12 @@ Exec context_exec(cpv_context* c, Buffers* b)
 
 // One immediate launch on the context: plan, upload, front half, pair-count
 // readback, back half.
-int run_launch(cpv_context* c, const Program& prog, const std::vector<TargetParams>& targets,
+int run_launch(cpv_context* c, const cpv_config& cfg, const Program& prog, const std::vector<TargetParams>& targets,
                const std::vector<LaunchDraw>& lds, const std::vector<cpv_pass_desc>& passes,
                const std::vector<uint32_t>& draw_target)
 {
     LaunchPlan P;
-    int rc = plan_launch(c->cfg, prog, targets, lds, passes, draw_target, &P);
+    int rc = plan_launch(cfg, prog, targets, lds, passes, draw_target, &P);
     if (rc) return rc;
     Staging st;
     stage_plan(P, st);
@@ -470,6 +481,7 @@ int run_launch(cpv_context* c, const Program& prog, const std::vector<TargetPara
     rc = upload_staging(c, c->buf.arena, st);
     if (rc) return rc;
     Exec x = context_exec(c, &c->buf);
+    x.cfg = cfg;
     rc = enqueue_front(x, P, arena);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(c->readback, c->buf.offsets.as<uint32_t>() + P.prim_total, 4, hipMemcpyDeviceToHost, c->stream));
@@ -480,12 +492,32 @@ int run_launch(cpv_context* c, const Program& prog, const std::vector<TargetPara
     rc = enqueue_back(x, P, arena, pairs);
     if (rc) return rc;
     c->last_tiles = P.tiles;
-    c->last_waves = c->cfg.waves_per_tile;
-    c->last_tile_size = c->cfg.tile_size;
-    c->last_traces = c->cfg.record_costs != 0;
+    c->last_waves = cfg.waves_per_tile;
+    c->last_tile_size = cfg.tile_size;
+    c->last_traces = cfg.record_costs != 0;
     c->last_target_base.clear();
     for (const TargetParams& t : P.targets) c->last_target_base.push_back(t.tile_base);
     return CPV_OK;
+}
+
+// Points and lines raster in the 16x16 tile, four-wave kernel variant that
+// carries them: a call with any such draw runs every launch in that
+// configuration.
+cpv_config call_config(const cpv_config& cfg, bool points_or_lines)
+{
+    cpv_config e = cfg;
+    if (points_or_lines) {
+        e.tile_size = 16;
+        e.waves_per_tile = 4;
+    }
+    return e;
+}
+
+bool jobs_have_points_or_lines(const cpv_job_desc* jobs, uint32_t n)
+{
+    for (uint32_t j = 0; j < n; j++)
+        if (jobs[j].pipeline.topology < CPV_TOPOLOGY_TRIANGLE_LIST) return true;
+    return false;
 }
 
 bool same_program(const cpv_pipeline_desc& a, const cpv_pipeline_desc& b)
@@ -549,7 +581,11 @@ extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const 
     if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
     fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
     HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
-    const int tile = (int)c->cfg.tile_size;
+    bool any_prim = false;
+    for (uint32_t k = 0; k < draw_count; k++)
+        any_prim = any_prim || pipelines[draw_pipeline ? draw_pipeline[k] : 0].topology < CPV_TOPOLOGY_TRIANGLE_LIST;
+    const cpv_config cfg = call_config(c->cfg, any_prim);
+    const int tile = (int)cfg.tile_size;
     std::vector<cpv_pass_desc> passes(1, *pass);
     bool first = true;
     uint32_t i = 0;
@@ -580,7 +616,7 @@ extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const 
         std::vector<TargetParams> targets(1);
         int rc = build_target(*pass, first, tile, 0, &targets[0]);
         if (rc) return rc;
-        rc = run_launch(c, prog, targets, lds, passes, dt);
+        rc = run_launch(c, cfg, prog, targets, lds, passes, dt);
         if (rc) return rc;
         first = false;
     } while (i < draw_count);
@@ -598,11 +634,12 @@ extern "C" int cpv_render_jobs(cpv_context* c, const cpv_job_desc* jobs, uint32_
     if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
     fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
     HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+    const cpv_config cfg = call_config(c->cfg, jobs_have_points_or_lines(jobs, job_count));
     std::vector<JobGroup> groups;
-    int rc = group_jobs(jobs, job_count, (int)c->cfg.tile_size, &groups);
+    int rc = group_jobs(jobs, job_count, (int)cfg.tile_size, &groups);
     if (rc) return rc;
     for (JobGroup& G : groups) {
-        rc = run_launch(c, G.prog, G.targets, G.lds, G.passes, G.dt);
+        rc = run_launch(c, cfg, G.prog, G.targets, G.lds, G.passes, G.dt);
         if (rc) return rc;
     }
     HIP_TRY(hipEventRecord(c->ev_end, c->stream));
@@ -722,8 +759,8 @@ static int batch_build(cpv_context* c, cpv_batch* B, const cpv_job_desc* jobs, u
         B->caps[g] = pairs + pairs / 8 + 4096;   // room for vertex data that moves a little between submits
         rc = ensure_back(x, B->plans[g], B->caps[g]);
         if (rc) return rc;
-        const int ci = tile_config_index(B->cfg.tile_size, B->cfg.waves_per_tile);
-        if (resident_workgroups(x, B->plans[g].prog.raster[ci], 64 * (int)B->cfg.waves_per_tile) <= 0) return CPV_ERROR_DEVICE;
+        const LaunchPlan& P = B->plans[g];
+        if (resident_workgroups(x, P.prog.raster[P.raster_variant], 64 * (int)B->cfg.waves_per_tile) <= 0) return CPV_ERROR_DEVICE;
     }
     if (B->b.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -756,8 +793,7 @@ extern "C" int cpv_batch_create(cpv_context* c, const cpv_job_desc* jobs, uint32
     HIP_TRY(hipSetDevice(c->device));
     cpv_batch* B = new cpv_batch();
     B->c = c;
-    B->cfg = c->cfg;
-    B->cfg.count_fragments = c->cfg.count_fragments;
+    B->cfg = call_config(c->cfg, jobs_have_points_or_lines(jobs, job_count));
     int rc = batch_build(c, B, jobs, job_count);
     if (rc) {
         const std::string msg = cpv_last_error();

@@ -159,7 +159,7 @@ void Buffers::release_all()
 {
     DevBuf* bufs[] = {&vout, &recs, &counts, &offsets, &keys[0], &keys[1], &vals[0], &vals[1], &tile_start, &hist,
                       &hist_base, &sums, &big, &arena, &stats, &traces, &features, &pred, &okeys[0], &okeys[1],
-                      &ovals[0], &ovals[1]};
+                      &ovals[0], &ovals[1], &line_ext};
     for (DevBuf* b : bufs) b->release();
 }
 
@@ -345,11 +345,12 @@ template <class FS>
 struct RasterTable {
     static const void* get(int i)
     {
-        static const void* fns[7] = {
+        static const void* fns[kRasterVariants] = {
             (const void*)&raster_kernel<FS, 8, 1>,  (const void*)&raster_kernel<FS, 16, 1>,
             (const void*)&raster_kernel<FS, 16, 2>, (const void*)&raster_kernel<FS, 16, 4>,
             (const void*)&raster_kernel<FS, 32, 1>, (const void*)&raster_kernel<FS, 32, 2>,
-            (const void*)&raster_kernel<FS, 32, 4>};
+            (const void*)&raster_kernel<FS, 32, 4>, (const void*)&raster_kernel<FS, 16, 4, true>};
+        static_assert(kRasterAnyPrim == 7, "the point/line variant is the last entry");
         return fns[i];
     }
 };
@@ -361,20 +362,33 @@ bool builtin_program(uint32_t vs, uint32_t fs, Program* p)
     case CPV_VS_PASSTHROUGH: p->vertex.host_fn = (const void*)&vertex_kernel<VsPassthrough>; p->vs_outputs = VsPassthrough::kOutputs; break;
     case CPV_VS_MVP_COLOR: p->vertex.host_fn = (const void*)&vertex_kernel<VsMvpColor>; p->vs_outputs = VsMvpColor::kOutputs; break;
     case CPV_VS_MVP_LAMBERT: p->vertex.host_fn = (const void*)&vertex_kernel<VsMvpLambert>; p->vs_outputs = VsMvpLambert::kOutputs; break;
+    case CPV_VS_PASSTHROUGH_POINT:
+        p->vertex.host_fn = (const void*)&vertex_kernel<VsPassthroughPoint>;
+        p->vs_outputs = VsPassthroughPoint::kOutputs;
+        p->psize = VsPassthroughPoint::kPointSize;
+        break;
     default: return false;
     }
     switch (fs) {
     case CPV_FS_COLOR:
-        for (int i = 0; i < 7; i++) p->raster[i].host_fn = RasterTable<FsColor>::get(i);
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsColor>::get(i);
         p->fs_inputs = FsColor::kInputs;
         break;
     case CPV_FS_TEX_LAMBERT:
-        for (int i = 0; i < 7; i++) p->raster[i].host_fn = RasterTable<FsTexLambert>::get(i);
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsTexLambert>::get(i);
         p->fs_inputs = FsTexLambert::kInputs;
         break;
     case CPV_FS_COLOR_DISCARD:
-        for (int i = 0; i < 7; i++) p->raster[i].host_fn = RasterTable<FsColorDiscard>::get(i);
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsColorDiscard>::get(i);
         p->fs_inputs = FsColorDiscard::kInputs;
+        break;
+    case CPV_FS_COLOR_FLAT:
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsColorFlat>::get(i);
+        p->fs_inputs = FsColorFlat::kInputs;
+        break;
+    case CPV_FS_COLOR_NOPERSPECTIVE:
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsColorNoPerspective>::get(i);
+        p->fs_inputs = FsColorNoPerspective::kInputs;
         break;
     default: return false;
     }
@@ -386,8 +400,9 @@ int resolve_program(const cpv_pipeline_desc& pd, Program* p)
     if (pd.program) {
         const cpv_program* prog = pd.program;
         p->vertex = prog->vertex;
-        for (int i = 0; i < 7; i++) p->raster[i] = prog->raster[i];
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i] = prog->raster[i];
         p->vs_outputs = prog->vs_outputs;
+        p->psize = prog->psize;
         p->fs_inputs = prog->fs_inputs;
         return CPV_OK;
     }
@@ -557,8 +572,12 @@ int build_target(const cpv_pass_desc& pass, bool first_segment, int tile, uint32
     return CPV_OK;
 }
 
+// CalculatePrimitives (CommandBuffer.Draw.cpp:567-673).
 uint32_t prim_count_of(uint32_t topology, uint32_t n)
 {
+    if (topology == CPV_TOPOLOGY_POINT_LIST) return n;
+    if (topology == CPV_TOPOLOGY_LINE_LIST) return n / 2;
+    if (topology == CPV_TOPOLOGY_LINE_STRIP) return n > 1 ? n - 1 : 0;
     if (topology == CPV_TOPOLOGY_TRIANGLE_LIST) return n / 3;
     if (topology == CPV_TOPOLOGY_TRIANGLE_STRIP || topology == CPV_TOPOLOGY_TRIANGLE_FAN) return n > 2 ? n - 2 : 0;
     return 0;

@@ -83,6 +83,25 @@ struct TriSetup {
 };
 static_assert(sizeof(TriSetup) == 128, "TriSetup must be 128 bytes");
 
+// TriSetup.flags: bit 0 front facing; the primitive kind.  A point record
+// holds its screen centre (int bits of ax[0], ay[0]), its size (dx[0]), its
+// clamped square [x0, x1) x [y0, y1) and its vertex in v[0]; a line record
+// holds three edges of its quad (the fourth in LineExt), |p1.xy - p0.xy|^2
+// in area, z = (p0.z, p1.z), w = (p0.w, p1.w) and v = (p0, p1).
+enum : uint32_t { kPrimFront = 1u, kPrimPoint = 2u, kPrimLine = 4u };
+
+// Raster kernel variants of a program: the tile configurations 8x1, 16x1,
+// 16x2, 16x4, 32x1, 32x2, 32x4, then 16x4 with point and line records.
+constexpr int kRasterVariants = 8;
+constexpr int kRasterAnyPrim = 7;
+
+// Per-primitive extension of a line record (only lines write it).
+struct LineExt {
+    float ax3, ay3, dy3, dx3;   // fourth edge
+    float p0x, p0y, ddx, ddy;   // p0.xy and p1.xy - p0.xy for t
+};
+static_assert(sizeof(LineExt) == 32, "LineExt must be 32 bytes");
+
 enum { kMaxVaryings = 16 };
 
 // Fixed-function state of one pipeline as the fragment stage needs it.
@@ -137,6 +156,8 @@ struct DrawDesc {
     float vp_w, vp_h, vp_min_depth, vp_max_depth;
     float half_x, half_y;      // (1 / width) * 0.5, (1 / height) * 0.5 (ProcessTriangles :1524)
     int32_t limit_x, limit_y;  // min(viewport extent, attachment extent)
+    float line_width;          // RasterizationState.LineWidth
+    int32_t psize;             // varying component holding gl_PointSize, -1 when the shader writes none
 };
 
 // How the slot of an assembled vertex is found.

@@ -298,12 +298,22 @@ void GetPhysicalDeviceProperties(VkPhysicalDevice, VkPhysicalDeviceProperties* p
     l.maxSampleMaskWords = 1;
     l.timestampPeriod = 1.0f;
     l.discreteQueuePriorities = 2;
-    l.pointSizeRange[0] = l.pointSizeRange[1] = 1.0f;
-    l.lineWidthRange[0] = l.lineWidthRange[1] = 1.0f;
+    // CPVulkanBase/Config.h:192-195: wide lines and large points, granularity 1
+    l.pointSizeRange[0] = 1.0f;
+    l.pointSizeRange[1] = 64.0f - 1.0f / 32.0f;
+    l.lineWidthRange[0] = 1.0f;
+    l.lineWidthRange[1] = 8.0f - 1.0f / 32.0f;
+    l.pointSizeGranularity = l.lineWidthGranularity = 1.0f;
     l.optimalBufferCopyOffsetAlignment = l.optimalBufferCopyRowPitchAlignment = l.nonCoherentAtomSize = 1;
 }
 
-void GetPhysicalDeviceFeatures(VkPhysicalDevice, VkPhysicalDeviceFeatures* f) { std::memset(f, 0, sizeof(*f)); }
+// VkPhysicalDeviceFeatures in declaration order: depthBounds (14), wideLines
+// (15) and largePoints (16) are on, as in PhysicalDevice.cpp:214-216.
+void GetPhysicalDeviceFeatures(VkPhysicalDevice, VkPhysicalDeviceFeatures* f)
+{
+    std::memset(f, 0, sizeof(*f));
+    f->feature[14] = f->feature[15] = f->feature[16] = 1;
+}
 
 void GetPhysicalDeviceMemoryProperties(VkPhysicalDevice, VkPhysicalDeviceMemoryProperties* m)
 {
@@ -708,7 +718,9 @@ VkResult build_pipeline(Device* d, const VkGraphicsPipelineCreateInfo& ci, Pipel
         o.topology = (uint32_t)ci.pInputAssemblyState->topology;
         o.primitive_restart = ci.pInputAssemblyState->primitiveRestartEnable;
     }
+    o.line_width = 1.0f;
     if (const VkPipelineRasterizationStateCreateInfo* r = ci.pRasterizationState) {
+        o.line_width = r->lineWidth;   // RasterizationState.LineWidth (Pipeline.cpp:224); ProcessLines reads no dynamic width
         o.cull_mode = r->cullMode;
         o.front_face = (uint32_t)r->frontFace;
         o.polygon_mode = (uint32_t)r->polygonMode;
@@ -748,8 +760,13 @@ VkResult build_pipeline(Device* d, const VkGraphicsPipelineCreateInfo& ci, Pipel
     }
     o.origin_upper_left = 1;   // Vulkan fragment shaders declare OriginUpperLeft (the translator requires it)
     if (ci.pDynamicState)
-        for (uint32_t i = 0; i < ci.pDynamicState->dynamicStateCount; i++)
-            if (ci.pDynamicState->pDynamicStates[i] == VK_DYNAMIC_STATE_VIEWPORT) p->dynamic_viewport = true;
+        for (uint32_t i = 0; i < ci.pDynamicState->dynamicStateCount; i++) {
+            const int32_t ds = ci.pDynamicState->pDynamicStates[i];
+            if (ds == VK_DYNAMIC_STATE_VIEWPORT) p->dynamic_viewport = true;
+            // a dynamic depth-bounds range with the test on is not compiled
+            // by the reference (PipelineCompiler.cpp:1141-1148)
+            if (ds == VK_DYNAMIC_STATE_DEPTH_BOUNDS && o.depth_bounds_test) return VK_ERROR_FEATURE_NOT_PRESENT;
+        }
     if (!p->dynamic_viewport && ci.pViewportState && ci.pViewportState->viewportCount && ci.pViewportState->pViewports)
         p->viewport = ci.pViewportState->pViewports[0];
     return VK_SUCCESS;

@@ -22,6 +22,19 @@ struct VsPassthrough {
     }
 };
 
+// in vec4 pos @0, vec4 colour @1, float size @2: gl_Position = pos,
+// gl_PointSize = size, the colour passed through (output 4 is the size).
+struct VsPassthroughPoint {
+    static constexpr int kOutputs = 5;
+    static constexpr int kPointSize = 4;
+    __device__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)
+    {
+        fetch_attribute(env, 0, 4, vid, iid, pos);
+        fetch_attribute(env, 1, 4, vid, iid, out);
+        fetch_attribute(env, 2, 1, vid, iid, out + 4);
+    }
+};
+
 // Samples/15-draw_cube/15-draw_cube.cpp:38-51: gl_Position = mvp * pos.
 struct VsMvpColor {
     static constexpr int kOutputs = 4;
@@ -60,6 +73,22 @@ struct FsColor {
         return false;
     }
 };
+
+// out = color with the input Flat (mode 2) or NoPerspective (mode 1).
+template <int MODE>
+struct FsColorInterp {
+    static constexpr int kInputs = 4;
+    static constexpr bool kMayDiscard = false;
+    static constexpr int kWavesPerEU = 6;
+    __device__ static int interp(int) { return MODE; }
+    __device__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
+    {
+        out[0] = in[0]; out[1] = in[1]; out[2] = in[2]; out[3] = in[3];
+        return false;
+    }
+};
+using FsColorFlat = FsColorInterp<2>;
+using FsColorNoPerspective = FsColorInterp<1>;
 
 // if (color.r < 0.25) discard; out = color.
 struct FsColorDiscard {

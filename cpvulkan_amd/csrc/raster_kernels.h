@@ -67,6 +67,7 @@ struct RasterArgs {
     const DrawDesc* draws;
     const FragmentState* fstates;
     const float* vout;
+    const LineExt* line_ext;        // per-primitive line extension (launches with lines)
     const uint32_t* order;          // queue order of tiles (nullptr: index order)
     uint32_t* queue;                // kRasterQueues work counters, kRasterQueueStride words apart
     unsigned long long* fragments;  // covered (primitive, pixel) pairs
@@ -122,6 +123,34 @@ __device__ __forceinline__ float interpolate(int mode, const float w[3], const f
     return num / den;
 }
 
+// SetDatum<..., 2> of a line (CommandBuffer.Draw.cpp:1450-1485): two terms.
+__device__ __forceinline__ float interpolate_line(int mode, const float w[3], const float pw[3], float v0, float v1, float flat)
+{
+    if (mode == 2) return flat;
+    if (mode == 1) {
+        float r = 0.0f;
+        r += w[0] * v0;
+        r += w[1] * v1;
+        return r;
+    }
+    float num = 0.0f, den = 0.0f;
+    num += w[0] * v0 / pw[0];
+    den += w[0] / pw[0];
+    num += w[1] * v1 / pw[1];
+    den += w[1] / pw[1];
+    return num / den;
+}
+
+__device__ __forceinline__ LineExt load_line_ext(const LineExt* ext, uint32_t gprim)
+{
+    const uint4* p = (const uint4*)(ext + gprim);
+    const uint4 a = p[0], b = p[1];
+    LineExt L;
+    L.ax3 = __uint_as_float(a.x); L.ay3 = __uint_as_float(a.y); L.dy3 = __uint_as_float(a.z); L.dx3 = __uint_as_float(a.w);
+    L.p0x = __uint_as_float(b.x); L.p0y = __uint_as_float(b.y); L.ddx = __uint_as_float(b.z); L.ddy = __uint_as_float(b.w);
+    return L;
+}
+
 // Perspective SetDatum with all three clip w equal to 1: (w*v)/1 and w/1 are
 // exact, so only the final division remains.
 __device__ __forceinline__ float interpolate_unit_w(const float w[3], float v0, float v1, float v2)
@@ -141,22 +170,30 @@ __device__ __forceinline__ float interpolate_unit_w(const float w[3], float v0, 
 // blend (PipelineCompiler.cpp:1415-1727).  The raster kernel calls it from
 // one place (the colour queue's drain), so it is inlined: an out-of-line call
 // would pass the pixel state through scratch memory.
-template <class FS>
-__device__ __forceinline__ PixelState shade_fragment(const TriSetup* recs, const DrawDesc* draws, const FragmentState* fstates,
-                                                  const float* vout, uint32_t stride, uint32_t gprim, float w0, float w1,
-                                                  float w2, int x, int y, PixelState ps)
+// Fragment-shader inputs of one covered pixel of primitive g with weights
+// (w0, w1, w2): SetDatum over the three vertices of a triangle, over the two
+// of a line, and a point's vertex outputs as they are (ProcessPoints copies
+// them, :1364-1368).
+template <class FS, bool ANYPRIM, class Rec>
+__device__ __forceinline__ void fragment_inputs(const Rec& g, const float* vout, uint32_t stride, float w0, float w1, float w2,
+                                                float* in)
 {
-    const TriSetup& g = recs[gprim];
-    const DrawDesc& d = draws[g.draw];
-    const FragmentState& fs = fstates[d.fstate];
-    const DrawEnv& de = *(const DrawEnv*)d.env;
     const float w[3] = {w0, w1, w2};
     const float pw[3] = {g.w[0], g.w[1], g.w[2]};
     const float* q0 = vout + (uint64_t)g.v[0] * stride + 4;
     const float* q1 = vout + (uint64_t)g.v[1] * stride + 4;
     const float* q2 = vout + (uint64_t)g.v[2] * stride + 4;
     const float* qp = vout + (uint64_t)g.provoking * stride + 4;
-    float in[FS::kInputs > 0 ? FS::kInputs : 1];
+    if (ANYPRIM && (g.flags & kPrimPoint)) {
+#pragma unroll
+        for (int c = 0; c < FS::kInputs; c++) in[c] = q0[c];
+        return;
+    }
+    if (ANYPRIM && (g.flags & kPrimLine)) {
+#pragma unroll
+        for (int c = 0; c < FS::kInputs; c++) in[c] = interpolate_line(FS::interp(c), w, pw, q0[c], q1[c], qp[c]);
+        return;
+    }
     // x / 1.0f == x exactly, so clip w == 1 drops the divisions
     const bool unit_w = pw[0] == 1.0f && pw[1] == 1.0f && pw[2] == 1.0f;
 #pragma unroll
@@ -165,7 +202,24 @@ __device__ __forceinline__ PixelState shade_fragment(const TriSetup* recs, const
         in[c] = (mode == 0 && unit_w) ? interpolate_unit_w(w, q0[c], q1[c], q2[c])
                                       : interpolate(mode, w, pw, q0[c], q1[c], q2[c], qp[c]);
     }
-    const float depth = g.z[0] * w0 + g.z[1] * w1 + g.z[2] * w2;
+}
+
+template <class FS, bool ANYPRIM = false>
+__device__ __forceinline__ PixelState shade_fragment(const TriSetup* recs, const DrawDesc* draws, const FragmentState* fstates,
+                                                  const float* vout, uint32_t stride, uint32_t gprim, float w0, float w1,
+                                                  float w2, int x, int y, PixelState ps)
+{
+    const TriSetup& g = recs[gprim];
+    const DrawDesc& d = draws[g.draw];
+    const FragmentState& fs = fstates[d.fstate];
+    const DrawEnv& de = *(const DrawEnv*)d.env;
+    float in[FS::kInputs > 0 ? FS::kInputs : 1];
+    fragment_inputs<FS, ANYPRIM>(g, vout, stride, w0, w1, w2, in);
+    // a point's depth is its vertex's; a line's is p0.z * t + p1.z * (1 - t)
+    // with t = w1 (ProcessLines :1494)
+    const float depth = (ANYPRIM && (g.flags & kPrimPoint)) ? g.z[0]
+                        : (ANYPRIM && (g.flags & kPrimLine)) ? g.z[0] * w1 + g.z[1] * (1.0f - w1)
+                                                               : g.z[0] * w0 + g.z[1] * w1 + g.z[2] * w2;
     const float fc[4] = {fs.origin_upper_left ? (float)x : d.vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
     float colour[4];
     FS::run(de, in, fc, (g.flags & 1u) != 0, colour);

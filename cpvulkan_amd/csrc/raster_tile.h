@@ -89,7 +89,9 @@ constexpr int raster_waves_per_eu()
     return (TILE / 8) * (TILE / 8) / WAVES == 1 ? RasterWavesPerEU<FS>::value : 1;
 }
 
-template <class FS, int TILE, int WAVES>
+// ANYPRIM: the launch may hold point and line records besides triangles
+// (one 16x16 / four-wave instantiation per fragment shader carries it).
+template <class FS, int TILE, int WAVES, bool ANYPRIM = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(raster_waves_per_eu<FS, TILE, WAVES>()))) void
 raster_kernel(RasterArgs a)
 {
@@ -216,7 +218,7 @@ raster_kernel(RasterArgs a)
 #pragma unroll
             for (int kk = 1; kk < BPW; kk++)
                 if (kk == k) { ps = px[kk]; sx = pxx[kk]; sy = pxy[kk]; }
-            ps = shade_fragment<FS>(a.recs, a.draws, a.fstates, a.vout, a.vout_stride, gprim, w0, w1, w2, sx, sy, ps);
+            ps = shade_fragment<FS, ANYPRIM>(a.recs, a.draws, a.fstates, a.vout, a.vout_stride, gprim, w0, w1, w2, sx, sy, ps);
 #pragma unroll
             for (int kk = 0; kk < BPW; kk++)
                 if (kk == k) px[kk] = ps;
@@ -315,34 +317,53 @@ raster_kernel(RasterArgs a)
                         const float cd = stored_depth(a.hiz_depth_kind, px[k].ds0);
                         if (hiz_rejects(a.hiz_op, r.dt_lo, r.dt_hi, cd, cd)) continue;
                     }
-                    // GetFragmentInput (:874-903); NaN edge values are not < 0 and pass
-                    const float e0 = (xf[k] - r.ax[0]) * r.dy[0] - (yf[k] - r.ay[0]) * r.dx[0];
-                    const float e1 = (xf[k] - r.ax[1]) * r.dy[1] - (yf[k] - r.ay[1]) * r.dx[1];
-                    const float e2 = (xf[k] - r.ax[2]) * r.dy[2] - (yf[k] - r.ay[2]) * r.dx[2];
-                    if (e0 < 0 || e1 < 0 || e2 < 0) continue;
-                    nfrag++;
-                    if (early) {
-                        const float cur = depth_kind == 32 ? __uint_as_float(px[k].ds0) : unpack_depth(fst->ds, px[k].ds0);
-                        if (vp_identity ? depth_surely_fails_unit(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2])
-                                        : depth_surely_fails(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2], vp_min, vp_max))
-                            continue;
+                    float w0, w1, w2, depth;
+                    if (ANYPRIM && (r.flags & kPrimPoint)) {
+                        // ProcessPoints (:1358-1378): s, t of the pixel in the
+                        // point's square, the vertex's outputs and depth as they are
+                        const float size = r.dx[0];
+                        const float s = 0.5f + (float)(int32_t)((uint32_t)x - (uint32_t)__float_as_int(r.ax[0])) / size;
+                        const float t = 0.5f + (float)(int32_t)((uint32_t)y - (uint32_t)__float_as_int(r.ay[0])) / size;
+                        if (!(s >= 0.0f && t >= 0.0f && s <= 1.0f && t <= 1.0f)) continue;
+                        nfrag++;
+                        w0 = 1.0f; w1 = 0.0f; w2 = 0.0f;
+                        depth = r.z[0];
+                    } else if (ANYPRIM && (r.flags & kPrimLine)) {
+                        // ProcessLines (:1420-1497): all four quad edges >= 0
+                        // (NaN fails), t = dot(p - p0, p1 - p0) / |p1 - p0|^2
+                        const LineExt L = load_line_ext(a.line_ext, gprim);
+                        const float e0 = (xf[k] - r.ax[0]) * r.dy[0] - (yf[k] - r.ay[0]) * r.dx[0];
+                        const float e1 = (xf[k] - r.ax[1]) * r.dy[1] - (yf[k] - r.ay[1]) * r.dx[1];
+                        const float e2 = (xf[k] - r.ax[2]) * r.dy[2] - (yf[k] - r.ay[2]) * r.dx[2];
+                        const float e3 = (xf[k] - L.ax3) * L.dy3 - (yf[k] - L.ay3) * L.dx3;
+                        if (!(e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f && e3 >= 0.0f)) continue;
+                        nfrag++;
+                        const float t = ((xf[k] - L.p0x) * L.ddx + (yf[k] - L.p0y) * L.ddy) / r.area;
+                        w0 = 1.0f - t; w1 = t; w2 = 0.0f;
+                        depth = r.z[0] * t + r.z[1] * (1.0f - t);
+                    } else {
+                        // GetFragmentInput (:874-903); NaN edge values are not < 0 and pass
+                        const float e0 = (xf[k] - r.ax[0]) * r.dy[0] - (yf[k] - r.ay[0]) * r.dx[0];
+                        const float e1 = (xf[k] - r.ax[1]) * r.dy[1] - (yf[k] - r.ay[1]) * r.dx[1];
+                        const float e2 = (xf[k] - r.ax[2]) * r.dy[2] - (yf[k] - r.ay[2]) * r.dx[2];
+                        if (e0 < 0 || e1 < 0 || e2 < 0) continue;
+                        nfrag++;
+                        if (early) {
+                            const float cur = depth_kind == 32 ? __uint_as_float(px[k].ds0) : unpack_depth(fst->ds, px[k].ds0);
+                            if (vp_identity ? depth_surely_fails_unit(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2])
+                                            : depth_surely_fails(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2], vp_min, vp_max))
+                                continue;
+                        }
+                        w0 = e0 / r.area; w1 = e1 / r.area; w2 = e2 / r.area;
+                        depth = r.z[0] * w0 + r.z[1] * w1 + r.z[2] * w2;
                     }
-                    const float w0 = e0 / r.area, w1 = e1 / r.area, w2 = e2 / r.area;
-                    const float depth = r.z[0] * w0 + r.z[1] * w1 + r.z[2] * w2;
                     // DrawPixel (:1300-1313)
                     const float dt = (vp_max - vp_min) * depth + vp_min;
                     const bool front = (r.flags & 1u) != 0;
                     if (FS::kMayDiscard) {
                         // the shader decides first whether anything is written
-                        const float wv[3] = {w0, w1, w2};
-                        const float pw[3] = {r.w[0], r.w[1], r.w[2]};
-                        const float* q0 = a.vout + (uint64_t)r.v[0] * a.vout_stride + 4;
-                        const float* q1 = a.vout + (uint64_t)r.v[1] * a.vout_stride + 4;
-                        const float* q2 = a.vout + (uint64_t)r.v[2] * a.vout_stride + 4;
-                        const float* qp = a.vout + (uint64_t)r.provoking * a.vout_stride + 4;
                         float in[FS::kInputs > 0 ? FS::kInputs : 1];
-#pragma unroll
-                        for (int c = 0; c < FS::kInputs; c++) in[c] = interpolate(FS::interp(c), wv, pw, q0[c], q1[c], q2[c], qp[c]);
+                        fragment_inputs<FS, ANYPRIM>(r, a.vout, a.vout_stride, w0, w1, w2, in);
                         const float fc[4] = {fst->origin_upper_left ? (float)x : a.draws[cur_draw].vp_w - (float)x - 1.0f, (float)y,
                                              depth, 1.0f};
                         float colour[4];

@@ -48,6 +48,7 @@
 
 #include "../../include/cpv.h"
 #include "cpv_program.h"
+#include "cpv_types.h"
 
 extern const char* const cpv_rtc_header_names[];
 extern const char* const cpv_rtc_header_sources[];
@@ -296,6 +297,8 @@ struct Gen {
     std::map<uint32_t, UPtr> uptr;           // pointers into uniform buffers
     std::set<uint32_t> structs_done;
     int out_total = 0;                       // vertex: varying components written
+    int psize = -1;                          // vertex: output component holding gl_PointSize
+    std::string psize_expr;                  // vertex: the variable written as gl_PointSize
     std::map<int, int> loc_offset;           // location -> first varying component
     std::map<int, int> loc_size;
     bool discard = false;
@@ -473,6 +476,21 @@ struct Gen {
         }
     }
 
+    // True when some function stores to `var` (member < 0) or takes an
+    // access chain to its member `member`: the shader writes that output.
+    bool writes(uint32_t var, int member) const
+    {
+        for (const auto& f : m.functions)
+            for (const Inst& in : f.second.body) {
+                if (member < 0 && in.op == OpStore && !in.w.empty() && in.w[0] == var) return true;
+                if (member >= 0 && in.op == OpAccessChain && in.w.size() >= 4 && in.w[2] == var) {
+                    auto c = m.consts.find(in.w[3]);
+                    if (c != m.consts.end() && !c->second.second.empty() && c->second.second[0] == (uint32_t)member) return true;
+                }
+            }
+        return false;
+    }
+
     MemberDec member_dec(uint32_t st, uint32_t k) const
     {
         auto d = m.dec.find(st);
@@ -506,7 +524,8 @@ struct Gen {
                         decl << "    " << ctype(pt.members[k]) << " " << mn << "{};\n";
                         if (bi == BiPosition) epi << "    pos[0] = " << mn << ".c[0]; pos[1] = " << mn << ".c[1]; pos[2] = " << mn
                                                   << ".c[2]; pos[3] = " << mn << ".c[3];\n";
-                        else if (bi != BiPointSize && bi != BiClipDistance && bi != BiCullDistance)
+                        else if (bi == BiPointSize) { if (writes(var, (int)k)) psize_expr = mn; }
+                        else if (bi != BiClipDistance && bi != BiCullDistance)
                             bad("unsupported gl_PerVertex member");
                     }
                     lval[var] = name;
@@ -520,7 +539,7 @@ struct Gen {
                     else if (vertex && sc == ScOutput && d.builtin == BiPosition)
                         epi << "    pos[0] = " << name << ".c[0]; pos[1] = " << name << ".c[1]; pos[2] = " << name << ".c[2]; pos[3] = "
                             << name << ".c[3];\n";
-                    else if (vertex && sc == ScOutput && d.builtin == BiPointSize) { /* points are not rasterised */ }
+                    else if (vertex && sc == ScOutput && d.builtin == BiPointSize) { if (writes(var, -1)) psize_expr = name; }
                     else if (!vertex && sc == ScInput && d.builtin == BiFragCoord)
                         pro << "    " << name << " = V<float, 4>{{fc[0], fc[1], fc[2], fc[3]}};\n";
                     else if (!vertex && sc == ScInput && d.builtin == BiFrontFacing) pro << "    " << name << " = front;\n";
@@ -592,6 +611,12 @@ struct Gen {
                 if (n != 4) bad("the fragment colour output must have four components");
                 for (int k = 0; k < 4; k++) epi << "    out[" << k << "] = " << store(k) << ";\n";
             }
+        }
+        // gl_PointSize travels after the varyings (the size ProcessPoints
+        // reads from the vertex's built-in outputs, CommandBuffer.Draw.cpp:1350)
+        if (vertex && !psize_expr.empty()) {
+            epi << "    out[" << off << "] = " << psize_expr << ";\n";
+            psize = off++;
         }
         out_total = off;
     }
@@ -1516,9 +1541,17 @@ __device__ __forceinline__ V<float, 2> spv_unpack_half2x16(uint32_t p)
 }
 )";
 
+// Template arguments of the raster kernel variants in Program::raster order:
+// the seven tile configurations, then the 16x16 / four-wave kernel that
+// also rasterises point and line records.
+std::vector<std::string> raster_instances()
+{
+    return {"8, 1", "16, 1", "16, 2", "16, 4", "32, 1", "32, 2", "32, 4", "16, 4, true"};
+}
+
 struct Translation {
     std::string source;
-    int vs_outputs = 0, fs_inputs = 0;
+    int vs_outputs = 0, fs_inputs = 0, psize = -1;
 };
 
 Translation translate(const uint32_t* vs, size_t vs_n, const char* vs_entry, const uint32_t* fs, size_t fs_n, const char* fs_entry)
@@ -1552,22 +1585,20 @@ Translation translate(const uint32_t* vs, size_t vs_n, const char* vs_entry, con
       << "    (void)env; (void)in; (void)fc; (void)front;\n        return false;\n    }\n};\n";
     s << "}  // namespace cpv\n";
     s << "template __global__ void cpv::vertex_kernel<cpv::SpvVs>(cpv::VertexArgs);\n";
-    static const int cfg[7][2] = {{8, 1}, {16, 1}, {16, 2}, {16, 4}, {32, 1}, {32, 2}, {32, 4}};
-    for (const auto& c : cfg)
-        s << "template __global__ void cpv::raster_kernel<cpv::SpvFs, " << c[0] << ", " << c[1] << ">(cpv::RasterArgs);\n";
+    for (const std::string& k : raster_instances())
+        s << "template __global__ void cpv::raster_kernel<cpv::SpvFs, " << k << ">(cpv::RasterArgs);\n";
     Translation t;
     t.source = s.str();
     t.vs_outputs = gv.out_total;
     t.fs_inputs = fin;
+    t.psize = gv.psize;
     return t;
 }
 
 std::vector<std::string> kernel_names()
 {
     std::vector<std::string> n{"cpv::vertex_kernel<cpv::SpvVs>"};
-    static const int cfg[7][2] = {{8, 1}, {16, 1}, {16, 2}, {16, 4}, {32, 1}, {32, 2}, {32, 4}};
-    for (const auto& c : cfg)
-        n.push_back("cpv::raster_kernel<cpv::SpvFs, " + std::to_string(c[0]) + ", " + std::to_string(c[1]) + ">");
+    for (const std::string& k : raster_instances()) n.push_back("cpv::raster_kernel<cpv::SpvFs, " + k + ">");
     return n;
 }
 
@@ -1624,7 +1655,7 @@ bool cache_load(const std::string& key, std::vector<char>* code, std::vector<std
     std::string line;
     while (std::getline(nm, line))
         if (!line.empty()) lowered->push_back(line);
-    return !code->empty() && lowered->size() == 8;
+    return !code->empty() && lowered->size() == 1 + (size_t)cpv::kRasterVariants;
 }
 
 void cache_store(const std::string& key, const std::vector<char>& code, const std::vector<std::string>& lowered)
@@ -1719,6 +1750,7 @@ int cpv_spirv_compile(cpv_context* c, const uint32_t* vs, size_t vs_n, const cha
     p->log = log;
     p->vs_outputs = t.vs_outputs;
     p->fs_inputs = t.fs_inputs;
+    p->psize = t.psize;
     hipError_t e = hipModuleLoadData(&p->module, code.data());
     if (e != hipSuccess) {
         *err = std::string("hipModuleLoadData: ") + hipGetErrorString(e);
@@ -1726,7 +1758,8 @@ int cpv_spirv_compile(cpv_context* c, const uint32_t* vs, size_t vs_n, const cha
         return CPV_ERROR_DEVICE;
     }
     e = hipModuleGetFunction(&p->vertex.module_fn, p->module, lowered[0].c_str());
-    for (int i = 0; i < 7 && e == hipSuccess; i++) e = hipModuleGetFunction(&p->raster[i].module_fn, p->module, lowered[1 + i].c_str());
+    for (int i = 0; i < cpv::kRasterVariants && e == hipSuccess; i++)
+        e = hipModuleGetFunction(&p->raster[i].module_fn, p->module, lowered[1 + i].c_str());
     if (e != hipSuccess) {
         *err = std::string("hipModuleGetFunction: ") + hipGetErrorString(e);
         (void)hipModuleUnload(p->module);

@@ -405,8 +405,52 @@ def fs_discard_dark() -> list[int]:
     return m.assemble("Fragment", f.end(), [col, out], True)
 
 
+def vs_point_size() -> list[int]:
+    """layout(location=0) in vec4 pos; layout(location=1) in vec4 inColor;
+    layout(location=2) in float size; layout(location=0) out vec4 outColor;
+    void main() { gl_Position = pos; gl_PointSize = size; outColor = inColor; }"""
+    m = Module()
+    v4, f32 = m.vec(4), m.f32()
+    pv_t, pv = _per_vertex(m)
+    pos = _io(m, "Input", v4, 0, "pos")
+    col = _io(m, "Input", v4, 1, "inColor")
+    size = _io(m, "Input", f32, 2, "size")
+    out = _io(m, "Output", v4, 0, "outColor")
+    f = Function(m)
+    f.store(f.chain("Output", v4, pv, 0), f.load(v4, pos))
+    f.store(f.chain("Output", f32, pv, 1), f.load(f32, size))
+    f.store(out, f.load(v4, col))
+    return m.assemble("Vertex", f.end(), [pv, pos, col, size, out], False)
+
+
+def _fs_color_decorated(dec: str) -> list[int]:
+    m = Module()
+    v4 = m.vec(4)
+    col = _io(m, "Input", v4, 0, "color")
+    m.decorate(col, dec)
+    out = _io(m, "Output", v4, 0, "outColor")
+    f = Function(m)
+    f.store(out, f.load(v4, col))
+    return m.assemble("Fragment", f.end(), [col, out], True)
+
+
+def fs_color_flat() -> list[int]:
+    """layout(location=0) flat in vec4 color; layout(location=0) out vec4 outColor;
+    void main() { outColor = color; }"""
+    return _fs_color_decorated("Flat")
+
+
+def fs_color_noperspective() -> list[int]:
+    """layout(location=0) noperspective in vec4 color; layout(location=0) out vec4 outColor;
+    void main() { outColor = color; }"""
+    return _fs_color_decorated("NoPerspective")
+
+
 SHADERS = {
     "vs_passthrough": vs_passthrough,
+    "vs_point_size": vs_point_size,
+    "fs_color_flat": fs_color_flat,
+    "fs_color_noperspective": fs_color_noperspective,
     "vs_mvp_color": vs_mvp_color,
     "vs_mvp_lambert": vs_mvp_lambert,
     "fs_color": fs_color,

@@ -14,6 +14,8 @@ Generators (all seeded, fp64 then rounded once to fp32):
   grid_scene      textured indexed heightfield with Lambert shading
   skewed_scene    triangle soup with centres from a mixture of Gaussians
   job_scenes      many independent soups of Zipf-distributed sizes
+  points_scene    point list with per-vertex gl_PointSize, perspective w
+  lines_scene     line list or strip of a given width, perspective w
 """
 from __future__ import annotations
 
@@ -82,9 +84,19 @@ class Scene:
     @property
     def primitive_count(self) -> int:
         topo = self.pipeline.get("topology", abi.TOPOLOGY["TRIANGLE_LIST"])
+        T = abi.TOPOLOGY
         n = 0
         for d in self.draws:
-            per = d.count // 3 if topo == abi.TOPOLOGY["TRIANGLE_LIST"] else max(d.count - 2, 0)
+            if topo == T["POINT_LIST"]:
+                per = d.count
+            elif topo == T["LINE_LIST"]:
+                per = d.count // 2
+            elif topo == T["LINE_STRIP"]:
+                per = max(d.count - 1, 0)
+            elif topo == T["TRIANGLE_LIST"]:
+                per = d.count // 3
+            else:
+                per = max(d.count - 2, 0)
             n += per * d.instance_count
         return n
 
@@ -129,6 +141,7 @@ def pipeline_desc(p: dict) -> abi.PipelineDesc:
     d.blend[0] = ba
     for i, v in enumerate(p.get("blend_constants", (0.0, 0.0, 0.0, 0.0))):
         d.blend_constants[i] = v
+    d.line_width = p.get("line_width", 1.0)
     return d
 
 
@@ -376,7 +389,7 @@ def cube_mvp(width: int = 500, height: int = 500) -> np.ndarray:
 
 
 def cube_scene(cull: str = "NONE", width: int = 500, height: int = 500) -> Scene:
-    """the 15-draw_cube sample on the CPU path."""
+    """The 15-draw_cube sample: 500x500 B8G8R8A8 + D16, MVP, 36 vertices."""
     verts = cube_vertices()
     mvp = cube_mvp(width, height)
     pipeline = dict(
@@ -444,7 +457,7 @@ def soup_pipeline() -> dict:
 
 
 def soup_scene(ntris: int = 1_000_000, width: int = 1920, height: int = 1080, seed: int = 2) -> Scene:
-    """triangle soup, R8G8B8A8 + D32 LESS, alpha blend."""
+    """Triangle soup: R8G8B8A8 + D32 LESS, alpha blend."""
     verts = soup_triangles(ntris, width, height, seed)
     draw = DrawData(vertices={0: verts.view(np.uint8).reshape(-1)}, count=ntris * 3,
                     viewport=(0.0, 0.0, float(width), float(height), 0.0, 1.0))
@@ -452,9 +465,80 @@ def soup_scene(ntris: int = 1_000_000, width: int = 1920, height: int = 1080, se
                  soup_pipeline(), [draw], clear_color=(0.0, 0.0, 0.0, 0.0), clear_depth=1.0)
 
 
+def _clip_positions(ndc: np.ndarray, rng: np.random.Generator, perspective: bool) -> np.ndarray:
+    """Clip-space vec4 positions for NDC (x, y) with z ~ U[0, 1]; w ~ U[0.5, 2]
+    (positions scaled by w) when perspective, else w = 1."""
+    n = ndc.shape[0]
+    w = rng.uniform(0.5, 2.0, size=n) if perspective else np.ones(n)
+    z = rng.uniform(0.0, 1.0, size=n)
+    return np.stack([ndc[:, 0] * w, ndc[:, 1] * w, z * w, w], axis=1)
+
+
+def points_scene(n: int = 400, width: int = 256, height: int = 160, seed: int = 11, perspective: bool = True,
+                 sizes: np.ndarray | None = None) -> Scene:
+    """Point list: centres uniform over the viewport and a margin, gl_PointSize
+    per vertex (default U[0.5, 14] with a few exact values), colour per
+    vertex, alpha blended over D32 LESS.  Vertex layout: pos vec4, colour
+    vec4, size float (36 bytes)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ndc = rng.uniform(-1.1, 1.1, size=(n, 2))
+    pos = _clip_positions(ndc, rng, perspective)
+    if sizes is None:
+        sizes = rng.uniform(0.5, 14.0, size=n)
+        fixed = np.array([0.0, 1.0, 1.5, 2.0, 3.0, 7.0, 63.96875])
+        sizes[: min(n, fixed.size)] = fixed[: min(n, fixed.size)]
+    col = np.concatenate([rng.uniform(0.0, 1.0, size=(n, 3)), rng.uniform(0.2, 1.0, size=(n, 1))], axis=1)
+    v = np.concatenate([pos, col, np.asarray(sizes, np.float64).reshape(n, 1)], axis=1).astype(np.float32)
+    pipe = dict(
+        vs=abi.VS["PASSTHROUGH_POINT"], fs=abi.FS["COLOR"], topology=abi.TOPOLOGY["POINT_LIST"],
+        bindings=[(0, 36, 0)],
+        attributes=[(0, 0, abi.FORMAT["R32G32B32A32_SFLOAT"], 0), (1, 0, abi.FORMAT["R32G32B32A32_SFLOAT"], 16),
+                    (2, 0, abi.FORMAT["R32_SFLOAT"], 32)],
+        depth_test=1, depth_write=1, depth_compare=abi.COMPARE["LESS"], blend=ALPHA_BLEND,
+    )
+    draw = DrawData(vertices={0: v.view(np.uint8).reshape(-1)}, count=n,
+                    viewport=(0.0, 0.0, float(width), float(height), 0.0, 1.0))
+    return Scene(f"points{n}", width, height, abi.FORMAT["R8G8B8A8_UNORM"], abi.FORMAT["D32_SFLOAT"], pipe, [draw],
+                 clear_color=(0.05, 0.1, 0.15, 1.0), clear_depth=1.0)
+
+
+def lines_scene(n: int = 300, width: int = 256, height: int = 160, seed: int = 12, strip: bool = False,
+                line_width: float = 1.0, perspective: bool = True, length_px: float = 40.0) -> Scene:
+    """Line list (2n vertices) or strip (n + 1 vertices) of the given width:
+    start points uniform over the viewport and a margin, segments of
+    LogNormal(log(length_px), 0.8) pixels in uniform directions, colour per
+    vertex, alpha blended over D32 LESS.  Vertex layout: pos vec4, colour
+    vec4 (32 bytes)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    m = n + 1 if strip else 2 * n
+    if strip:
+        steps = rng.lognormal(math.log(length_px), 0.8, size=m)
+        ang = rng.uniform(0.0, 2.0 * math.pi, size=m)
+        px = np.cumsum(steps * np.cos(ang)) + rng.uniform(0, width)
+        py = np.cumsum(steps * np.sin(ang)) + rng.uniform(0, height)
+        ndc = np.stack([(np.mod(px, width * 1.2) - 0.1 * width) * 2.0 / width - 1.0,
+                        (np.mod(py, height * 1.2) - 0.1 * height) * 2.0 / height - 1.0], axis=1)
+    else:
+        a = rng.uniform(-1.1, 1.1, size=(n, 2))
+        L = rng.lognormal(math.log(length_px), 0.8, size=n)
+        ang = rng.uniform(0.0, 2.0 * math.pi, size=n)
+        b = a + np.stack([L * np.cos(ang) * 2.0 / width, L * np.sin(ang) * 2.0 / height], axis=1)
+        ndc = np.stack([a, b], axis=1).reshape(m, 2)
+    pos = _clip_positions(ndc, rng, perspective)
+    col = np.concatenate([rng.uniform(0.0, 1.0, size=(m, 3)), rng.uniform(0.2, 1.0, size=(m, 1))], axis=1)
+    v = np.concatenate([pos, col], axis=1).astype(np.float32)
+    pipe = soup_pipeline()
+    pipe["topology"] = abi.TOPOLOGY["LINE_STRIP" if strip else "LINE_LIST"]
+    pipe["line_width"] = line_width
+    draw = DrawData(vertices={0: v.view(np.uint8).reshape(-1)}, count=m,
+                    viewport=(0.0, 0.0, float(width), float(height), 0.0, 1.0))
+    return Scene(f"lines{n}", width, height, abi.FORMAT["R8G8B8A8_UNORM"], abi.FORMAT["D32_SFLOAT"], pipe, [draw],
+                 clear_color=(0.05, 0.1, 0.15, 1.0), clear_depth=1.0)
+
+
 def skewed_scene(ntris: int = 4_000_000, width: int = 3840, height: int = 2160, seed: int = 5,
                  clusters: int = 8) -> Scene:
-    """soup whose centres come from a mixture of Gaussians."""
+    """Triangle soup whose centres come from a mixture of Gaussians."""
     rng = np.random.Generator(np.random.PCG64(seed + 1000))
     means = rng.uniform(-0.8, 0.8, size=(clusters, 2))
     sig = rng.uniform(0.02, 0.2, size=clusters)
@@ -532,7 +616,7 @@ def grid_mvp(width: int, height: int, orbit_deg: float = 34.0) -> np.ndarray:
 
 def grid_scene(n: int = 512, width: int = 3840, height: int = 2160, seed: int = 3, tex_size: int = 1024,
                uv_repeat: float = 4.0, orbit_deg: float = 34.0) -> Scene:
-    """textured indexed heightfield with Lambert shading.
+    """Textured indexed heightfield with Lambert shading.
 
     n x n vertices over [-1,1]^2 in xz with y = 0.2 * Perlin; vertex = pos vec3,
     normal vec3, uv vec2 (32 B); uint32 indices, two triangles (a, b, c),
@@ -589,7 +673,7 @@ def zipf_counts(n: int, rng: np.random.Generator, s: float = 1.2, lo: int = 12, 
 
 def job_scenes(n_jobs: int = 256, seed: int = 4, sides=(256, 512, 1024, 2048, 4096), lo: int = 12,
                hi: int = 200_000) -> list:
-    """independent render jobs of uneven size.  Triangle
+    """Independent render jobs of uneven size.  Triangle
     counts ~ Zipf(1.2) over [lo, hi]; framebuffer side uniform over `sides`
     (square); R8G8B8A8 + D32; triangles as soup_scene, from per-job seeds, in
     each job's own pixel units."""

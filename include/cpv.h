@@ -90,14 +90,18 @@ enum {
     CPV_VS_PASSTHROUGH = 1,   /* in vec4 pos @0, in vec4 color @1; gl_Position = pos; out vec4 = color */
     CPV_VS_MVP_COLOR = 2,     /* uniform(binding 0) mat4 mvp; gl_Position = mvp * pos; out vec4 = color
                                  (Samples/15-draw_cube/15-draw_cube.cpp:38-51) */
-    CPV_VS_MVP_LAMBERT = 3    /* in vec3 pos @0, vec3 normal @1, vec2 uv @2; uniform(binding 0) mat4 mvp;
+    CPV_VS_MVP_LAMBERT = 3,   /* in vec3 pos @0, vec3 normal @1, vec2 uv @2; uniform(binding 0) mat4 mvp;
                                  gl_Position = mvp * vec4(pos, 1); out vec3 normal; out vec2 uv */
+    CPV_VS_PASSTHROUGH_POINT = 4  /* in vec4 pos @0, in vec4 color @1, in float size @2; gl_Position = pos;
+                                     gl_PointSize = size; out vec4 = color (point topologies) */
 };
 enum {
     CPV_FS_COLOR = 1,         /* in vec4 color; out vec4 = color (15-draw_cube.cpp:53-61) */
     CPV_FS_TEX_LAMBERT = 2,   /* in vec3 n, in vec2 uv; sampler2D(binding 1); uniform(binding 2) vec4 L;
                                  out = texture(tex, uv) * max(dot(normalize(n), L.xyz), 0) */
-    CPV_FS_COLOR_DISCARD = 3  /* in vec4 color; if (color.r < 0.25) discard; out vec4 = color */
+    CPV_FS_COLOR_DISCARD = 3, /* in vec4 color; if (color.r < 0.25) discard; out vec4 = color */
+    CPV_FS_COLOR_FLAT = 4,    /* flat in vec4 color; out vec4 = color (the provoking vertex's colour) */
+    CPV_FS_COLOR_NOPERSPECTIVE = 5  /* noperspective in vec4 color; out vec4 = color */
 };
 
 /* Scheduling policies of the tile work queue (cpv_config.scheduler). */
@@ -158,6 +162,8 @@ typedef struct cpv_pipeline_desc {
     uint32_t color_formats[CPV_MAX_COLOR_ATTACHMENTS];
     uint32_t depth_stencil_format;  /* CPV_FORMAT_UNDEFINED when absent */
     cpv_blend_attachment blend[CPV_MAX_COLOR_ATTACHMENTS];
+    float line_width;               /* RasterizationState.LineWidth (line topologies) */
+    uint32_t reserved;
 } cpv_pipeline_desc;
 
 typedef struct cpv_sampler_desc {

@@ -623,6 +623,10 @@ static int vs_info(uint32_t kind, vs_info_t* o)
         o->nout = 5; o->nin_attr = 3;
         o->loc[0] = 0; o->comps[0] = 3; o->loc[1] = 1; o->comps[1] = 3; o->loc[2] = 2; o->comps[2] = 2;
         return 1;
+    case CPV_VS_PASSTHROUGH_POINT:
+        o->nout = 5; o->nin_attr = 3;
+        o->loc[0] = 0; o->comps[0] = 4; o->loc[1] = 1; o->comps[1] = 4; o->loc[2] = 2; o->comps[2] = 1;
+        return 1;
     default: return 0;
     }
 }
@@ -633,8 +637,20 @@ static int fs_inputs(uint32_t kind)
     case CPV_FS_COLOR: return 4;
     case CPV_FS_COLOR_DISCARD: return 4;
     case CPV_FS_TEX_LAMBERT: return 5;
+    case CPV_FS_COLOR_FLAT: return 4;
+    case CPV_FS_COLOR_NOPERSPECTIVE: return 4;
     default: return -1;
     }
+}
+
+/* InterpolationType of the fragment shader's inputs (every input of a
+ * built-in shader shares one): 0 Perspective, 1 Linear (NoPerspective),
+ * 2 Flat. */
+static int fs_interp(uint32_t kind)
+{
+    if (kind == CPV_FS_COLOR_FLAT) return 2;
+    if (kind == CPV_FS_COLOR_NOPERSPECTIVE) return 1;
+    return 0;
 }
 
 static uint32_t float_format_for(int comps)
@@ -702,6 +718,12 @@ static int run_vs(const cpv_pipeline_desc* p, const cpv_draw_desc* d, uint32_t v
         memcpy(pos, in[0], 16);
         memcpy(vary, in[1], 16);
         break;
+    case CPV_VS_PASSTHROUGH_POINT:
+        /* gl_Position = pos, gl_PointSize = size; the size is output 4 */
+        memcpy(pos, in[0], 16);
+        memcpy(vary, in[1], 16);
+        vary[4] = in[2][0];
+        break;
     case CPV_VS_MVP_COLOR: {
         const float* mvp = uniform_ptr(d, 0);
         if (!mvp) return CPV_ERROR_INVALID;
@@ -728,6 +750,8 @@ static int run_fs(const cpv_pipeline_desc* p, const cpv_draw_desc* d, const floa
 {
     switch (p->fs_kind) {
     case CPV_FS_COLOR:
+    case CPV_FS_COLOR_FLAT:
+    case CPV_FS_COLOR_NOPERSPECTIVE:
         memcpy(out, in, 16);
         return 0;
     case CPV_FS_COLOR_DISCARD:
@@ -927,11 +951,17 @@ static void fragment_ops(ctx_t* c, int x, int y, float depth, int front, const f
     cpvref_pack_f32(p->color_formats[0], value, px);
 }
 
-/* SetDatum (CommandBuffer.Draw.cpp:816-837) for one float component. */
-static float set_datum_perspective(const float pw[3], const float v[3], const float w[3])
+/* SetDatum (CommandBuffer.Draw.cpp:816-837) for one float component over
+ * n vertices: perspective (mode 0) or linear (mode 1). */
+static float set_datum(int mode, int n, const float* pw, const float* v, const float* w)
 {
+    if (mode == 1) {
+        float r = 0.0f;
+        for (int i = 0; i < n; i++) r += w[i] * v[i];
+        return r;
+    }
     float num = 0.0f, den = 0.0f;
-    for (int i = 0; i < 3; i++) {
+    for (int i = 0; i < n; i++) {
         num += w[i] * v[i] / pw[i];
         den += w[i] / pw[i];
     }
@@ -1013,16 +1043,129 @@ static void process_triangles(ctx_t* c, const uint32_t (*prims)[4], uint64_t npr
                 float in[16];
                 const float pw[3] = {P[0][3], P[1][3], P[2][3]};
                 const float w[3] = {w0, w1, w2};
+                const int mode = fs_interp(p->fs_kind);
                 for (int k = 0; k < nvary; k++) {
                     const float v[3] = {q[0][4 + k], q[1][4 + k], q[2][4 + k]};
-                    in[k] = set_datum_perspective(pw, v, w);
+                    /* Flat: the provoking vertex's value (:943-946) */
+                    in[k] = mode == 2 ? vout[(uint64_t)provoking * vstride + 4 + k] : set_datum(mode, 3, pw, v, w);
                 }
-                (void)provoking;
                 /* DrawPixel (:1300-1313) */
                 float dt = (vp[5] - vp[4]) * depth + vp[4];
                 float colour[4];
                 if (run_fs(p, c->draw, in, colour)) continue;
                 fragment_ops(c, x, y, dt, front, colour);
+            }
+        }
+    }
+}
+
+/* int32 sum with two's-complement wrap, as the reference's compiled int
+ * arithmetic behaves. */
+static int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+static int32_t wrap_sub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+
+/* ProcessPoints (CommandBuffer.Draw.cpp:1315-1383) over points [pb, pe):
+ * the square around (int)((p.x + 1) * 0.5f * (width - 1)) of half extent
+ * (int)ceil(pointSize / 2), pixels with s, t in [0, 1], the vertex's
+ * outputs copied and its depth. */
+static void process_points(ctx_t* c, uint64_t nverts, const float* vout, int vstride, int nvary, int psize,
+                           uint64_t pb, uint64_t pe)
+{
+    const float* vp = c->draw->viewport;
+    const float vpw = vp[2], vph = vp[3];
+    int32_t limit_x = f2i_trunc(vpw), limit_y = f2i_trunc(vph);
+    if (limit_x > (int32_t)c->pass->width) limit_x = (int32_t)c->pass->width;
+    if (limit_y > (int32_t)c->pass->height) limit_y = (int32_t)c->pass->height;
+    if (pe > nverts) pe = nverts;
+    for (uint64_t pi = pb; pi < pe; pi++) {
+        const float* q = vout + pi * (uint64_t)vstride;
+        const float P[4] = {q[0] / q[3], q[1] / q[3], q[2] / q[3], q[3]};
+        const int32_t cx = f2i_trunc((P[0] + 1.0f) * 0.5f * (vpw - 1.0f));
+        const int32_t cy = f2i_trunc((P[1] + 1.0f) * 0.5f * (vph - 1.0f));
+        const float size = psize >= 0 ? q[4 + psize] : 0.0f;
+        const int32_t h = f2i_trunc(ceilf(size / 2.0f));
+        int32_t sx = wrap_sub(cx, h), sy = wrap_sub(cy, h);
+        int32_t ex = wrap_add(wrap_add(cx, h), 1), ey = wrap_add(wrap_add(cy, h), 1);
+        if (sx < 0) sx = 0;
+        if (sy < 0) sy = 0;
+        if (ex > limit_x) ex = limit_x;
+        if (ey > limit_y) ey = limit_y;
+        for (int32_t y = sy; y < ey; y++) {
+            for (int32_t x = sx; x < ex; x++) {
+                const float s = 0.5f + (float)wrap_sub(x, cx) / size;
+                const float t = 0.5f + (float)wrap_sub(y, cy) / size;
+                if (!(s >= 0 && t >= 0 && s <= 1 && t <= 1)) continue;
+                c->fragments++;
+                float in[16];
+                for (int k = 0; k < nvary; k++) in[k] = q[4 + k];
+                const float dt = (vp[5] - vp[4]) * P[2] + vp[4];
+                float colour[4];
+                if (run_fs(c->pipe, c->draw, in, colour)) continue;
+                fragment_ops(c, x, y, dt, 1, colour);
+            }
+        }
+    }
+}
+
+/* ProcessLines (CommandBuffer.Draw.cpp:1385-1508) over lines [pb, pe):
+ * every pixel of the viewport (x < width, y < height as floats) whose centre
+ * passes the four edge tests of the quad around p0p1. */
+static void process_lines(ctx_t* c, uint64_t nverts, const float* vout, int vstride, int nvary, uint64_t pb, uint64_t pe)
+{
+    const cpv_pipeline_desc* p = c->pipe;
+    const float* vp = c->draw->viewport;
+    const float vpw = vp[2], vph = vp[3];
+    const float half_x = 1.0f / vpw * 0.5f, half_y = 1.0f / vph * 0.5f;
+    uint64_t n = 0;
+    if (p->topology == CPV_TOPOLOGY_LINE_LIST) n = nverts / 2;
+    else if (nverts > 1) n = nverts - 1;
+    if (pe > n) pe = n;
+    /* the attachment extent guards the scan (the reference writes out of
+     * bounds for a viewport larger than the framebuffer) */
+    uint32_t lx = 0, ly = 0;
+    while (lx < c->pass->width && (float)lx < vpw) lx++;
+    while (ly < c->pass->height && (float)ly < vph) ly++;
+    const float lw = p->line_width;
+    for (uint64_t pi = pb; pi < pe; pi++) {
+        const uint64_t i0 = p->topology == CPV_TOPOLOGY_LINE_LIST ? 2 * pi : pi;
+        const float* q0 = vout + i0 * (uint64_t)vstride;
+        const float* q1 = vout + (i0 + 1) * (uint64_t)vstride;
+        const float P0[4] = {q0[0] / q0[3], q0[1] / q0[3], q0[2] / q0[3], q0[3]};
+        const float P1[4] = {q1[0] / q1[3], q1[1] / q1[3], q1[2] / q1[3], q1[3]};
+        const float lwx = lw / vpw, lwy = lw / vph;
+        const float d[4] = {P1[0] - P0[0], P1[1] - P0[1], P1[2] - P0[2], P1[3] - P0[3]};
+        /* glm::normalize: v * (1 / sqrt(x*x + y*y + z*z + w*w)), left to right */
+        const float inv = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + d[3] * d[3]);
+        const float dir[2] = {d[0] * inv, d[1] * inv};
+        const float perp[2] = {dir[1], -dir[0]};
+        const float p00[2] = {P0[0] + perp[0] * lwx, P0[1] + perp[1] * lwy};
+        const float p01[2] = {P0[0] - perp[0] * lwx, P0[1] - perp[1] * lwy};
+        const float p10[2] = {P1[0] + perp[0] * lwx, P1[1] + perp[1] * lwy};
+        const float p11[2] = {P1[0] - perp[0] * lwx, P1[1] - perp[1] * lwy};
+        const float len = sqrtf(d[0] * d[0] + d[1] * d[1]);
+        for (uint32_t y = 0; y < ly; y++) {
+            const float yf = ((float)y / vph + half_y) * 2.0f - 1.0f;
+            for (uint32_t x = 0; x < lx; x++) {
+                const float xf = ((float)x / vpw + half_x) * 2.0f - 1.0f;
+                if (!(edge(p00, p01, xf, yf) >= 0 && edge(p11, p10, xf, yf) >= 0 && edge(p10, p00, xf, yf) >= 0 &&
+                      edge(p01, p11, xf, yf) >= 0))
+                    continue;
+                c->fragments++;
+                const float t = ((xf - P0[0]) * d[0] + (yf - P0[1]) * d[1]) / (len * len);
+                const float w[2] = {1 - t, t};
+                const float pw[2] = {P0[3], P1[3]};
+                const int mode = fs_interp(p->fs_kind);
+                float in[16];
+                for (int k = 0; k < nvary; k++) {
+                    /* SetDatum<..., 2>; Flat reads the provoking vertex, p0 (:1480-1482) */
+                    const float v[2] = {q0[4 + k], q1[4 + k]};
+                    in[k] = mode == 2 ? q0[4 + k] : set_datum(mode, 2, pw, v, w);
+                }
+                const float depth = P0[2] * t + P1[2] * (1 - t);
+                const float dt = (vp[5] - vp[4]) * depth + vp[4];
+                float colour[4];
+                if (run_fs(p, c->draw, in, colour)) continue;
+                fragment_ops(c, (int)x, (int)y, dt, 1, colour);
             }
         }
     }
@@ -1054,9 +1197,7 @@ static int draw_one(ctx_t* c, uint64_t pb, uint64_t pe)
 {
     const cpv_pipeline_desc* p = c->pipe;
     const cpv_draw_desc* d = c->draw;
-    if (p->topology != CPV_TOPOLOGY_TRIANGLE_LIST && p->topology != CPV_TOPOLOGY_TRIANGLE_STRIP &&
-        p->topology != CPV_TOPOLOGY_TRIANGLE_FAN)
-        return CPV_ERROR_UNSUPPORTED;
+    if (p->topology > CPV_TOPOLOGY_TRIANGLE_FAN) return CPV_ERROR_UNSUPPORTED;
     if (p->polygon_mode != CPV_POLYGON_FILL || p->rasterizer_discard || p->depth_clamp || p->logic_op_enable)
         return CPV_ERROR_UNSUPPORTED;
     if (p->color_count && p->blend[0].color_write_mask != CPV_COLOR_RGBA) return CPV_ERROR_UNSUPPORTED;
@@ -1093,10 +1234,16 @@ static int draw_one(ctx_t* c, uint64_t pb, uint64_t pe)
             int rc = run_vs(p, d, vid[i], d->first_instance + inst, vout + i * vstride, vout + i * vstride + 4);
             if (rc) { free(vid); free(vout); return rc; }
         }
-        uint64_t nprims;
-        uint32_t (*prims)[4] = assemble(p->topology, n, &nprims);
-        process_triangles(c, (const uint32_t (*)[4])prims, nprims, vout, vstride, nvary, pb, pe);
-        free(prims);
+        if (p->topology == CPV_TOPOLOGY_POINT_LIST) {
+            process_points(c, n, vout, vstride, nvary, p->vs_kind == CPV_VS_PASSTHROUGH_POINT ? 4 : -1, pb, pe);
+        } else if (p->topology < CPV_TOPOLOGY_TRIANGLE_LIST) {
+            process_lines(c, n, vout, vstride, nvary, pb, pe);
+        } else {
+            uint64_t nprims;
+            uint32_t (*prims)[4] = assemble(p->topology, n, &nprims);
+            process_triangles(c, (const uint32_t (*)[4])prims, nprims, vout, vstride, nvary, pb, pe);
+            free(prims);
+        }
         free(vout);
         free(vid);
     }

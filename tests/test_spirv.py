@@ -14,7 +14,15 @@ def test_translation_of_the_cube_shaders():
     assert "mat4_mul_vec4(" in src and "env.uniform[0]" in src
     assert "static constexpr int kOutputs = 4;" in src and "static constexpr int kInputs = 4;" in src
     assert "kMayDiscard = false" in src
-    assert src.count("template __global__ void cpv::raster_kernel<cpv::SpvFs") == 7
+    assert src.count("template __global__ void cpv::raster_kernel<cpv::SpvFs") == 8   # 7 tile configurations + points/lines
+
+
+def test_translation_of_point_size():
+    """gl_PointSize travels as the vertex output after the varyings only
+    when the shader writes it."""
+    src = cpvulkan_amd.spirv_translate(S.vs_point_size(), S.fs_color())
+    assert "static constexpr int kOutputs = 5;" in src and "out[4] = " in src
+    assert "cpv::raster_kernel<cpv::SpvFs, 16, 4, true>" in src
 
 
 def test_translation_of_the_lambert_shaders():
