@@ -449,37 +449,64 @@ struct EmitArgs {
     uint32_t cap;                 // capacity of keys / vals; pairs at or beyond it are dropped
 };
 
-constexpr uint32_t kEmitSmall = 32;   // pairs a single thread writes
+constexpr uint32_t kEmitBig = 2048;   // pairs above which a primitive goes to emit_big_kernel
 
 // Writes the (tile, primitive) pairs of every primitive at its scanned
-// offset, so the pair array is in primitive order.  Primitives with more
-// than kEmitSmall tiles are queued for emit_big_kernel, which spreads them
-// over a whole workgroup.
+// offset, so the pair array is in primitive order.  A workgroup takes 256
+// consecutive primitives, whose pairs are one contiguous run of the output
+// (less the runs of primitives with more than kEmitBig pairs, queued for
+// emit_big_kernel): its threads stride over the run, so consecutive lanes
+// write consecutive pairs, and each finds its primitive by binary search of
+// the workgroup's scanned counts in LDS.  Tiles of a primitive go in
+// row-major order of its tile rectangle.
 __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
 {
-    for (uint32_t gp = blockIdx.x * 256u + threadIdx.x; gp < a.prims; gp += gridDim.x * 256u) {
-        const uint32_t c = a.counts[gp];
-        if (c == 0) continue;
-        if (c > kEmitSmall) {
+    __shared__ uint32_t s_tmp[4];
+    __shared__ uint32_t s_off[257];   // exclusive scan of the kept pair counts
+    __shared__ uint32_t s_dst[256];   // output offset of each primitive's first pair
+    __shared__ uint32_t s_key[256];   // global tile index of each primitive's first tile
+    __shared__ uint32_t s_w[256];     // tile columns of each primitive's rectangle
+    __shared__ uint32_t s_tx[256];    // tile columns of each primitive's target
+    const uint32_t nblk = (a.prims + 255u) / 256u;
+    for (uint32_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const uint32_t gp = blk * 256u + threadIdx.x;
+        uint32_t c = gp < a.prims ? a.counts[gp] : 0u;
+        if (c > kEmitBig) {
             a.big[1 + atomicAdd(&a.big[0], 1u)] = gp;
-            continue;
+            c = 0;
         }
-        const TriSetup& r = a.recs[gp];
-        const TargetParams& t = a.targets[a.draws[r.draw].target];
-        const int sh = a.tile_shift;
-        const uint32_t tx0 = (uint32_t)r.x0 >> sh, tx1 = (uint32_t)(r.x1 - 1) >> sh;
-        const uint32_t ty0 = (uint32_t)r.y0 >> sh, ty1 = (uint32_t)(r.y1 - 1) >> sh;
-        uint32_t off = a.offsets[gp];
-        for (uint32_t ty = ty0; ty <= ty1; ty++) {
-            for (uint32_t tx = tx0; tx <= tx1; tx++) {
-                const uint32_t key = t.tile_base + ty * t.tiles_x + tx;
-                if (off < a.cap) {
-                    a.keys[off] = key;
-                    a.vals[off] = gp;
-                }
-                off++;
+        if (c) {
+            const TriSetup& r = a.recs[gp];
+            const TargetParams& t = a.targets[a.draws[r.draw].target];
+            const int sh = a.tile_shift;
+            const uint32_t tx0 = (uint32_t)r.x0 >> sh, tx1 = (uint32_t)(r.x1 - 1) >> sh;
+            const uint32_t ty0 = (uint32_t)r.y0 >> sh;
+            s_dst[threadIdx.x] = a.offsets[gp];
+            s_key[threadIdx.x] = t.tile_base + ty0 * t.tiles_x + tx0;
+            s_w[threadIdx.x] = tx1 - tx0 + 1;
+            s_tx[threadIdx.x] = t.tiles_x;
+        }
+        uint32_t total;
+        s_off[threadIdx.x] = block_exclusive_scan(c, s_tmp, total);
+        if (threadIdx.x == 0) s_off[256] = total;
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < total; j += 256u) {
+            // q: the last primitive whose run starts at or before j
+            uint32_t lo = 0, hi = 256;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_off[mid] <= j) lo = mid; else hi = mid;
+            }
+            const uint32_t k = j - s_off[lo];
+            const uint32_t w = s_w[lo];
+            const uint32_t row = k / w;
+            const uint32_t out = s_dst[lo] + k;
+            if (out < a.cap) {
+                a.keys[out] = s_key[lo] + row * s_tx[lo] + (k - row * w);
+                a.vals[out] = blk * 256u + lo;
             }
         }
+        __syncthreads();
     }
 }
 

@@ -164,6 +164,9 @@ struct Pipeline {
     cpv_pipeline_desc desc{};
     bool dynamic_viewport = false;
     VkViewport viewport{};
+    // state taken from vkCmdSet* at draw time instead of from the pipeline
+    bool dyn_blend_constants = false, dyn_depth_bounds = false;
+    bool dyn_compare_mask = false, dyn_write_mask = false, dyn_reference = false;
 };
 
 struct CommandPool {};
@@ -171,7 +174,8 @@ struct CommandPool {};
 enum class Op {
     BeginPass, EndPass, BindPipeline, BindVertex, BindIndex, BindSets, SetViewport, Draw, DrawIndexed, DrawIndirect,
     DrawIndexedIndirect, CopyBuffer, CopyImage, CopyBufferToImage, CopyImageToBuffer, FillBuffer, UpdateBuffer, ClearColorImage,
-    ClearDepthStencilImage, ClearAttachments, BlitImage
+    ClearDepthStencilImage, ClearAttachments, BlitImage, SetBlendConstants, SetDepthBounds, SetStencilCompareMask,
+    SetStencilWriteMask, SetStencilReference
 };
 
 struct Cmd {
@@ -205,6 +209,7 @@ struct Cmd {
     uint32_t data = 0, stride = 0;
     int32_t filter = 0;
     std::vector<uint8_t> bytes;
+    float values[4] = {};   // dynamic state: blend constants, depth bounds (min, max)
 };
 
 struct CommandBuffer : Dispatchable {
@@ -762,10 +767,16 @@ VkResult build_pipeline(Device* d, const VkGraphicsPipelineCreateInfo& ci, Pipel
     if (ci.pDynamicState)
         for (uint32_t i = 0; i < ci.pDynamicState->dynamicStateCount; i++) {
             const int32_t ds = ci.pDynamicState->pDynamicStates[i];
+            // The reference compiles these from the pipeline and stops
+            // (TODO_ERROR) when they are dynamic and used
+            // (PipelineCompiler.cpp:1141-1148, 1201-1214, 1305-1311); they take
+            // the vkCmdSet* values current at each draw here.
             if (ds == VK_DYNAMIC_STATE_VIEWPORT) p->dynamic_viewport = true;
-            // a dynamic depth-bounds range with the test on is not compiled
-            // by the reference (PipelineCompiler.cpp:1141-1148)
-            if (ds == VK_DYNAMIC_STATE_DEPTH_BOUNDS && o.depth_bounds_test) return VK_ERROR_FEATURE_NOT_PRESENT;
+            if (ds == VK_DYNAMIC_STATE_BLEND_CONSTANTS) p->dyn_blend_constants = true;
+            if (ds == VK_DYNAMIC_STATE_DEPTH_BOUNDS) p->dyn_depth_bounds = true;
+            if (ds == VK_DYNAMIC_STATE_STENCIL_COMPARE_MASK) p->dyn_compare_mask = true;
+            if (ds == VK_DYNAMIC_STATE_STENCIL_WRITE_MASK) p->dyn_write_mask = true;
+            if (ds == VK_DYNAMIC_STATE_STENCIL_REFERENCE) p->dyn_reference = true;
         }
     if (!p->dynamic_viewport && ci.pViewportState && ci.pViewportState->viewportCount && ci.pViewportState->pViewports)
         p->viewport = ci.pViewportState->pViewports[0];
@@ -899,6 +910,37 @@ void CmdSetViewport(VkCommandBuffer cb, uint32_t first, uint32_t n, const VkView
 }
 
 void CmdSetScissor(VkCommandBuffer, uint32_t, uint32_t, const VkRect2D*) {}   // ignored, as in the reference
+// Dynamic state (CommandBuffer.Binding.cpp:234-242 records the depth
+// bounds the same way); a draw reads the values current when it replays.
+void CmdSetDepthBounds(VkCommandBuffer cb, float mn, float mx)
+{
+    Cmd c{Op::SetDepthBounds};
+    c.values[0] = mn;
+    c.values[1] = mx;
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdSetBlendConstants(VkCommandBuffer cb, const float constants[4])
+{
+    Cmd c{Op::SetBlendConstants};
+    for (int k = 0; k < 4; k++) c.values[k] = constants[k];
+    CB(cb)->cmds.push_back(c);
+}
+
+void set_stencil(VkCommandBuffer cb, Op op, VkFlags faces, uint32_t v)
+{
+    Cmd c{op};
+    c.first = faces;
+    c.data = v;
+    CB(cb)->cmds.push_back(c);
+}
+void CmdSetStencilCompareMask(VkCommandBuffer cb, VkFlags faces, uint32_t v) { set_stencil(cb, Op::SetStencilCompareMask, faces, v); }
+void CmdSetStencilWriteMask(VkCommandBuffer cb, VkFlags faces, uint32_t v) { set_stencil(cb, Op::SetStencilWriteMask, faces, v); }
+void CmdSetStencilReference(VkCommandBuffer cb, VkFlags faces, uint32_t v) { set_stencil(cb, Op::SetStencilReference, faces, v); }
+
+// ProcessLines reads the pipeline's LineWidth only (CommandBuffer.Draw.cpp:1421),
+// so a dynamic width changes nothing, as in the reference.
+void CmdSetLineWidth(VkCommandBuffer, float) {}
 
 void CmdDraw(VkCommandBuffer cb, uint32_t vertexCount, uint32_t instanceCount, uint32_t firstVertex, uint32_t firstInstance)
 {
@@ -1051,6 +1093,9 @@ struct ReplayState {
     int32_t ib_type = 0;
     DescriptorSet* set0 = nullptr;
     VkViewport viewport{};
+    float blend_constants[4] = {};
+    float depth_bounds[2] = {};
+    uint32_t compare_mask[2] = {}, write_mask[2] = {}, reference[2] = {};   // front, back
     bool pass_split = false;   // a vkCmdClearAttachments ended an earlier part of the current pass
     std::vector<cpv_pipeline_desc> pipelines;
     std::vector<Pipeline*> pipeline_objs;
@@ -1362,12 +1407,26 @@ VkResult record_draw(ReplayState& s, const Cmd& c)
 {
     if (!s.pass || !s.pipeline) return VK_ERROR_DEVICE_LOST;
     if (!c.count || !c.instances) return VK_SUCCESS;
+    // the pipeline's state with its dynamic parts from the command buffer
+    const Pipeline& p = *s.pipeline;
+    cpv_pipeline_desc eff = p.desc;
+    if (p.dyn_blend_constants) std::memcpy(eff.blend_constants, s.blend_constants, sizeof(eff.blend_constants));
+    if (p.dyn_depth_bounds) {
+        eff.min_depth_bounds = s.depth_bounds[0];
+        eff.max_depth_bounds = s.depth_bounds[1];
+    }
+    cpv_stencil_state* face[2] = {&eff.front, &eff.back};
+    for (int k = 0; k < 2; k++) {
+        if (p.dyn_compare_mask) face[k]->compare_mask = s.compare_mask[k];
+        if (p.dyn_write_mask) face[k]->write_mask = s.write_mask[k];
+        if (p.dyn_reference) face[k]->reference = s.reference[k];
+    }
     uint32_t pi = 0;
-    for (; pi < s.pipeline_objs.size(); pi++)
-        if (s.pipeline_objs[pi] == s.pipeline) break;
-    if (pi == s.pipeline_objs.size()) {
+    for (; pi < s.pipelines.size(); pi++)
+        if (s.pipeline_objs[pi] == s.pipeline && std::memcmp(&s.pipelines[pi], &eff, sizeof(eff)) == 0) break;
+    if (pi == s.pipelines.size()) {
         s.pipeline_objs.push_back(s.pipeline);
-        s.pipelines.push_back(s.pipeline->desc);
+        s.pipelines.push_back(eff);
     }
     s.draws.push_back(make_draw(s, c));
     s.draw_pipeline.push_back(pi);
@@ -1407,6 +1466,14 @@ VkResult replay(Device* dev, const CommandBuffer* cb)
             if (c.first == 0 && !c.sets.empty()) s.set0 = c.sets[0];
             break;
         case Op::SetViewport: s.viewport = c.viewport; break;
+        case Op::SetBlendConstants: std::memcpy(s.blend_constants, c.values, sizeof(s.blend_constants)); break;
+        case Op::SetDepthBounds: s.depth_bounds[0] = c.values[0]; s.depth_bounds[1] = c.values[1]; break;
+        case Op::SetStencilCompareMask: case Op::SetStencilWriteMask: case Op::SetStencilReference: {
+            uint32_t* v = c.op == Op::SetStencilCompareMask ? s.compare_mask : c.op == Op::SetStencilWriteMask ? s.write_mask : s.reference;
+            if (c.first & VK_STENCIL_FACE_FRONT_BIT) v[0] = c.data;
+            if (c.first & VK_STENCIL_FACE_BACK_BIT) v[1] = c.data;
+            break;
+        }
         case Op::Draw: case Op::DrawIndexed:
             if (!s.pass || !s.pipeline) return VK_ERROR_DEVICE_LOST;
             if (VkResult e = record_draw(s, c)) return e;
@@ -1514,7 +1581,9 @@ const Entry kDevice[] = {
     E(CreateFramebuffer), E(DestroyFramebuffer), E(CreateGraphicsPipelines), E(DestroyPipeline), E(CreateCommandPool),
     E(DestroyCommandPool), E(AllocateCommandBuffers), E(FreeCommandBuffers), E(BeginCommandBuffer), E(EndCommandBuffer),
     E(ResetCommandBuffer), E(CmdBeginRenderPass), E(CmdEndRenderPass), E(CmdBindPipeline), E(CmdBindVertexBuffers),
-    E(CmdBindIndexBuffer), E(CmdBindDescriptorSets), E(CmdSetViewport), E(CmdSetScissor), E(CmdDraw), E(CmdDrawIndexed),
+    E(CmdBindIndexBuffer), E(CmdBindDescriptorSets), E(CmdSetViewport), E(CmdSetScissor), E(CmdSetDepthBounds),
+    E(CmdSetLineWidth), E(CmdSetBlendConstants), E(CmdSetStencilCompareMask), E(CmdSetStencilWriteMask),
+    E(CmdSetStencilReference), E(CmdDraw), E(CmdDrawIndexed),
     E(CmdDrawIndirect), E(CmdDrawIndexedIndirect), E(CmdCopyBuffer), E(CmdCopyImage), E(CmdCopyBufferToImage),
     E(CmdCopyImageToBuffer), E(CmdFillBuffer), E(CmdUpdateBuffer), E(CmdClearColorImage), E(CmdClearDepthStencilImage),
     E(CmdClearAttachments), E(CmdBlitImage),

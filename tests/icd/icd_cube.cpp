@@ -2,13 +2,16 @@
 // frame through the driver's own entry points (as the loader would reach
 // them) and writes the colour and depth images.
 //
-//   icd_cube <icd.so> <dir> <width> <height> [draw|indirect|midclear]
+//   icd_cube <icd.so> <dir> <width> <height> [draw|indirect|midclear|lines|dynamic]
 //
 // draw: vkCmdDraw(36, 1, 0, 0); indirect: the same vertices as two
 // VkDrawIndirectCommand records read by vkCmdDrawIndirect; midclear:
 // vkCmdDraw, then vkCmdClearAttachments of a colour and a depth rectangle
 // (colour (1, 0, 0.5, 1), depth 0.5) over [W/4, W/4 + W/2) x [H/4, H/4 + H/3),
-// then vkCmdDraw again, all in one render pass.
+// then vkCmdDraw again, all in one render pass; lines: the vertices as a
+// LINE_LIST of width 3; dynamic: constant-colour blending and a depth-bounds
+// test with dynamic constants and range, drawn twice: constants (0.25, 0.5,
+// 0.75, 1) and bounds [0, 1], then (0.8, 0.1, 0.3, 0.5) and [0, 0.995].
 //
 // <dir> holds vs.spv, fs.spv (SPIR-V), vertices.bin (36 x {vec4 pos, vec4
 // colour}) and mvp.bin (mat4); the program writes color.bin (B8G8R8A8) and
@@ -127,6 +130,8 @@ int main(int argc, char** argv)
         const uint32_t*);
     VFN(CmdSetViewport, VkCommandBuffer, uint32_t, uint32_t, const VkViewport*);
     VFN(CmdSetScissor, VkCommandBuffer, uint32_t, uint32_t, const VkRect2D*);
+    VFN(CmdSetBlendConstants, VkCommandBuffer, const float*);
+    VFN(CmdSetDepthBounds, VkCommandBuffer, float, float);
     VFN(CmdDraw, VkCommandBuffer, uint32_t, uint32_t, uint32_t, uint32_t);
     VFN(CmdDrawIndirect, VkCommandBuffer, VkBuffer, VkDeviceSize, uint32_t, uint32_t);
     VFN(CmdClearAttachments, VkCommandBuffer, uint32_t, const VkClearAttachment*, uint32_t, const VkClearRect*);
@@ -256,8 +261,23 @@ int main(int argc, char** argv)
     VkPipelineColorBlendAttachmentState cba{0, 0, 0, 0, 0, 0, 0, 0xF};
     VkPipelineColorBlendStateCreateInfo cbs{VK_STRUCTURE_TYPE_PIPELINE_COLOR_BLEND_STATE_CREATE_INFO, nullptr, 0, 0, 0, 1, &cba,
                                             {1.0f, 1.0f, 1.0f, 1.0f}};
-    int32_t dyn[2] = {VK_DYNAMIC_STATE_VIEWPORT, VK_DYNAMIC_STATE_SCISSOR};
+    if (mode == "lines") {
+        // the cube's 36 vertices as 18 lines, 3 pixels wide
+        ias.topology = CPV_TOPOLOGY_LINE_LIST;
+        rs.lineWidth = 3.0f;
+    }
+    int32_t dyn[4] = {VK_DYNAMIC_STATE_VIEWPORT, VK_DYNAMIC_STATE_SCISSOR, VK_DYNAMIC_STATE_BLEND_CONSTANTS,
+                      VK_DYNAMIC_STATE_DEPTH_BOUNDS};
     VkPipelineDynamicStateCreateInfo dys{VK_STRUCTURE_TYPE_PIPELINE_DYNAMIC_STATE_CREATE_INFO, nullptr, 0, 2, dyn};
+    if (mode == "dynamic") {
+        // constant-colour blending and a depth-bounds test whose constants
+        // and range come from vkCmdSetBlendConstants / vkCmdSetDepthBounds
+        // (the pipeline's own values would reject every fragment)
+        cba = VkPipelineColorBlendAttachmentState{1, 10, 11, 0, 12, 13, 0, 0xF};   // CONSTANT_COLOR, ONE_MINUS_..., CONSTANT_ALPHA, ...
+        dss.depthBoundsTestEnable = 1;
+        dss.minDepthBounds = dss.maxDepthBounds = -1.0f;
+        dys.dynamicStateCount = 4;
+    }
     VkGraphicsPipelineCreateInfo gpi{VK_STRUCTURE_TYPE_GRAPHICS_PIPELINE_CREATE_INFO, nullptr, 0, 2, stages, &vis, &ias, nullptr, &vps,
                                      &rs, &ms, &dss, &cbs, &dys, layout, rp, 0, nullptr, -1};
     VkPipeline pipe;
@@ -290,6 +310,14 @@ int main(int argc, char** argv)
         std::vector<char> bytes((const char*)recs, (const char*)recs + sizeof(recs));
         buffer(bytes, &ibuf);
         vkCmdDrawIndirect(cmd, ibuf, 0, 2, sizeof(VkDrawIndirectCommand));
+    } else if (mode == "dynamic") {
+        const float k1[4] = {0.25f, 0.5f, 0.75f, 1.0f}, k2[4] = {0.8f, 0.1f, 0.3f, 0.5f};
+        vkCmdSetBlendConstants(cmd, k1);
+        vkCmdSetDepthBounds(cmd, 0.0f, 1.0f);
+        vkCmdDraw(cmd, 36, 1, 0, 0);
+        vkCmdSetBlendConstants(cmd, k2);
+        vkCmdSetDepthBounds(cmd, 0.0f, 0.995f);
+        vkCmdDraw(cmd, 36, 1, 0, 0);
     } else {
         vkCmdDraw(cmd, 36, 1, 0, 0);   // 15-draw_cube.cpp:149
     }

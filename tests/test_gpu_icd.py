@@ -61,6 +61,41 @@ def test_draw_indirect_through_the_icd(tmp_path):
     assert np.array_equal(color, ref_color) and np.array_equal(depth, ref_ds)
 
 
+def test_lines_through_the_icd(tmp_path):
+    """The cube's vertices as a LINE_LIST with lineWidth 3 from the
+    pipeline's rasterization state (ProcessLines through the ICD)."""
+    color, depth = _run_cube(tmp_path, 180, "lines")
+    scene = W.cube_scene(width=180, height=180)
+    scene.pipeline["topology"] = abi.TOPOLOGY["LINE_LIST"]
+    scene.pipeline["line_width"] = 3.0
+    ref_color, ref_ds, _ = oracle_lib.render_scene(scene)
+    px = ref_color.reshape(-1, 4)
+    assert (px != px[0]).any(axis=1).sum() > 100      # the lines cover pixels
+    assert np.array_equal(color, ref_color), f"colour differs in {np.count_nonzero(color != ref_color)} bytes"
+    assert np.array_equal(depth, ref_ds), f"depth differs in {np.count_nonzero(depth != ref_ds)} bytes"
+
+
+def test_dynamic_blend_constants_and_depth_bounds(tmp_path):
+    """vkCmdSetBlendConstants / vkCmdSetDepthBounds between two draws of a
+    pipeline that declares both dynamic: the oracle draws the same with two
+    pipelines holding those values statically."""
+    size = 160
+    color, depth = _run_cube(tmp_path, size, "dynamic")
+    scene = W.cube_scene(width=size, height=size)
+    blend = dict(src_color=abi.BLEND["CONSTANT_COLOR"], dst_color=abi.BLEND["ONE_MINUS_CONSTANT_COLOR"], color_op=0,
+                 src_alpha=abi.BLEND["CONSTANT_ALPHA"], dst_alpha=abi.BLEND["ONE_MINUS_CONSTANT_ALPHA"], alpha_op=0)
+    pipes = []
+    for k, bounds in (((0.25, 0.5, 0.75, 1.0), (0.0, 1.0)), ((0.8, 0.1, 0.3, 0.5), (0.0, 0.995))):
+        p = dict(scene.pipeline, blend=blend, blend_constants=k, depth_bounds_test=1, depth_bounds=bounds)
+        pipes.append(p)
+    scene.pipelines = pipes
+    scene.draws = [scene.draws[0], copy.deepcopy(scene.draws[0])]
+    scene.draw_pipeline = [0, 1]
+    ref_color, ref_ds, _ = oracle_lib.render_scene(scene)
+    assert np.array_equal(color, ref_color), f"colour differs in {np.count_nonzero(color != ref_color)} bytes"
+    assert np.array_equal(depth, ref_ds), f"depth differs in {np.count_nonzero(depth != ref_ds)} bytes"
+
+
 def test_clear_attachments_inside_a_render_pass(tmp_path):
     """Draw, vkCmdClearAttachments (colour and depth rectangles), draw again:
     the oracle renders the cube, clears the rectangles, and renders the cube

@@ -63,7 +63,8 @@ def test_instance_and_physical_device(icd):
     for name in (b"vkCmdDraw", b"vkCmdDrawIndexed", b"vkQueueSubmit", b"vkCreateGraphicsPipelines", b"vkCmdDrawIndirect",
                  b"vkCmdDrawIndexedIndirect", b"vkCmdCopyBuffer", b"vkCmdCopyImage", b"vkCmdCopyBufferToImage",
                  b"vkCmdCopyImageToBuffer", b"vkCmdFillBuffer", b"vkCmdUpdateBuffer", b"vkCmdClearColorImage",
-                 b"vkCmdClearDepthStencilImage", b"vkCmdClearAttachments", b"vkCmdBlitImage"):
+                 b"vkCmdClearDepthStencilImage", b"vkCmdClearAttachments", b"vkCmdBlitImage", b"vkCmdSetDepthBounds",
+                 b"vkCmdSetLineWidth"):
         assert icd.vk_icdGetInstanceProcAddr(inst, name), name
     enum = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p))(
         icd.vk_icdGetInstanceProcAddr(inst, b"vkEnumeratePhysicalDevices"))
