@@ -1,6 +1,6 @@
 """Draw-path benchmark: frames/s of one workload on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload soup|cube|grid|skewed|jobs]
+    python bench.py [--steps K] [--warmup W] [--workload soup|cube|grid|skewed|jobs|transfers]
                     [--cpu-baseline full|sample|none] [--no-check]
 
 A step is one frame: the render pass of the workload (clear, vertex stage,
@@ -18,6 +18,11 @@ default workload is the 1920x1080, 1M-triangle alpha-blended soup
           square targets) recorded as one hipGraph batch after the
           autotuner picked the tile configuration; a step is one replay of
           all jobs.
+  transfers  the transfer and clear commands on 1 GiB buffers / images:
+          vkCmdFillBuffer, a buffer copy, a full R8G8B8A8 clear of a
+          16384x16384 image, a LINEAR blit of an 8192x8192 image; `value`
+          (GB/s) is the blit's algorithmic bytes (source read once,
+          destination written once) over its time.  A step is one of each.
 
 Before timing, the GPU frames are compared byte for byte with the CPU
 oracle's; the oracle's single-threaded wall time is the reported CPU
@@ -140,6 +145,85 @@ def cpu_entry(dt: float, mode: str, what: str) -> dict:
     note = f"{what} single-threaded, {dt:.2f} s" if mode == "full" else \
         f"{what}: first 50000 triangles per frame, time scaled by triangle count"
     return dict(value=round(1.0 / dt, 5), unit="frames/s", cores=1, kind="port", sample=f"{note}; {cpu_model()}")
+
+
+def bench_transfers(args, ctx):
+    import ctypes as C
+    import numpy as np
+    from cpvulkan_amd import abi
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+    rgba8 = abi.FORMAT["R8G8B8A8_UNORM"]
+    value = [int(np.float32(v).view(np.uint32)) for v in (0.3, 0.6, 0.9, 1.0)]
+    # correctness gate: a scaled, offset blit, a rectangle clear and an
+    # unaligned fill against the oracle
+    rng = np.random.Generator(np.random.PCG64(9))
+    SW, SH, DW, DH = 301, 203, 517, 389
+    src = rng.integers(0, 256, SW * SH * 4, dtype=np.uint8)
+    dst = rng.integers(0, 256, DW * DH * 4, dtype=np.uint8)
+    sa, da = ctx.alloc(src.nbytes), ctx.alloc(dst.nbytes)
+    ctx.upload(sa, src)
+    ctx.upload(da, dst)
+    reg = [abi.BlitRegion((C.c_int32 * 4)(0, 0, SW, SH), (C.c_int32 * 4)(3, 2, DW - 5, DH - 1), 1, 0)]
+    rect = [abi.Rect(7, 9, 200, 150)]
+    ctx.blit_image(abi.ImageLevel(sa, rgba8, SW, SH, 0), abi.ImageLevel(da, rgba8, DW, DH, 0), reg, 1)
+    ctx.clear_rects(abi.ImageLevel(da, rgba8, DW, DH, 0), value, rect)
+    ctx.fill_buffer(sa + 4, 10001, 0xDEADBEEF)
+    ctx.synchronize()
+    got_d, got_s = ctx.download(da, dst.nbytes), ctx.download(sa, src.nbytes)
+    want_d, want_s = dst.copy(), src.copy()
+    oracle_lib.blit_image(src.copy(), rgba8, SW, SH, want_d, rgba8, DW, DH, reg, 1)
+    oracle_lib.clear_rects(want_d, rgba8, DW, DH, value, rect)
+    oracle_lib.fill_buffer(want_s, 10001, 0xDEADBEEF, 4)
+    check = "bit-exact" if np.array_equal(got_d, want_d) and np.array_equal(got_s, want_s) else "MISMATCH"
+    ctx.free(sa)
+    ctx.free(da)
+    log(f"transfer check {check}")
+    if check != "bit-exact":
+        return None, dict(error="GPU transfers differ from the CPU oracle")
+    # timed, on 1 GiB buffers (launch overhead well under 1% of a command)
+    N = 1 << 30
+    a, b = ctx.alloc(N), ctx.alloc(N)
+    big = abi.ImageLevel(a, rgba8, 16384, 16384, 0)
+    bs, bd = abi.ImageLevel(a, rgba8, 8192, 8192, 0), abi.ImageLevel(b, rgba8, 8192, 8192, 0)
+    blit_reg = [abi.BlitRegion((C.c_int32 * 4)(0, 0, 8192, 8192), (C.c_int32 * 4)(8192, 0, 0, 8192), 1, 0)]
+    ops = dict(
+        fill=(lambda: ctx.fill_buffer(a, N, 0x01020304), N),
+        copy=(lambda: ctx.copy_regions([abi.CopyRegion(a, b, 0, 0, N, 1, 0)]), 2 * N),
+        clear=(lambda: ctx.clear_rects(big, value, [abi.Rect(0, 0, 16384, 16384)]), N),
+        blit=(lambda: ctx.blit_image(bs, bd, blit_reg, 1), 2 * 8192 * 8192 * 4),
+    )
+    rates, times = {}, {}
+    for name, (fn, nbytes) in ops.items():
+        for _ in range(args.warmup):
+            fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        times[name] = dt
+        rates[name] = round(nbytes / dt / 1e9, 1)
+    ctx.free(a)
+    ctx.free(b)
+    # CPU baseline: the oracle's LINEAR blit of a 1024x1024 image, per byte
+    cs = rng.integers(0, 256, 1024 * 1024 * 4, dtype=np.uint8)
+    cd = np.zeros_like(cs)
+    creg = [abi.BlitRegion((C.c_int32 * 4)(0, 0, 1024, 1024), (C.c_int32 * 4)(1024, 0, 0, 1024), 1, 0)]
+    t0 = time.perf_counter()
+    oracle_lib.blit_image(cs, rgba8, 1024, 1024, cd, rgba8, 1024, 1024, creg, 1)
+    cpu_dt = time.perf_counter() - t0
+    cpu = dict(value=round(2 * cs.nbytes / cpu_dt / 1e9, 4), unit="GB/s", cores=1, kind="port",
+               sample=f"LINEAR blit of one 1024x1024 R8G8B8A8 image single-threaded, {cpu_dt:.2f} s; {cpu_model()}")
+    blit_ms = times["blit"] * 1e3
+    out = dict(
+        metric="GB/s", unit="GB/s", value=rates["blit"], ms_per_step=round(sum(times.values()) * 1e3, 4),
+        config=dict(workload="transfers-1GiB", fill_bytes=N, copy_bytes=N, clear="16384x16384 R8G8B8A8_UNORM",
+                    blit="8192x8192 R8G8B8A8_UNORM LINEAR, mirrored in x"),
+        ops_gbs=rates, ops_ms={k: round(v * 1e3, 4) for k, v in times.items()},
+        roofline=roofline(blit_ms, 2 * 8192 * 8192 * 4), cpu_baseline=cpu, check=check)
+    return out, None
 
 
 def bench_single(args, ctx, scene, workload):
@@ -380,6 +464,8 @@ def main() -> int:
         return 0
     if args.workload == "jobs":
         out, err = bench_jobs(args, ctx)
+    elif args.workload == "transfers":
+        out, err = bench_transfers(args, ctx)
     else:
         scene, workload = make_scene(args.workload)
         log(f"workload {workload}: {scene.primitive_count} triangles, {scene.width}x{scene.height}")
@@ -390,7 +476,8 @@ def main() -> int:
     if err:
         print(json.dumps(dict(metric="frames/s", **err)), flush=True)
         return 1
-    line = dict(metric="frames/s", value=out.pop("value"), unit="frames/s", n_gpus=1, steps=args.steps,
+    metric, unit = out.pop("metric", "frames/s"), out.pop("unit", "frames/s")
+    line = dict(metric=metric, value=out.pop("value"), unit=unit, n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=out.pop("ms_per_step"), higher_is_better=True, scaling="weak",
                 vs_baseline=None, dtype="f32", data="synthetic", config=out.pop("config"))
     line.update(out)

@@ -451,6 +451,25 @@ struct EmitArgs {
 
 constexpr uint32_t kEmitBig = 2048;   // pairs above which a primitive goes to emit_big_kernel
 
+// Block bits of a pair value (kPairBlockShift): for 16x16 tiles, which
+// 8x8 blocks of tile (col, row) of the primitive's tile rectangle its pixel
+// box overlaps; box is relative to the rectangle's first tile.  0 for other
+// tile sizes.
+__device__ __forceinline__ uint32_t pair_blocks(int tile_shift, int4 box, uint32_t col, uint32_t row)
+{
+    if (tile_shift != 4) return 0u;
+    const int X = (int)col * 16, Y = (int)row * 16;
+    uint32_t m = 0u;
+#pragma unroll
+    for (int by = 0; by < 2; by++)
+#pragma unroll
+        for (int bx = 0; bx < 2; bx++) {
+            const int x0 = X + bx * 8, y0 = Y + by * 8;
+            if (box.x < x0 + 8 && box.z > x0 && box.y < y0 + 8 && box.w > y0) m |= 1u << (by * 2 + bx);
+        }
+    return m << kPairBlockShift;
+}
+
 // Writes the (tile, primitive) pairs of every primitive at its scanned
 // offset, so the pair array is in primitive order.  A workgroup takes 256
 // consecutive primitives, whose pairs are one contiguous run of the output
@@ -467,6 +486,7 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
     __shared__ uint32_t s_key[256];   // global tile index of each primitive's first tile
     __shared__ uint32_t s_w[256];     // tile columns of each primitive's rectangle
     __shared__ uint32_t s_tx[256];    // tile columns of each primitive's target
+    __shared__ int4 s_box[256];       // pixel box of each primitive, relative to its first tile
     const uint32_t nblk = (a.prims + 255u) / 256u;
     for (uint32_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
         const uint32_t gp = blk * 256u + threadIdx.x;
@@ -485,6 +505,8 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
             s_key[threadIdx.x] = t.tile_base + ty0 * t.tiles_x + tx0;
             s_w[threadIdx.x] = tx1 - tx0 + 1;
             s_tx[threadIdx.x] = t.tiles_x;
+            const int ox = (int)(tx0 << sh), oy = (int)(ty0 << sh);
+            s_box[threadIdx.x] = make_int4(r.x0 - ox, r.y0 - oy, r.x1 - ox, r.y1 - oy);
         }
         uint32_t total;
         s_off[threadIdx.x] = block_exclusive_scan(c, s_tmp, total);
@@ -502,8 +524,9 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
             const uint32_t row = k / w;
             const uint32_t out = s_dst[lo] + k;
             if (out < a.cap) {
-                a.keys[out] = s_key[lo] + row * s_tx[lo] + (k - row * w);
-                a.vals[out] = blk * 256u + lo;
+                const uint32_t col = k - row * w;
+                a.keys[out] = s_key[lo] + row * s_tx[lo] + col;
+                a.vals[out] = (blk * 256u + lo) | pair_blocks(a.tile_shift, s_box[lo], col, row);
             }
         }
         __syncthreads();
@@ -525,10 +548,12 @@ __global__ __launch_bounds__(256) void emit_big_kernel(EmitArgs a)
         const uint32_t ty0 = (uint32_t)r.y0 >> sh;
         const uint32_t w = tx1 - tx0 + 1;
         const uint32_t c = a.counts[gp], off = a.offsets[gp];
+        const int4 box = make_int4(r.x0 - (int)(tx0 << sh), r.y0 - (int)(ty0 << sh), r.x1 - (int)(tx0 << sh),
+                                   r.y1 - (int)(ty0 << sh));
         for (uint32_t k = threadIdx.x; k < c && off + k < a.cap; k += 256u) {
-            const uint32_t key = t.tile_base + (ty0 + k / w) * t.tiles_x + tx0 + k % w;
-            a.keys[off + k] = key;
-            a.vals[off + k] = gp;
+            const uint32_t row = k / w, col = k % w;
+            a.keys[off + k] = t.tile_base + (ty0 + row) * t.tiles_x + tx0 + col;
+            a.vals[off + k] = gp | pair_blocks(sh, box, col, row);
         }
     }
 }

@@ -90,6 +90,13 @@ static_assert(sizeof(TriSetup) == 128, "TriSetup must be 128 bytes");
 // in area, z = (p0.z, p1.z), w = (p0.w, p1.w) and v = (p0, p1).
 enum : uint32_t { kPrimFront = 1u, kPrimPoint = 2u, kPrimLine = 4u };
 
+// A (tile, primitive) pair's value: the primitive index in the low 28 bits
+// (launches hold fewer than 2^28 primitives) and, for 16x16 tiles, in bits
+// 28-31 which of the tile's four 8x8 blocks (bit by * 2 + bx) the
+// primitive's pixel box overlaps.
+constexpr uint32_t kPairPrim = 0x0FFFFFFFu;
+constexpr int kPairBlockShift = 28;
+
 // Raster kernel variants of a program: the tile configurations 8x1, 16x1,
 // 16x2, 16x4, 32x1, 32x2, 32x4, then 16x4 with point and line records.
 constexpr int kRasterVariants = 8;

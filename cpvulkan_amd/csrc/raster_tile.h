@@ -266,9 +266,15 @@ raster_kernel(RasterArgs a)
             uint32_t gp_l = 0;
             bool ov = false;
             if (lane < n) {
-                gp_l = a.tile_prims[base + lane];
-                const int4 bb = *(const int4*)&a.recs[gp_l].x0;
-                ov = !(bb.z <= wx0 || bb.x >= wx1 || bb.w <= wy0 || bb.y >= wy1);
+                const uint32_t e = a.tile_prims[base + lane];
+                gp_l = e & kPairPrim;
+                if (TILE == 16 && BPW == 1) {
+                    // the pair carries the blocks the primitive's box overlaps
+                    ov = ((e >> (kPairBlockShift + slot)) & 1u) != 0;
+                } else {
+                    const int4 bb = *(const int4*)&a.recs[gp_l].x0;
+                    ov = !(bb.z <= wx0 || bb.x >= wx1 || bb.w <= wy0 || bb.y >= wy1);
+                }
                 if (ov && a.hiz_op >= 0) {
                     const float2 dz = *(const float2*)&a.recs[gp_l].dt_lo;
                     ov = !hiz_rejects(a.hiz_op, dz.x, dz.y, bmin, bmax);

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void tile_features_kernel(FeatureArgs a)
     const uint32_t m = min(n_all, 64u);
     if ((uint32_t)lane < m) {
         const uint32_t i = b + (uint32_t)(((uint64_t)lane * n_all) / m);
-        const TriSetup& r = a.recs[a.tile_prims[i]];
+        const TriSetup& r = a.recs[a.tile_prims[i] & kPairPrim];
         const int cx0 = max(r.x0, x0), cy0 = max(r.y0, y0), cx1 = min(r.x1, x1), cy1 = min(r.y1, y1);
         const float box = (float)(max(cx1 - cx0, 0) * max(cy1 - cy0, 0));
         s_box += box;
