@@ -74,10 +74,20 @@ __global__ void clear_rect_kernel(ClearArgs a)
 {
     uint32_t words[4];
     pack_clear(a.fmt, a.value, words);
+    // rectangle pixels in row-major order, grid-strided; the stride's row
+    // and column steps are divided out once
     const uint64_t n = (uint64_t)a.w * a.h;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t x = (uint32_t)(a.x0 + (int32_t)(i % a.w)), y = (uint32_t)(a.y0 + (int32_t)(i / a.w));
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t rx = (uint32_t)(i % a.w), ry = (uint32_t)(i / a.w);
+    const uint32_t sx = (uint32_t)(step % a.w), sy = (uint32_t)(step / a.w);
+    for (; i < n; i += step) {
+        const uint32_t x = (uint32_t)a.x0 + rx, y = (uint32_t)a.y0 + ry;
         store_pixel(a.base + ((uint64_t)y * a.width + x) * (uint32_t)a.fmt.total, stored_bytes(a.fmt), words);
+        rx += sx;
+        ry += sy;
+        if (rx >= a.w) { rx -= a.w; ry++; }
     }
 }
 
@@ -101,7 +111,10 @@ CPV_DEV uint32_t raw_channel(const FormatDesc& f, const uint32_t px[4], int c)
 CPV_DEV void read_pixel(const uint8_t* p, int bytes, uint32_t w[4])
 {
     w[0] = w[1] = w[2] = w[3] = 0u;
-    for (int b = 0; b < bytes; b++) w[b >> 2] |= (uint32_t)p[b] << ((b & 3) * 8);
+    // unrolled over the largest pixel so w[] keeps constant indices (registers)
+#pragma unroll
+    for (int b = 0; b < 16; b++)
+        if (b < bytes) w[b >> 2] |= (uint32_t)p[b] << ((b & 3) * 8);
 }
 
 // BlitImageCommand::Process (CPVulkan/CommandBuffer.cpp:57-238): per
@@ -112,12 +125,19 @@ CPV_DEV void read_pixel(const uint8_t* p, int bytes, uint32_t w[4])
 // formats read the texel at the truncated (u, v).
 __global__ void blit_kernel(BlitArgs a)
 {
-    const uint64_t n = (uint64_t)a.w * a.h;
     const bool negx = a.d1x < a.d0x, negy = a.d1y < a.d0y;
     const float rx = (float)(a.s1x - a.s0x) / (float)(a.d1x - a.d0x);
     const float ry = (float)(a.s1y - a.s0y) / (float)(a.d1y - a.d0y);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const int32_t x = (int32_t)(i % a.w), y = (int32_t)(i / a.w);
+    // destination pixels in row-major order, grid-strided as clear_rect_kernel
+    const uint64_t n = (uint64_t)a.w * a.h;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t ux = (uint32_t)(i % a.w), uy = (uint32_t)(i / a.w);
+    const uint32_t sx = (uint32_t)(step % a.w), sy = (uint32_t)(step / a.w);
+    for (; i < n; i += step, ux += sx, uy += sy) {
+        if (ux >= a.w) { ux -= a.w; uy++; }
+        const int32_t x = (int32_t)ux, y = (int32_t)uy;
         const int32_t dx = negx ? x + a.d1x : x + a.d0x;
         const int32_t dy = negy ? y + a.d1y : y + a.d0y;
         const float u = ((float)dx + 0.5f - (float)a.d0x) * rx + (float)a.s0x;

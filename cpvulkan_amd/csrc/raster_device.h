@@ -231,7 +231,10 @@ CPV_DEV void texel(const TextureDesc& t, uint32_t level, int32_t x, int32_t y, f
     if (t.fmt.total == 4) {
         px[0] = *(const uint32_t*)p;
     } else {
-        for (int i = 0; i < t.fmt.total; i++) px[i >> 2] |= (uint32_t)p[i] << ((i & 3) * 8);
+        // unrolled over the largest pixel so px[] keeps constant indices (registers)
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (i < t.fmt.total) px[i >> 2] |= (uint32_t)p[i] << ((i & 3) * 8);
     }
     unpack_color(t.fmt, px, out);
 }

@@ -47,7 +47,10 @@ __device__ __forceinline__ void load_pixel(const uint8_t* p, int bytes, uint32_t
         const uint4 v = *(const uint4*)p;
         w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
     } else {
-        for (int b = 0; b < bytes; b++) w[b >> 2] |= (uint32_t)p[b] << ((b & 3) * 8);
+        // unrolled over the largest pixel so w[] keeps constant indices (registers)
+#pragma unroll
+        for (int b = 0; b < 16; b++)
+            if (b < bytes) w[b >> 2] |= (uint32_t)p[b] << ((b & 3) * 8);
     }
 }
 
@@ -62,7 +65,9 @@ __device__ __forceinline__ void store_pixel(uint8_t* p, int bytes, const uint32_
     } else if (bytes == 16) {
         *(uint4*)p = make_uint4(w[0], w[1], w[2], w[3]);
     } else {
-        for (int b = 0; b < bytes; b++) p[b] = (uint8_t)(w[b >> 2] >> ((b & 3) * 8));
+#pragma unroll
+        for (int b = 0; b < 16; b++)
+            if (b < bytes) p[b] = (uint8_t)(w[b >> 2] >> ((b & 3) * 8));
     }
 }
 
