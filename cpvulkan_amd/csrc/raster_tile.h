@@ -273,9 +273,15 @@ raster_kernel(RasterArgs a)
             if (lane < n) {
                 const uint32_t e = a.tile_prims[base + lane];
                 gp_l = e & kPairPrim;
-                if (TILE == 16 && BPW == 1) {
+                if (TILE == 8) {
+                    // one block per tile: a pair exists only where the box meets the tile
+                    ov = true;
+                } else if (TILE == 16) {
                     // the pair carries the blocks the primitive's box overlaps
-                    ov = ((e >> (kPairBlockShift + slot)) & 1u) != 0;
+                    uint32_t mine = 0u;
+#pragma unroll
+                    for (int k = 0; k < BPW; k++) mine |= 1u << (slot + k * WAVES);
+                    ov = ((e >> kPairBlockShift) & mine) != 0u;
                 } else {
                     const int4 bb = *(const int4*)&a.recs[gp_l].x0;
                     ov = !(bb.z <= wx0 || bb.x >= wx1 || bb.w <= wy0 || bb.y >= wy1);

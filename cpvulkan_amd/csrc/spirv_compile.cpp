@@ -19,14 +19,25 @@
 // with every float operation rounding once (the kernels are compiled with
 // contraction off and correctly rounded division and square root).
 //
-// Supported: logical addressing; float scalars, vectors, column-major
-// matrices (MatrixStride = rows * 4); uniform blocks and combined image
-// samplers of descriptor set 0, bindings 0-3; vertex inputs and
-// varyings by location; gl_PerVertex / Position, VertexIndex,
-// InstanceIndex, FragCoord, FrontFacing; structured or unstructured
-// control flow (labels become goto targets, OpPhi becomes assignments on
-// the incoming edges); OpKill.  Anything else is reported with its
-// opcode.
+// Supported: logical addressing; 32-bit float, int and uint scalars and
+// vectors, booleans, column-major float matrices, arrays and structs
+// (function variables, constants, OpConstantNull, specialization
+// constants at their defaults); uniform blocks of set 0, bindings 0-3, with
+// nested structs, arrays and matrices at their declared Offset /
+// ArrayStride / MatrixStride; combined image samplers of set 0, bindings
+// 0-3 (implicit and explicit-LOD sampling, OpImageFetch, size and level
+// queries); vertex inputs, float and Flat integer varyings by location,
+// Flat / NoPerspective interpolation; gl_PerVertex Position and
+// PointSize, VertexIndex, InstanceIndex, FragCoord, FrontFacing; integer
+// and float fragment outputs at location 0; function calls (each function
+// a lambda, callees first, no recursion); structured or unstructured
+// control flow (labels become goto targets, OpPhi assignments on the
+// incoming edges, OpSwitch); OpKill; the core arithmetic, bit, logical,
+// relational, conversion, composite and matrix instructions with LLVM's
+// integer semantics; GLSL.std.450 instructions 1-81 except Determinant,
+// MatrixInverse, Modf / ModfStruct, IMix, Frexp / FrexpStruct, the double
+// packing pair and InterpolateAt*.  Anything else is reported as
+// unsupported with what it was.
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
