@@ -100,7 +100,7 @@ struct BlitArgs {
     uint32_t w, h;          // |d1 - d0|
     uint32_t filter;
     uint32_t aspect;        // VkImageAspectFlags of the region
-    uint32_t src_kind;      // 0 float, 1 uint, 2 sint, 3 depth/stencil
+    uint32_t src_kind;      // 0 float, 1 uint, 2 sint, 3 depth/stencil, 4 8-bit UNORM x4 to 8-bit UNORM x4
 };
 
 CPV_DEV uint32_t raw_channel(const FormatDesc& f, const uint32_t px[4], int c)
@@ -123,8 +123,26 @@ CPV_DEV void read_pixel(const uint8_t* p, int bytes, uint32_t w[4])
 // (u / W, v / H) with the blit filter and CLAMP_TO_EDGE (SampleImage,
 // ImageSampler.cpp:676-681) and written with SetPixel; depth/stencil
 // formats read the texel at the truncated (u, v).
+// KIND is a.src_kind and FILTER a.filter, fixed per instantiation so each
+// carries only its own path; KIND 4 is a colour blit whose source and
+// destination formats both hold four 8-bit UNORM channels in 4 bytes, for
+// which the generic format code folds to fixed shifts (the same arithmetic).
+template <int KIND, int FILTER>
 __global__ void blit_kernel(BlitArgs a)
 {
+    TextureDesc src = a.src;
+    FormatDesc dfmt = a.dfmt;
+    if (KIND == 4) {
+        src.fmt.base = dfmt.base = kBaseUNorm;
+        src.fmt.element = dfmt.element = 1;
+        src.fmt.total = dfmt.total = 4;
+        src.fmt.depth_off = src.fmt.stencil_off = dfmt.depth_off = dfmt.stencil_off = -1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            __builtin_assume(src.fmt.off[k] >= 0 && src.fmt.off[k] < 4);
+            __builtin_assume(dfmt.off[k] >= 0 && dfmt.off[k] < 4);
+        }
+    }
     const bool negx = a.d1x < a.d0x, negy = a.d1y < a.d0y;
     const float rx = (float)(a.s1x - a.s0x) / (float)(a.d1x - a.d0x);
     const float ry = (float)(a.s1y - a.s0y) / (float)(a.d1y - a.d0y);
@@ -142,9 +160,9 @@ __global__ void blit_kernel(BlitArgs a)
         const int32_t dy = negy ? y + a.d1y : y + a.d0y;
         const float u = ((float)dx + 0.5f - (float)a.d0x) * rx + (float)a.s0x;
         const float v = ((float)dy + 0.5f - (float)a.d0y) * ry + (float)a.s0y;
-        uint8_t* out = a.dst + ((uint64_t)dy * a.dst_width + (uint64_t)dx) * a.dfmt.total;
+        uint8_t* out = a.dst + ((uint64_t)dy * a.dst_width + (uint64_t)dx) * dfmt.total;
         uint32_t px[4] = {0u, 0u, 0u, 0u};
-        if (a.src_kind == 3) {
+        if (KIND == 3) {
             const TextureDesc& t = a.src;
             // (uint32_t)u as GetDepthPixel receives it; held inside the level
             const uint32_t sx = min((uint32_t)fmaxf(u, 0.0f), t.level_w[0] - 1u), sy = min((uint32_t)fmaxf(v, 0.0f), t.level_h[0] - 1u);
@@ -162,14 +180,14 @@ __global__ void blit_kernel(BlitArgs a)
             pack_depth_stencil(a.dfmt, depth, stencil, d0, d1);
             px[0] = d0;
             px[1] = d1;
-        } else if (a.src_kind == 0) {
+        } else if (KIND == 0 || KIND == 4) {
             float c[4];
-            sample_level(a.src, 0, u / (float)a.src.level_w[0], v / (float)a.src.level_h[0], (int32_t)a.filter, c);
-            if (a.src.fmt.off[0] < 0) c[0] = 0.0f;
-            if (a.src.fmt.off[1] < 0) c[1] = 0.0f;
-            if (a.src.fmt.off[2] < 0) c[2] = 0.0f;
-            if (a.src.fmt.off[3] < 0) c[3] = 1.0f;
-            pack_color_f32(a.dfmt, c, px);
+            sample_level(src, 0, u / (float)src.level_w[0], v / (float)src.level_h[0], FILTER, c);
+            if (src.fmt.off[0] < 0) c[0] = 0.0f;
+            if (src.fmt.off[1] < 0) c[1] = 0.0f;
+            if (src.fmt.off[2] < 0) c[2] = 0.0f;
+            if (src.fmt.off[3] < 0) c[3] = 1.0f;
+            pack_color_f32(dfmt, c, px);
         } else {
             // integer formats: the nearest texel's channel values
             const TextureDesc& t = a.src;
@@ -182,7 +200,7 @@ __global__ void blit_kernel(BlitArgs a)
             uint32_t ch[4];
             for (int k = 0; k < 4; k++) ch[k] = raw_channel(t.fmt, sw, k);
             if (t.fmt.off[3] < 0) ch[3] = 1u;
-            if (a.src_kind == 1) {
+            if (KIND == 1) {
                 pack_color_u32(a.dfmt, ch, px);
             } else {
                 int32_t si[4];
@@ -193,7 +211,7 @@ __global__ void blit_kernel(BlitArgs a)
                 pack_color_i32(a.dfmt, si, px);
             }
         }
-        store_pixel(out, stored_bytes(a.dfmt), px);
+        store_pixel(out, stored_bytes(dfmt), px);
     }
 }
 
@@ -315,6 +333,11 @@ extern "C" int cpv_blit_image(cpv_context* c, const cpv_image_level* src, const 
     a.dst_width = dst->width;
     a.filter = filter;
     a.src_kind = sds ? 3u : sf.base == kBaseUInt ? 1u : sf.base == kBaseSInt ? 2u : 0u;
+    auto unorm8x4 = [](const FormatDesc& f) {
+        return f.base == kBaseUNorm && f.element == 1 && f.total == 4 && f.depth_off < 0 && f.stencil_off < 0 &&
+               f.off[0] >= 0 && f.off[1] >= 0 && f.off[2] >= 0 && f.off[3] >= 0;
+    };
+    if (a.src_kind == 0 && unorm8x4(sf) && unorm8x4(df)) a.src_kind = 4;
     for (uint32_t i = 0; i < count; i++) {
         const cpv_blit_region& r = regions[i];
         const int32_t w = std::abs(r.dst[2] - r.dst[0]), h = std::abs(r.dst[3] - r.dst[1]);
@@ -332,7 +355,12 @@ extern "C" int cpv_blit_image(cpv_context* c, const cpv_image_level* src, const 
         a.h = (uint32_t)h;
         a.aspect = r.aspect;
         void* args[] = {&a};
-        HIP_TRY(hipLaunchKernel((const void*)blit_kernel, dim3(grid_for((uint64_t)w * h)), dim3(256), args, 0, c->stream));
+        const void* fn = a.src_kind == 3 ? (const void*)blit_kernel<3, 0>
+                         : a.src_kind == 2 ? (const void*)blit_kernel<2, 0>
+                         : a.src_kind == 1 ? (const void*)blit_kernel<1, 0>
+                         : a.src_kind == 4 ? (filter ? (const void*)blit_kernel<4, 1> : (const void*)blit_kernel<4, 0>)
+                         : filter ? (const void*)blit_kernel<0, 1> : (const void*)blit_kernel<0, 0>;
+        HIP_TRY(hipLaunchKernel(fn, dim3(grid_for((uint64_t)w * h)), dim3(256), args, 0, c->stream));
     }
     return CPV_OK;
 }
