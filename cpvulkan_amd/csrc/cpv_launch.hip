@@ -168,6 +168,22 @@ int plan_launch(const cpv_config& cfg, const Program& prog, const std::vector<Ta
     bool any_prim = false;
     for (const DrawDesc& o : P->dd) any_prim = any_prim || o.topology < CPV_TOPOLOGY_TRIANGLE_LIST;
     P->raster_variant = any_prim ? kRasterAnyPrim : tile_config_index(cfg.tile_size, cfg.waves_per_tile);
+    // the fixed-format variant when folding the formats changes no pipeline
+    // or target of the launch
+    if (P->raster_variant == tile_config_index(16, 4) && !any_prim) {
+        bool fixed = true;
+        for (const FragmentState& f : P->fstates) {
+            FragmentState g = f;
+            fold_fixed_formats(g);
+            fixed = fixed && std::memcmp(&g, &f, sizeof(f)) == 0;
+        }
+        for (const TargetParams& t : P->targets) {
+            TargetParams g = t;
+            fold_fixed_formats(g);
+            fixed = fixed && std::memcmp(&g, &t, sizeof(t)) == 0;
+        }
+        if (fixed) P->raster_variant = kRasterFixedFormats;
+    }
     uint32_t tiles = 0;
     for (const TargetParams& t : targets) tiles = std::max(tiles, t.tile_base + t.tiles_x * t.tiles_y);
     P->tiles = tiles;

@@ -349,8 +349,9 @@ struct RasterTable {
             (const void*)&raster_kernel<FS, 8, 1>,  (const void*)&raster_kernel<FS, 16, 1>,
             (const void*)&raster_kernel<FS, 16, 2>, (const void*)&raster_kernel<FS, 16, 4>,
             (const void*)&raster_kernel<FS, 32, 1>, (const void*)&raster_kernel<FS, 32, 2>,
-            (const void*)&raster_kernel<FS, 32, 4>, (const void*)&raster_kernel<FS, 16, 4, true>};
-        static_assert(kRasterAnyPrim == 7, "the point/line variant is the last entry");
+            (const void*)&raster_kernel<FS, 32, 4>, (const void*)&raster_kernel<FS, 16, 4, true>,
+            (const void*)&raster_kernel<FS, 16, 4, false, true>};
+        static_assert(kRasterAnyPrim == 7 && kRasterFixedFormats == 8, "variant order");
         return fns[i];
     }
 };

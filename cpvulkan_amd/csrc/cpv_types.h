@@ -98,9 +98,12 @@ constexpr uint32_t kPairPrim = 0x0FFFFFFFu;
 constexpr int kPairBlockShift = 28;
 
 // Raster kernel variants of a program: the tile configurations 8x1, 16x1,
-// 16x2, 16x4, 32x1, 32x2, 32x4, then 16x4 with point and line records.
-constexpr int kRasterVariants = 8;
+// 16x2, 16x4, 32x1, 32x2, 32x4, then 16x4 with point and line records, then
+// 16x4 for launches whose every pipeline and target has the fixed formats
+// (fold_fixed_formats).
+constexpr int kRasterVariants = 9;
 constexpr int kRasterAnyPrim = 7;
+constexpr int kRasterFixedFormats = 8;
 
 // Per-primitive extension of a line record (only lines write it).
 struct LineExt {
@@ -142,6 +145,48 @@ struct TargetParams {
     uint32_t clear_ds[2];          // VkClearDepthStencilValue: depth bits, stencil
     FormatDesc color_fmt, ds_fmt;
 };
+
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+#define CPV_HOST_DEVICE __host__ __device__
+#else
+#define CPV_HOST_DEVICE
+#endif
+
+// The fixed formats: a colour attachment of four 8-bit UNORM channels in 4
+// bytes (any channel order) and a D32_SFLOAT depth attachment, no stencil
+// test.  Setting these fields to their values leaves such a state unchanged,
+// which the host checks before it picks the fixed-format raster variant; in
+// that variant the kernel sets them again, so the compiler sees constants
+// and folds the generic format code (same arithmetic).
+CPV_HOST_DEVICE inline void fold_fixed_formats(FormatDesc& c, FormatDesc& d)
+{
+    c.base = kBaseUNorm;
+    c.element = 1;
+    c.total = 4;
+    c.depth_off = c.stencil_off = -1;
+    c.depth_kind = 0;
+    d.vk_format = 126u;   // VK_FORMAT_D32_SFLOAT
+    d.base = kBaseSFloat;
+    d.element = 4;
+    d.total = 4;
+    d.off[0] = d.off[1] = d.off[2] = d.off[3] = -1;
+    d.depth_off = 0;
+    d.stencil_off = -1;
+    d.depth_kind = 32;
+    d.pad = 0;
+}
+CPV_HOST_DEVICE inline void fold_fixed_formats(FragmentState& s)
+{
+    fold_fixed_formats(s.color, s.ds);
+    s.has_color = s.has_ds = 1;
+    s.stencil_test = 0;
+}
+CPV_HOST_DEVICE inline void fold_fixed_formats(TargetParams& t)
+{
+    fold_fixed_formats(t.color_fmt, t.ds_fmt);
+    t.has_color = t.has_ds = 1;
+    t.color_bytes = t.ds_bytes = 4;
+}
 
 // Per-draw parameters.  Vertex-output slots and primitive indices of all
 // draws of one launch live in shared arrays; a draw owns the ranges

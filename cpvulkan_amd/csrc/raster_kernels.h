@@ -204,7 +204,7 @@ __device__ __forceinline__ void fragment_inputs(const Rec& g, const float* vout,
     }
 }
 
-template <class FS, bool ANYPRIM = false>
+template <class FS, bool ANYPRIM = false, bool FIXED = false>
 __device__ __forceinline__ PixelState shade_fragment(const TriSetup* recs, const DrawDesc* draws, const FragmentState* fstates,
                                                   const float* vout, uint32_t stride, uint32_t gprim, float w0, float w1,
                                                   float w2, int x, int y, PixelState ps)
@@ -223,7 +223,7 @@ __device__ __forceinline__ PixelState shade_fragment(const TriSetup* recs, const
     const float fc[4] = {fs.origin_upper_left ? (float)x : d.vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
     float colour[4];
     FS::run(de, in, fc, (g.flags & 1u) != 0, colour);
-    color_write(fs, ps, colour);
+    color_write_state<FIXED>(fs, ps, colour);
     return ps;
 }
 
@@ -234,12 +234,32 @@ struct DepthStencilResult {
 
 // depth_stencil_ops with the pixel state passed and returned by value (one
 // call site per raster kernel instantiation, inlined).
+template <bool FIXED = false>
 __device__ __forceinline__ DepthStencilResult depth_stencil_call(const FragmentState* st, PixelState px, float depth, uint32_t front)
 {
     DepthStencilResult r;
     r.px = px;
-    r.write = depth_stencil_ops(*st, r.px, depth, front != 0) ? 1u : 0u;
+    if constexpr (FIXED) {
+        FragmentState s = *st;
+        fold_fixed_formats(s);
+        r.write = depth_stencil_ops(s, r.px, depth, front != 0) ? 1u : 0u;
+    } else {
+        r.write = depth_stencil_ops(*st, r.px, depth, front != 0) ? 1u : 0u;
+    }
     return r;
+}
+
+// color_write with the fixed formats folded in when FIXED.
+template <bool FIXED>
+__device__ __forceinline__ void color_write_state(const FragmentState& st, PixelState& px, const float colour[4])
+{
+    if constexpr (FIXED) {
+        FragmentState s = st;
+        fold_fixed_formats(s);
+        color_write(s, px, colour);
+    } else {
+        color_write(st, px, colour);
+    }
 }
 
 __device__ __forceinline__ uint32_t find_target(const TargetParams* t, uint32_t n, uint32_t tile)

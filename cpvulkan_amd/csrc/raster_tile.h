@@ -96,7 +96,9 @@ constexpr int raster_waves_per_eu()
 
 // ANYPRIM: the launch may hold point and line records besides triangles
 // (one 16x16 / four-wave instantiation per fragment shader carries it).
-template <class FS, int TILE, int WAVES, bool ANYPRIM = false>
+// FIXED: every pipeline and target of the launch has the fixed formats
+// (fold_fixed_formats).
+template <class FS, int TILE, int WAVES, bool ANYPRIM = false, bool FIXED = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(raster_waves_per_eu<FS, TILE, WAVES>()))) void
 raster_kernel(RasterArgs a)
 {
@@ -223,7 +225,7 @@ raster_kernel(RasterArgs a)
 #pragma unroll
             for (int kk = 1; kk < BPW; kk++)
                 if (kk == k) { ps = px[kk]; sx = pxx[kk]; sy = pxy[kk]; }
-            ps = shade_fragment<FS, ANYPRIM>(a.recs, a.draws, a.fstates, a.vout, a.vout_stride, gprim, w0, w1, w2, sx, sy, ps);
+            ps = shade_fragment<FS, ANYPRIM, FIXED>(a.recs, a.draws, a.fstates, a.vout, a.vout_stride, gprim, w0, w1, w2, sx, sy, ps);
 #pragma unroll
             for (int kk = 0; kk < BPW; kk++)
                 if (kk == k) px[kk] = ps;
@@ -315,10 +317,15 @@ raster_kernel(RasterArgs a)
                     // Rejecting on the depth test alone is exact when nothing
                     // else observes the fragment: no depth-bounds or stencil
                     // test (and so no stencil write).
-                    early = fst->depth_test && fst->has_ds && fst->ds.depth_off >= 0 && fst->ds.vk_format != 127u &&
-                            !fst->depth_bounds_test && !fst->stencil_test;
+                    if constexpr (FIXED) {
+                        early = fst->depth_test && !fst->depth_bounds_test;
+                        depth_kind = 32;
+                    } else {
+                        early = fst->depth_test && fst->has_ds && fst->ds.depth_off >= 0 && fst->ds.vk_format != 127u &&
+                                !fst->depth_bounds_test && !fst->stencil_test;
+                        depth_kind = fst->ds.depth_kind;
+                    }
                     depth_op = fst->depth_compare;
-                    depth_kind = fst->ds.depth_kind;
                 }
 #pragma unroll
                 for (int k = 0; k < BPW; k++) {
@@ -366,7 +373,7 @@ raster_kernel(RasterArgs a)
                         if (e0 < 0 || e1 < 0 || e2 < 0) continue;
                         nfrag++;
                         if (early) {
-                            const float cur = depth_kind == 32 ? __uint_as_float(px[k].ds0) : unpack_depth(fst->ds, px[k].ds0);
+                            const float cur = (FIXED || depth_kind == 32) ? __uint_as_float(px[k].ds0) : unpack_depth(fst->ds, px[k].ds0);
                             if (vp_identity ? depth_surely_fails_unit(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2])
                                             : depth_surely_fails(depth_op, cur, e0, e1, e2, r.area, r.z[0], r.z[1], r.z[2], vp_min, vp_max))
                                 continue;
@@ -385,13 +392,13 @@ raster_kernel(RasterArgs a)
                                              depth, 1.0f};
                         float colour[4];
                         if (FS::run(*env, in, fc, front, colour)) continue;
-                        const DepthStencilResult dr = depth_stencil_call(fst, px[k], dt, front ? 1u : 0u);
+                        const DepthStencilResult dr = depth_stencil_call<FIXED>(fst, px[k], dt, front ? 1u : 0u);
                         px[k] = dr.px;
-                        if (dr.write && fst->has_color) color_write(*fst, px[k], colour);
+                        if (dr.write && (FIXED || fst->has_color)) color_write_state<FIXED>(*fst, px[k], colour);
                     } else {
-                        const DepthStencilResult dr = depth_stencil_call(fst, px[k], dt, front ? 1u : 0u);
+                        const DepthStencilResult dr = depth_stencil_call<FIXED>(fst, px[k], dt, front ? 1u : 0u);
                         px[k] = dr.px;
-                        if (!(dr.write && fst->has_color)) continue;
+                        if (!(dr.write && (FIXED || fst->has_color))) continue;
                         const uint32_t tag = gprim | ((uint32_t)k << 28);
 #pragma unroll
                         for (int e = 0; e < Q; e++) {
@@ -412,10 +419,10 @@ raster_kernel(RasterArgs a)
         for (int k = 0; k < BPW; k++) {
             if (!inside[k]) continue;
             const uint64_t pix = (uint64_t)pxy[k] * tp.width + (uint64_t)pxx[k];
-            if (tp.has_color) store_pixel((uint8_t*)tp.color_addr + pix * tp.color_bytes, tp.color_bytes, px[k].color);
+            if (tp.has_color) store_pixel((uint8_t*)tp.color_addr + pix * tp.color_bytes, FIXED ? 4 : tp.color_bytes, px[k].color);
             if (tp.has_ds) {
                 const uint32_t w4[4] = {px[k].ds0, px[k].ds1, 0u, 0u};
-                store_pixel((uint8_t*)tp.ds_addr + pix * tp.ds_bytes, tp.ds_bytes, w4);
+                store_pixel((uint8_t*)tp.ds_addr + pix * tp.ds_bytes, FIXED ? 4 : tp.ds_bytes, w4);
             }
         }
 
