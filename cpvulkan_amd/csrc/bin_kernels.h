@@ -591,6 +591,7 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t* keys, 
 // stable.
 constexpr int kSortSegments = 4096;
 constexpr int kSortWaves = 4;
+constexpr int kSortBatch = 4;   // 64-item rounds whose loads are issued together
 
 __device__ __forceinline__ void segment_range(uint32_t n, uint32_t seg, uint32_t& b, uint32_t& e)
 {
@@ -609,7 +610,18 @@ __global__ __launch_bounds__(64 * kSortWaves) void radix_hist_kernel(const uint3
     const uint32_t seg = blockIdx.x * kSortWaves + wave;
     uint32_t b, e;
     segment_range(*n_ptr, seg, b, e);
-    for (uint32_t i = b + lane; i < e; i += 64) atomicAdd(&s_cnt[wave][(keys[i] >> shift) & 255u], 1u);
+    // four rounds of keys loaded before they are counted, so the loads overlap
+    for (uint32_t i0 = b; i0 < e; i0 += 64u * kSortBatch) {
+        uint32_t ks[kSortBatch];
+#pragma unroll
+        for (int r = 0; r < kSortBatch; r++) {
+            const uint32_t i = i0 + (uint32_t)r * 64u + (uint32_t)lane;
+            ks[r] = i < e ? keys[i] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < kSortBatch; r++)
+            if (i0 + (uint32_t)r * 64u + (uint32_t)lane < e) atomicAdd(&s_cnt[wave][(ks[r] >> shift) & 255u], 1u);
+    }
     __syncthreads();
     for (int i = lane; i < 256; i += 64) hist[(uint32_t)i * kSortSegments + seg] = s_cnt[wave][i];
 }
@@ -628,30 +640,41 @@ __global__ __launch_bounds__(64 * kSortWaves) void radix_scatter_kernel(const ui
     const uint32_t e_total = *n_ptr;
     segment_range(e_total, seg, b, e);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint32_t i0 = b; i0 < e; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        const bool valid = i < e;
-        const uint32_t key = valid ? keys_in[i] : 0u;
-        const uint32_t val = valid ? vals_in[i] : 0u;
-        const uint32_t dg = (key >> shift) & 255u;
-        // lanes holding the same digit
-        uint64_t match = __ballot(valid);
+    for (uint32_t i0 = b; i0 < e; i0 += 64u * kSortBatch) {
+        // four rounds loaded before they are ranked, so the loads overlap
+        uint32_t ks[kSortBatch], vs[kSortBatch];
 #pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
-            const bool set = (dg >> bit) & 1u;
-            const uint64_t bal = __ballot(valid && set);
-            match &= set ? bal : ~bal;
+        for (int r = 0; r < kSortBatch; r++) {
+            const uint32_t i = i0 + (uint32_t)r * 64u + (uint32_t)lane;
+            ks[r] = i < e ? keys_in[i] : 0u;
+            vs[r] = i < e ? vals_in[i] : 0u;
         }
-        const uint32_t rank = (uint32_t)__popcll(match & lt);
-        const int leader = __ffsll((long long)match) - 1;
-        uint32_t run = 0;
-        if (valid) run = s_run[wave][dg];
-        __builtin_amdgcn_wave_barrier();
-        if (valid && lane == leader) s_run[wave][dg] = run + (uint32_t)__popcll(match);
-        __builtin_amdgcn_wave_barrier();
-        if (valid && run + rank < e_total) {
-            keys_out[run + rank] = key;
-            vals_out[run + rank] = val;
+#pragma unroll
+        for (int r = 0; r < kSortBatch; r++) {
+            const uint32_t i = i0 + (uint32_t)r * 64u + (uint32_t)lane;
+            const bool valid = i < e;
+            const uint32_t key = ks[r];
+            const uint32_t val = vs[r];
+            const uint32_t dg = (key >> shift) & 255u;
+            // lanes holding the same digit
+            uint64_t match = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 8; bit++) {
+                const bool set = (dg >> bit) & 1u;
+                const uint64_t bal = __ballot(valid && set);
+                match &= set ? bal : ~bal;
+            }
+            const uint32_t rank = (uint32_t)__popcll(match & lt);
+            const int leader = __ffsll((long long)match) - 1;
+            uint32_t run = 0;
+            if (valid) run = s_run[wave][dg];
+            __builtin_amdgcn_wave_barrier();
+            if (valid && lane == leader) s_run[wave][dg] = run + (uint32_t)__popcll(match);
+            __builtin_amdgcn_wave_barrier();
+            if (valid && run + rank < e_total) {
+                keys_out[run + rank] = key;
+                vals_out[run + rank] = val;
+            }
         }
     }
 }

@@ -170,6 +170,19 @@ __device__ __forceinline__ float interpolate_unit_w(const float w[3], float v0, 
 // blend (PipelineCompiler.cpp:1415-1727).  The raster kernel calls it from
 // one place (the colour queue's drain), so it is inlined: an out-of-line call
 // would pass the pixel state through scratch memory.
+// color_write with the fixed formats folded in when FIXED.
+template <bool FIXED>
+__device__ __forceinline__ void color_write_state(const FragmentState& st, PixelState& px, const float colour[4])
+{
+    if constexpr (FIXED) {
+        FragmentState s = st;
+        fold_fixed_formats(s);
+        color_write(s, px, colour);
+    } else {
+        color_write(st, px, colour);
+    }
+}
+
 // Fragment-shader inputs of one covered pixel of primitive g with weights
 // (w0, w1, w2): SetDatum over the three vertices of a triangle, over the two
 // of a line, and a point's vertex outputs as they are (ProcessPoints copies
@@ -249,18 +262,6 @@ __device__ __forceinline__ DepthStencilResult depth_stencil_call(const FragmentS
     return r;
 }
 
-// color_write with the fixed formats folded in when FIXED.
-template <bool FIXED>
-__device__ __forceinline__ void color_write_state(const FragmentState& st, PixelState& px, const float colour[4])
-{
-    if constexpr (FIXED) {
-        FragmentState s = st;
-        fold_fixed_formats(s);
-        color_write(s, px, colour);
-    } else {
-        color_write(st, px, colour);
-    }
-}
 
 __device__ __forceinline__ uint32_t find_target(const TargetParams* t, uint32_t n, uint32_t tile)
 {
