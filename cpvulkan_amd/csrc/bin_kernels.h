@@ -591,7 +591,7 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t* keys, 
 // stable.
 constexpr int kSortSegments = 4096;
 constexpr int kSortWaves = 4;
-constexpr int kSortBatch = 4;   // 64-item rounds whose loads are issued together
+constexpr int kSortBatch = 8;   // 64-item rounds whose loads are issued together
 
 __device__ __forceinline__ void segment_range(uint32_t n, uint32_t seg, uint32_t& b, uint32_t& e)
 {
@@ -610,7 +610,7 @@ __global__ __launch_bounds__(64 * kSortWaves) void radix_hist_kernel(const uint3
     const uint32_t seg = blockIdx.x * kSortWaves + wave;
     uint32_t b, e;
     segment_range(*n_ptr, seg, b, e);
-    // four rounds of keys loaded before they are counted, so the loads overlap
+    // kSortBatch rounds of keys loaded before they are counted, so the loads overlap
     for (uint32_t i0 = b; i0 < e; i0 += 64u * kSortBatch) {
         uint32_t ks[kSortBatch];
 #pragma unroll
@@ -641,7 +641,7 @@ __global__ __launch_bounds__(64 * kSortWaves) void radix_scatter_kernel(const ui
     segment_range(e_total, seg, b, e);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (uint32_t i0 = b; i0 < e; i0 += 64u * kSortBatch) {
-        // four rounds loaded before they are ranked, so the loads overlap
+        // kSortBatch rounds loaded before they are ranked, so the loads overlap
         uint32_t ks[kSortBatch], vs[kSortBatch];
 #pragma unroll
         for (int r = 0; r < kSortBatch; r++) {
