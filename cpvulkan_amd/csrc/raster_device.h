@@ -637,10 +637,7 @@ CPV_DEV void fetch_attribute(const DrawEnv& env, int location, int comps, uint32
         return;
     }
     uint32_t px[4] = {0u, 0u, 0u, 0u};
-    // unrolled over the largest pixel so px[] keeps constant indices (registers)
-#pragma unroll
-    for (int i = 0; i < 16; i++)
-        if (i < a.fmt.total) px[i >> 2] |= (uint32_t)p[i] << ((i & 3) * 8);
+    for (int i = 0; i < a.fmt.total; i++) px[i >> 2] |= (uint32_t)p[i] << ((i & 3) * 8);
     float v[4];
     unpack_color(a.fmt, px, v);
     for (int i = 0; i < comps; i++) out[i] = v[i];
