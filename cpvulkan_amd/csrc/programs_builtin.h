@@ -15,7 +15,7 @@ namespace cpv {
 // gl_Position = pos; out vec4 = color.
 struct VsPassthrough {
     static constexpr int kOutputs = 4;
-    __device__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)
+    __device__ __forceinline__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)
     {
         fetch_attribute(env, 0, 4, vid, iid, pos);
         fetch_attribute(env, 1, 4, vid, iid, out);
@@ -27,7 +27,7 @@ struct VsPassthrough {
 struct VsPassthroughPoint {
     static constexpr int kOutputs = 5;
     static constexpr int kPointSize = 4;
-    __device__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)
+    __device__ __forceinline__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)
     {
         fetch_attribute(env, 0, 4, vid, iid, pos);
         fetch_attribute(env, 1, 4, vid, iid, out);
@@ -38,7 +38,7 @@ struct VsPassthroughPoint {
 // Samples/15-draw_cube/15-draw_cube.cpp:38-51: gl_Position = mvp * pos.
 struct VsMvpColor {
     static constexpr int kOutputs = 4;
-    __device__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)
+    __device__ __forceinline__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)
     {
         float p[4];
         fetch_attribute(env, 0, 4, vid, iid, p);
@@ -50,7 +50,7 @@ struct VsMvpColor {
 // in vec3 pos @0, vec3 normal @1, vec2 uv @2: gl_Position = mvp * vec4(pos, 1).
 struct VsMvpLambert {
     static constexpr int kOutputs = 5;
-    __device__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)
+    __device__ __forceinline__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)
     {
         float p[4];
         fetch_attribute(env, 0, 3, vid, iid, p);
@@ -67,7 +67,7 @@ struct FsColor {
     static constexpr bool kMayDiscard = false;
     static constexpr int kWavesPerEU = 6;
     __device__ static int interp(int) { return 0; }
-    __device__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
+    __device__ __forceinline__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
     {
         out[0] = in[0]; out[1] = in[1]; out[2] = in[2]; out[3] = in[3];
         return false;
@@ -81,7 +81,7 @@ struct FsColorInterp {
     static constexpr bool kMayDiscard = false;
     static constexpr int kWavesPerEU = 6;
     __device__ static int interp(int) { return MODE; }
-    __device__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
+    __device__ __forceinline__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
     {
         out[0] = in[0]; out[1] = in[1]; out[2] = in[2]; out[3] = in[3];
         return false;
@@ -96,7 +96,7 @@ struct FsColorDiscard {
     static constexpr bool kMayDiscard = true;
     static constexpr int kWavesPerEU = 6;
     __device__ static int interp(int) { return 0; }
-    __device__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
+    __device__ __forceinline__ static bool run(const DrawEnv&, const float* in, const float*, bool, float out[4])
     {
         if (in[0] < 0.25f) return true;
         out[0] = in[0]; out[1] = in[1]; out[2] = in[2]; out[3] = in[3];
@@ -109,7 +109,7 @@ struct FsTexLambert {
     static constexpr int kInputs = 5;
     static constexpr bool kMayDiscard = false;
     __device__ static int interp(int) { return 0; }
-    __device__ static bool run(const DrawEnv& env, const float* in, const float*, bool, float out[4])
+    __device__ __forceinline__ static bool run(const DrawEnv& env, const float* in, const float*, bool, float out[4])
     {
         float t[4];
         sample_implicit(env.tex[1], in[3], in[4], t);

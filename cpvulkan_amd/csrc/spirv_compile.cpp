@@ -1582,7 +1582,7 @@ Translation translate(const uint32_t* vs, size_t vs_n, const char* vs_entry, con
     std::ostringstream s;
     s << kPrelude;
     s << "struct SpvVs {\n    static constexpr int kOutputs = " << gv.out_total << ";\n" << gv.types.str()
-      << "    __device__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)\n    {\n"
+      << "    __device__ __forceinline__ static void run(const DrawEnv& env, uint32_t vid, uint32_t iid, float pos[4], float* out)\n    {\n"
       << gv.decl.str() << gv.fns.str() << gv.pro.str() << gv.code.str() << "L_end:;\n" << gv.epi.str()
       << "    (void)env; (void)vid; (void)iid; (void)out; (void)pos;\n    }\n};\n";
     const int fin = (int)gf.interp.size();
@@ -1592,7 +1592,7 @@ Translation translate(const uint32_t* vs, size_t vs_n, const char* vs_entry, con
     for (int c = 0; c < fin; c++)
         if (gf.interp[c]) s << "        if (c == " << c << ") return " << gf.interp[c] << ";\n";
     s << "        (void)c;\n        return 0;\n    }\n"
-      << "    __device__ static bool run(const DrawEnv& env, const float* in, const float* fc, bool front, float out[4])\n    {\n"
+      << "    __device__ __forceinline__ static bool run(const DrawEnv& env, const float* in, const float* fc, bool front, float out[4])\n    {\n"
       << gf.decl.str() << gf.fns.str() << gf.pro.str() << gf.code.str() << "L_end:;\n" << gf.epi.str()
       << "    (void)env; (void)in; (void)fc; (void)front;\n        return false;\n    }\n};\n";
     s << "}  // namespace cpv\n";
