@@ -97,6 +97,11 @@ struct cpv_context {
 namespace cpvrt {
 
 bool format_desc(uint32_t f, cpv::FormatDesc* o);
+
+// Host address of raster kernel variant i of built-in fragment shader FS
+// (instantiated by raster_instances.hip, one translation unit per shader).
+template <class FS>
+const void* raster_table(int i);
 int tile_config_index(uint32_t t, uint32_t w);
 bool valid_tile_config(uint32_t t, uint32_t w);
 bool builtin_program(uint32_t vs, uint32_t fs, Program* p);

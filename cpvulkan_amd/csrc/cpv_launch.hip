@@ -170,7 +170,7 @@ int plan_launch(const cpv_config& cfg, const Program& prog, const std::vector<Ta
     P->raster_variant = any_prim ? kRasterAnyPrim : tile_config_index(cfg.tile_size, cfg.waves_per_tile);
     // the fixed-format variant when folding the formats changes no pipeline
     // or target of the launch
-    if (P->raster_variant == tile_config_index(16, 4) && !any_prim) {
+    if (!any_prim) {
         bool fixed = true;
         for (const FragmentState& f : P->fstates) {
             FragmentState g = f;
@@ -182,7 +182,7 @@ int plan_launch(const cpv_config& cfg, const Program& prog, const std::vector<Ta
             fold_fixed_formats(g);
             fixed = fixed && std::memcmp(&g, &t, sizeof(t)) == 0;
         }
-        if (fixed) P->raster_variant = kRasterFixedFormats;
+        if (fixed) P->raster_variant = kRasterFixedFormats + tile_config_index(cfg.tile_size, cfg.waves_per_tile);
     }
     uint32_t tiles = 0;
     for (const TargetParams& t : targets) tiles = std::max(tiles, t.tile_base + t.tiles_x * t.tiles_y);

@@ -15,9 +15,9 @@ struct KernelRef {
 struct cpv_program {
     hipModule_t module = nullptr;
     KernelRef vertex;
-    KernelRef raster[9];                  // tile configurations 8x1, 16x1, 16x2, 16x4, 32x1, 32x2, 32x4,
-                                          // then 16x4 with point and line records, then 16x4 for
-                                          // the fixed formats
+    KernelRef raster[15];                 // tile configurations 8x1, 16x1, 16x2, 16x4, 32x1, 32x2, 32x4,
+                                          // then 16x4 with point and line records, then the seven
+                                          // configurations for the fixed formats
     int vs_outputs = 0;
     int psize = -1;                       // vertex output component holding gl_PointSize, -1: none
     int fs_inputs = 0;

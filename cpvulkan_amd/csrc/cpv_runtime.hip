@@ -341,22 +341,6 @@ int tile_config_index(uint32_t t, uint32_t w)
     return w == 1 ? 4 : w == 2 ? 5 : 6;
 }
 
-template <class FS>
-struct RasterTable {
-    static const void* get(int i)
-    {
-        static const void* fns[kRasterVariants] = {
-            (const void*)&raster_kernel<FS, 8, 1>,  (const void*)&raster_kernel<FS, 16, 1>,
-            (const void*)&raster_kernel<FS, 16, 2>, (const void*)&raster_kernel<FS, 16, 4>,
-            (const void*)&raster_kernel<FS, 32, 1>, (const void*)&raster_kernel<FS, 32, 2>,
-            (const void*)&raster_kernel<FS, 32, 4>, (const void*)&raster_kernel<FS, 16, 4, true>,
-            (const void*)&raster_kernel<FS, 16, 4, false, true>};
-        static_assert(kRasterAnyPrim == 7 && kRasterFixedFormats == 8, "variant order");
-        return fns[i];
-    }
-};
-
-
 bool builtin_program(uint32_t vs, uint32_t fs, Program* p)
 {
     switch (vs) {
@@ -372,23 +356,23 @@ bool builtin_program(uint32_t vs, uint32_t fs, Program* p)
     }
     switch (fs) {
     case CPV_FS_COLOR:
-        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsColor>::get(i);
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = raster_table<FsColor>(i);
         p->fs_inputs = FsColor::kInputs;
         break;
     case CPV_FS_TEX_LAMBERT:
-        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsTexLambert>::get(i);
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = raster_table<FsTexLambert>(i);
         p->fs_inputs = FsTexLambert::kInputs;
         break;
     case CPV_FS_COLOR_DISCARD:
-        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsColorDiscard>::get(i);
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = raster_table<FsColorDiscard>(i);
         p->fs_inputs = FsColorDiscard::kInputs;
         break;
     case CPV_FS_COLOR_FLAT:
-        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsColorFlat>::get(i);
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = raster_table<FsColorFlat>(i);
         p->fs_inputs = FsColorFlat::kInputs;
         break;
     case CPV_FS_COLOR_NOPERSPECTIVE:
-        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = RasterTable<FsColorNoPerspective>::get(i);
+        for (int i = 0; i < kRasterVariants; i++) p->raster[i].host_fn = raster_table<FsColorNoPerspective>(i);
         p->fs_inputs = FsColorNoPerspective::kInputs;
         break;
     default: return false;

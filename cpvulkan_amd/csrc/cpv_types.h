@@ -99,11 +99,11 @@ constexpr int kPairBlockShift = 28;
 
 // Raster kernel variants of a program: the tile configurations 8x1, 16x1,
 // 16x2, 16x4, 32x1, 32x2, 32x4, then 16x4 with point and line records, then
-// 16x4 for launches whose every pipeline and target has the fixed formats
-// (fold_fixed_formats).
-constexpr int kRasterVariants = 9;
+// the seven configurations again for launches whose every pipeline and
+// target has the fixed formats (fold_fixed_formats).
+constexpr int kRasterVariants = 15;
 constexpr int kRasterAnyPrim = 7;
-constexpr int kRasterFixedFormats = 8;
+constexpr int kRasterFixedFormats = 8;   // + tile configuration index
 
 // Per-primitive extension of a line record (only lines write it).
 struct LineExt {

@@ -1554,11 +1554,13 @@ __device__ __forceinline__ V<float, 2> spv_unpack_half2x16(uint32_t p)
 
 // Template arguments of the raster kernel variants in Program::raster order:
 // the seven tile configurations, then the 16x16 / four-wave kernel that
-// also rasterises point and line records, then the 16x16 / four-wave
-// kernel for the fixed formats.
+// also rasterises point and line records, then the seven configurations
+// for the fixed formats.
 std::vector<std::string> raster_instances()
 {
-    return {"8, 1", "16, 1", "16, 2", "16, 4", "32, 1", "32, 2", "32, 4", "16, 4, true", "16, 4, false, true"};
+    return {"8, 1",  "16, 1", "16, 2", "16, 4", "32, 1", "32, 2", "32, 4", "16, 4, true",
+            "8, 1, false, true",  "16, 1, false, true", "16, 2, false, true", "16, 4, false, true",
+            "32, 1, false, true", "32, 2, false, true", "32, 4, false, true"};
 }
 
 struct Translation {

@@ -14,8 +14,8 @@ def test_translation_of_the_cube_shaders():
     assert "mat4_mul_vec4(" in src and "env.uniform[0]" in src
     assert "static constexpr int kOutputs = 4;" in src and "static constexpr int kInputs = 4;" in src
     assert "kMayDiscard = false" in src
-    # 7 tile configurations, points/lines, fixed formats
-    assert src.count("template __global__ void cpv::raster_kernel<cpv::SpvFs") == 9
+    # 7 tile configurations, points/lines, the 7 again for the fixed formats
+    assert src.count("template __global__ void cpv::raster_kernel<cpv::SpvFs") == 15
 
 
 def test_translation_of_point_size():
