@@ -470,6 +470,52 @@ __device__ __forceinline__ uint32_t pair_blocks(int tile_shift, int4 box, uint32
     return m << kPairBlockShift;
 }
 
+// Whether a pixel of [px0, px1) x [py0, py1) (target pixels, non-empty) can
+// pass all three edge tests of triangle record r, evaluated exactly as the
+// raster kernel evaluates them: the pixel centre ((x / vp_w + half) * 2 - 1,
+// GetFragmentInput / ProcessTriangles, CommandBuffer.Draw.cpp:1571-1580),
+// then (x - a.x) * dy - (y - a.y) * dx, fp32 without contraction.  Each
+// rounded step is monotone in x alone or in y alone, so the largest rounded
+// edge value over the rectangle is taken at one of its four corners: when
+// it is negative (and no corner gives NaN) no pixel of the rectangle is
+// covered.
+__device__ __forceinline__ bool rect_may_cover(const TriSetup& r, const DrawDesc& d, int px0, int py0, int px1, int py1)
+{
+    const float xs[2] = {((float)px0 / d.vp_w + d.half_x) * 2.0f - 1.0f, ((float)(px1 - 1) / d.vp_w + d.half_x) * 2.0f - 1.0f};
+    const float ys[2] = {((float)py0 / d.vp_h + d.half_y) * 2.0f - 1.0f, ((float)(py1 - 1) / d.vp_h + d.half_y) * 2.0f - 1.0f};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        bool neg = true;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const float e = (xs[c & 1] - r.ax[i]) * r.dy[i] - (ys[c >> 1] - r.ay[i]) * r.dx[i];
+            neg = neg && e < 0.0f;   // NaN is not < 0: no cull
+        }
+        if (neg) return false;
+    }
+    return true;
+}
+
+// pair_blocks less the blocks where triangle r covers no pixel (points and
+// lines keep their box's blocks).  box is relative to the primitive's first
+// tile, whose pixel origin is (r.x0 - box.x, r.y0 - box.y).
+__device__ __forceinline__ uint32_t pair_blocks_covered(int tile_shift, int4 box, uint32_t col, uint32_t row, const TriSetup& r,
+                                                        const DrawDesc& d)
+{
+    const uint32_t m = pair_blocks(tile_shift, box, col, row);
+    if (m == 0u || (r.flags & (kPrimPoint | kPrimLine))) return m;
+    const int ox = r.x0 - box.x, oy = r.y0 - box.y;
+    uint32_t keep = 0u;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        if (!((m >> (kPairBlockShift + b)) & 1u)) continue;
+        const int x0 = (int)col * 16 + (b & 1) * 8, y0 = (int)row * 16 + (b >> 1) * 8;
+        const int px0 = max(x0, box.x), px1 = min(x0 + 8, box.z), py0 = max(y0, box.y), py1 = min(y0 + 8, box.w);
+        if (rect_may_cover(r, d, ox + px0, oy + py0, ox + px1, oy + py1)) keep |= 1u << (kPairBlockShift + b);
+    }
+    return keep;
+}
+
 // Writes the (tile, primitive) pairs of every primitive at its scanned
 // offset, so the pair array is in primitive order.  A workgroup takes 256
 // consecutive primitives, whose pairs are one contiguous run of the output
@@ -526,7 +572,9 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
             if (out < a.cap) {
                 const uint32_t col = k - row * w;
                 a.keys[out] = s_key[lo] + row * s_tx[lo] + col;
-                a.vals[out] = (blk * 256u + lo) | pair_blocks(a.tile_shift, s_box[lo], col, row);
+                const uint32_t gp = blk * 256u + lo;
+                const TriSetup& r = a.recs[gp];
+                a.vals[out] = gp | pair_blocks_covered(a.tile_shift, s_box[lo], col, row, r, a.draws[r.draw]);
             }
         }
         __syncthreads();
@@ -553,7 +601,7 @@ __global__ __launch_bounds__(256) void emit_big_kernel(EmitArgs a)
         for (uint32_t k = threadIdx.x; k < c && off + k < a.cap; k += 256u) {
             const uint32_t row = k / w, col = k % w;
             a.keys[off + k] = t.tile_base + (ty0 + row) * t.tiles_x + tx0 + col;
-            a.vals[off + k] = gp | pair_blocks(sh, box, col, row);
+            a.vals[off + k] = gp | pair_blocks_covered(sh, box, col, row, r, a.draws[r.draw]);
         }
     }
 }

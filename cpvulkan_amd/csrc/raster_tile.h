@@ -296,20 +296,21 @@ raster_kernel(RasterArgs a)
                 }
             }
             // each lane tests one primitive of the chunk against the wave's area
-            uint32_t gp_l = 0;
+            uint32_t gp_l = 0, blocks_l = 0;
             bool ov = false;
             if (lane < n) {
                 const uint32_t e = a.tile_prims[base + lane];
                 gp_l = e & kPairPrim;
+                blocks_l = e >> kPairBlockShift;
                 if (TILE == 8) {
                     // one block per tile: a pair exists only where the box meets the tile
                     ov = true;
                 } else if (TILE == 16) {
-                    // the pair carries the blocks the primitive's box overlaps
+                    // the pair carries the blocks the primitive may cover
                     uint32_t mine = 0u;
 #pragma unroll
                     for (int k = 0; k < BPW; k++) mine |= 1u << (slot + k * WAVES);
-                    ov = ((e >> kPairBlockShift) & mine) != 0u;
+                    ov = (blocks_l & mine) != 0u;
                 } else {
                     const int4 bb = *(const int4*)&a.recs[gp_l].x0;
                     ov = !(bb.z <= wx0 || bb.x >= wx1 || bb.w <= wy0 || bb.y >= wy1);
@@ -337,6 +338,7 @@ raster_kernel(RasterArgs a)
                 const int j = (int)__builtin_ctzll(mask);
                 mask &= mask - 1;
                 const uint32_t gprim = __builtin_amdgcn_readlane(gp_l, j);
+                const uint32_t blocks = (TILE == 16 && BPW > 1) ? __builtin_amdgcn_readlane(blocks_l, j) : 0u;
                 TriSetup r;
                 if constexpr (STAGE) {
                     uint32_t* dst = (uint32_t*)&r;
@@ -387,7 +389,11 @@ raster_kernel(RasterArgs a)
                     if (!FS::kMayDiscard && __any(qn == Q)) drain_one();
                     const int blk = slot + k * WAVES;
                     const int bx0 = tile_x0 + (blk % BW) * 8, by0 = tile_y0 + (blk / BW) * 8;
-                    if (BPW > 1 && (r.x1 <= bx0 || r.x0 >= bx0 + 8 || r.y1 <= by0 || r.y0 >= by0 + 8)) continue;   // wave-uniform
+                    if (TILE == 16 && BPW > 1) {
+                        if (!((blocks >> blk) & 1u)) continue;   // wave-uniform
+                    } else if (BPW > 1 && (r.x1 <= bx0 || r.x0 >= bx0 + 8 || r.y1 <= by0 || r.y0 >= by0 + 8)) {
+                        continue;   // wave-uniform
+                    }
                     const int x = pxx[k], y = pxy[k];
                     if (x < r.x0 || x >= r.x1 || y < r.y0 || y >= r.y1) continue;
                     if (a.hiz_op >= 0) {
