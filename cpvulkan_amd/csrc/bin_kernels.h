@@ -470,19 +470,22 @@ __device__ __forceinline__ uint32_t pair_blocks(int tile_shift, int4 box, uint32
     return m << kPairBlockShift;
 }
 
-// Whether a pixel of [px0, px1) x [py0, py1) (target pixels, non-empty) can
-// pass all three edge tests of triangle record r, evaluated exactly as the
-// raster kernel evaluates them: the pixel centre ((x / vp_w + half) * 2 - 1,
-// GetFragmentInput / ProcessTriangles, CommandBuffer.Draw.cpp:1571-1580),
-// then (x - a.x) * dy - (y - a.y) * dx, fp32 without contraction.  Each
-// rounded step is monotone in x alone or in y alone, so the largest rounded
-// edge value over the rectangle is taken at one of its four corners: when
-// it is negative (and no corner gives NaN) no pixel of the rectangle is
-// covered.
-__device__ __forceinline__ bool rect_may_cover(const TriSetup& r, const DrawDesc& d, int px0, int py0, int px1, int py1)
+// Pixel centre in NDC as the raster kernel computes it (ProcessTriangles,
+// CommandBuffer.Draw.cpp:1571-1580).
+__device__ __forceinline__ float pixel_centre(int p, float extent, float half)
 {
-    const float xs[2] = {((float)px0 / d.vp_w + d.half_x) * 2.0f - 1.0f, ((float)(px1 - 1) / d.vp_w + d.half_x) * 2.0f - 1.0f};
-    const float ys[2] = {((float)py0 / d.vp_h + d.half_y) * 2.0f - 1.0f, ((float)(py1 - 1) / d.vp_h + d.half_y) * 2.0f - 1.0f};
+    return ((float)p / extent + half) * 2.0f - 1.0f;
+}
+
+// Whether a pixel of the rectangle with corner centres xs x ys can pass all
+// three edge tests of triangle record r, evaluated exactly as the raster
+// kernel evaluates them: (x - a.x) * dy - (y - a.y) * dx, fp32 without
+// contraction, at the centres pixel_centre gives.  Each rounded step is
+// monotone in x alone or in y alone, so the largest rounded edge value over
+// the rectangle is taken at one of its four corners: when it is negative
+// (and no corner gives NaN) no pixel of the rectangle is covered.
+__device__ __forceinline__ bool rect_may_cover(const TriSetup& r, const float xs[2], const float ys[2])
+{
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         bool neg = true;
@@ -498,21 +501,34 @@ __device__ __forceinline__ bool rect_may_cover(const TriSetup& r, const DrawDesc
 
 // pair_blocks less the blocks where triangle r covers no pixel (points and
 // lines keep their box's blocks).  box is relative to the primitive's first
-// tile, whose pixel origin is (r.x0 - box.x, r.y0 - box.y).
+// tile, whose pixel origin is (r.x0 - box.x, r.y0 - box.y).  The corner
+// centres of the tile's two block columns and rows are computed once.
 __device__ __forceinline__ uint32_t pair_blocks_covered(int tile_shift, int4 box, uint32_t col, uint32_t row, const TriSetup& r,
                                                         const DrawDesc& d)
 {
     const uint32_t m = pair_blocks(tile_shift, box, col, row);
     if (m == 0u || (r.flags & (kPrimPoint | kPrimLine))) return m;
     const int ox = r.x0 - box.x, oy = r.y0 - box.y;
+    const int X = (int)col * 16, Y = (int)row * 16;
+    const uint32_t bits = m >> kPairBlockShift;
+    float xs[2][2], ys[2][2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        // block column h is in the box when a block of it is (bits h, h + 2)
+        if (bits & (5u << h)) {
+            xs[h][0] = pixel_centre(ox + max(X + 8 * h, box.x), d.vp_w, d.half_x);
+            xs[h][1] = pixel_centre(ox + min(X + 8 * h + 8, box.z) - 1, d.vp_w, d.half_x);
+        }
+        // block row h (bits 2h, 2h + 1)
+        if (bits & (3u << (2 * h))) {
+            ys[h][0] = pixel_centre(oy + max(Y + 8 * h, box.y), d.vp_h, d.half_y);
+            ys[h][1] = pixel_centre(oy + min(Y + 8 * h + 8, box.w) - 1, d.vp_h, d.half_y);
+        }
+    }
     uint32_t keep = 0u;
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-        if (!((m >> (kPairBlockShift + b)) & 1u)) continue;
-        const int x0 = (int)col * 16 + (b & 1) * 8, y0 = (int)row * 16 + (b >> 1) * 8;
-        const int px0 = max(x0, box.x), px1 = min(x0 + 8, box.z), py0 = max(y0, box.y), py1 = min(y0 + 8, box.w);
-        if (rect_may_cover(r, d, ox + px0, oy + py0, ox + px1, oy + py1)) keep |= 1u << (kPairBlockShift + b);
-    }
+    for (int b = 0; b < 4; b++)
+        if (((bits >> b) & 1u) && rect_may_cover(r, xs[b & 1], ys[b >> 1])) keep |= 1u << (kPairBlockShift + b);
     return keep;
 }
 
