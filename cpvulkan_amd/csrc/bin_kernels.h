@@ -632,17 +632,32 @@ __global__ void clamp_pairs_kernel(const uint32_t* total, uint32_t cap, uint32_t
     if (t > cap) *overflow = 1u;
 }
 
-// Start of every tile's range in the tile-sorted pair array: each boundary
-// between different keys writes the starts of the tiles it spans.  The grid
-// covers the capacity; the pair count is read on the device.
+// First index in [lo, hi) of the ascending keys whose tile is not before
+// t; hi when there is none.
+__device__ __forceinline__ uint32_t lower_bound_key(const uint32_t* keys, uint32_t lo, uint32_t hi, uint32_t t)
+{
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] < t) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Start of every tile's range in the tile-sorted pair array, start[t] =
+// the first pair whose tile is not before t (start[tiles] = pair count).
+// A workgroup takes 256 consecutive tiles: two searches over all pairs bound
+// its tiles' ranges, and each thread searches only between them (empty
+// tiles cost nothing, however many a sparse frame has).  The pair count is
+// read on the device.
 __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t* keys, const uint32_t* n_ptr, uint32_t tiles, uint32_t* start)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    __shared__ uint32_t s_lim[2];
+    const uint32_t t0 = blockIdx.x * 256u;
     const uint32_t n = *n_ptr;
-    if (i > n) return;
-    const uint32_t prev = i == 0 ? 0u : keys[i - 1] + 1u;   // first tile not before this boundary
-    const uint32_t cur = i == n ? tiles : keys[i];          // last tile starting here
-    for (uint32_t t = prev; t <= cur && t <= tiles; t++) start[t] = i;
+    if (threadIdx.x < 2) s_lim[threadIdx.x] = lower_bound_key(keys, 0u, n, min(t0 + threadIdx.x * 256u, tiles + 1u));
+    __syncthreads();
+    const uint32_t t = t0 + threadIdx.x;
+    if (t <= tiles) start[t] = lower_bound_key(keys, s_lim[0], s_lim[1], t);
 }
 
 // ---------------------------------------------------------- radix sort

@@ -395,7 +395,7 @@ int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap
         rc = radix_sort(x, b.keys, b.vals, stats + kStatPairs, bits, &src);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(tile_ranges_kernel, dim3(cap / 256 + 1), dim3(256), 0, x.s, (const uint32_t*)b.keys[src].p,
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3(P.tiles / 256 + 1), dim3(256), 0, x.s, (const uint32_t*)b.keys[src].p,
                        (const uint32_t*)(stats + kStatPairs), P.tiles, b.tile_start.as<uint32_t>());
     HIP_TRY(hipGetLastError());
 
