@@ -451,21 +451,22 @@ struct EmitArgs {
 
 constexpr uint32_t kEmitBig = 2048;   // pairs above which a primitive goes to emit_big_kernel
 
-// Block bits of a pair value (kPairBlockShift): for 16x16 tiles, which
-// 8x8 blocks of tile (col, row) of the primitive's tile rectangle its pixel
-// box overlaps; box is relative to the rectangle's first tile.  0 for other
-// tile sizes.
+// Block bits of a pair value (kPairBlockShift): which quarters of tile
+// (col, row) of the primitive's tile rectangle its pixel box overlaps (the
+// 8x8 blocks of a 16x16 tile, the 16x16 quadrants of a 32x32 tile); box is
+// relative to the rectangle's first tile.  0 for 8x8 tiles.
 __device__ __forceinline__ uint32_t pair_blocks(int tile_shift, int4 box, uint32_t col, uint32_t row)
 {
-    if (tile_shift != 4) return 0u;
-    const int X = (int)col * 16, Y = (int)row * 16;
+    if (tile_shift != 4 && tile_shift != 5) return 0u;
+    const int half = 1 << (tile_shift - 1);
+    const int X = (int)col << tile_shift, Y = (int)row << tile_shift;
     uint32_t m = 0u;
 #pragma unroll
     for (int by = 0; by < 2; by++)
 #pragma unroll
         for (int bx = 0; bx < 2; bx++) {
-            const int x0 = X + bx * 8, y0 = Y + by * 8;
-            if (box.x < x0 + 8 && box.z > x0 && box.y < y0 + 8 && box.w > y0) m |= 1u << (by * 2 + bx);
+            const int x0 = X + bx * half, y0 = Y + by * half;
+            if (box.x < x0 + half && box.z > x0 && box.y < y0 + half && box.w > y0) m |= 1u << (by * 2 + bx);
         }
     return m << kPairBlockShift;
 }
@@ -499,30 +500,32 @@ __device__ __forceinline__ bool rect_may_cover(const TriSetup& r, const float xs
     return true;
 }
 
-// pair_blocks less the blocks where triangle r covers no pixel (points and
-// lines keep their box's blocks).  box is relative to the primitive's first
-// tile, whose pixel origin is (r.x0 - box.x, r.y0 - box.y).  The corner
-// centres of the tile's two block columns and rows are computed once.
+// pair_blocks less the quarters where triangle r covers no pixel (points
+// and lines keep their box's quarters).  box is relative to the primitive's
+// first tile, whose pixel origin is (r.x0 - box.x, r.y0 - box.y).  The
+// corner centres of the tile's two quarter columns and rows are computed
+// once.
 __device__ __forceinline__ uint32_t pair_blocks_covered(int tile_shift, int4 box, uint32_t col, uint32_t row, const TriSetup& r,
                                                         const DrawDesc& d)
 {
     const uint32_t m = pair_blocks(tile_shift, box, col, row);
     if (m == 0u || (r.flags & (kPrimPoint | kPrimLine))) return m;
     const int ox = r.x0 - box.x, oy = r.y0 - box.y;
-    const int X = (int)col * 16, Y = (int)row * 16;
+    const int half = 1 << (tile_shift - 1);
+    const int X = (int)col << tile_shift, Y = (int)row << tile_shift;
     const uint32_t bits = m >> kPairBlockShift;
     float xs[2][2], ys[2][2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        // block column h is in the box when a block of it is (bits h, h + 2)
+        // quarter column h is in the box when a quarter of it is (bits h, h + 2)
         if (bits & (5u << h)) {
-            xs[h][0] = pixel_centre(ox + max(X + 8 * h, box.x), d.vp_w, d.half_x);
-            xs[h][1] = pixel_centre(ox + min(X + 8 * h + 8, box.z) - 1, d.vp_w, d.half_x);
+            xs[h][0] = pixel_centre(ox + max(X + half * h, box.x), d.vp_w, d.half_x);
+            xs[h][1] = pixel_centre(ox + min(X + half * h + half, box.z) - 1, d.vp_w, d.half_x);
         }
-        // block row h (bits 2h, 2h + 1)
+        // quarter row h (bits 2h, 2h + 1)
         if (bits & (3u << (2 * h))) {
-            ys[h][0] = pixel_centre(oy + max(Y + 8 * h, box.y), d.vp_h, d.half_y);
-            ys[h][1] = pixel_centre(oy + min(Y + 8 * h + 8, box.w) - 1, d.vp_h, d.half_y);
+            ys[h][0] = pixel_centre(oy + max(Y + half * h, box.y), d.vp_h, d.half_y);
+            ys[h][1] = pixel_centre(oy + min(Y + half * h + half, box.w) - 1, d.vp_h, d.half_y);
         }
     }
     uint32_t keep = 0u;

@@ -91,10 +91,11 @@ static_assert(sizeof(TriSetup) == 128, "TriSetup must be 128 bytes");
 enum : uint32_t { kPrimFront = 1u, kPrimPoint = 2u, kPrimLine = 4u };
 
 // A (tile, primitive) pair's value: the primitive index in the low 28 bits
-// (launches hold fewer than 2^28 primitives) and, for 16x16 tiles, in bits
-// 28-31 which of the tile's four 8x8 blocks (bit by * 2 + bx) the
-// primitive's pixel box overlaps, less, for a triangle, the blocks where an
-// edge function is negative at every pixel.
+// (launches hold fewer than 2^28 primitives) and, for 16x16 and 32x32
+// tiles, in bits 28-31 which of the tile's four quarters (8x8 blocks or
+// 16x16 quadrants, bit by * 2 + bx) the primitive's pixel box overlaps,
+// less, for a triangle, the quarters where an edge function is negative at
+// every pixel.
 constexpr uint32_t kPairPrim = 0x0FFFFFFFu;
 constexpr int kPairBlockShift = 28;
 

@@ -194,7 +194,6 @@ raster_kernel(RasterArgs a)
         PixelState px[BPW];
         int pxx[BPW], pxy[BPW];
         bool inside[BPW];
-        int wx0 = 1 << 30, wy0 = 1 << 30, wx1 = -(1 << 30), wy1 = -(1 << 30);
 #pragma unroll
         for (int k = 0; k < BPW; k++) {
             const int blk = slot + k * WAVES;
@@ -202,7 +201,6 @@ raster_kernel(RasterArgs a)
             pxx[k] = bx0 + (lane & 7);
             pxy[k] = by0 + (lane >> 3);
             inside[k] = pxx[k] < (int)tp.width && pxy[k] < (int)tp.height;
-            wx0 = min(wx0, bx0); wy0 = min(wy0, by0); wx1 = max(wx1, bx0 + 8); wy1 = max(wy1, by0 + 8);
             px[k] = clear;
             const uint64_t pix = (uint64_t)pxy[k] * tp.width + (uint64_t)pxx[k];
             if (inside[k] && tp.has_color && tp.color_load != 1)
@@ -312,8 +310,15 @@ raster_kernel(RasterArgs a)
                     for (int k = 0; k < BPW; k++) mine |= 1u << (slot + k * WAVES);
                     ov = (blocks_l & mine) != 0u;
                 } else {
-                    const int4 bb = *(const int4*)&a.recs[gp_l].x0;
-                    ov = !(bb.z <= wx0 || bb.x >= wx1 || bb.w <= wy0 || bb.y >= wy1);
+                    // the pair carries the 16x16 quadrants the primitive may
+                    // cover; a wave's blocks lie in the quadrants of mine
+                    uint32_t mine = 0u;
+#pragma unroll
+                    for (int k = 0; k < BPW; k++) {
+                        const int blk = slot + k * WAVES;
+                        mine |= 1u << ((blk / BW) / 2 * 2 + (blk % BW) / 2);
+                    }
+                    ov = (blocks_l & mine) != 0u;
                 }
                 if (ov && a.hiz_op >= 0) {
                     const float2 dz = *(const float2*)&a.recs[gp_l].dt_lo;
@@ -338,7 +343,7 @@ raster_kernel(RasterArgs a)
                 const int j = (int)__builtin_ctzll(mask);
                 mask &= mask - 1;
                 const uint32_t gprim = __builtin_amdgcn_readlane(gp_l, j);
-                const uint32_t blocks = (TILE == 16 && BPW > 1) ? __builtin_amdgcn_readlane(blocks_l, j) : 0u;
+                const uint32_t blocks = (TILE >= 16 && BPW > 1) ? __builtin_amdgcn_readlane(blocks_l, j) : 0u;
                 TriSetup r;
                 if constexpr (STAGE) {
                     uint32_t* dst = (uint32_t*)&r;
@@ -391,8 +396,11 @@ raster_kernel(RasterArgs a)
                     const int bx0 = tile_x0 + (blk % BW) * 8, by0 = tile_y0 + (blk / BW) * 8;
                     if (TILE == 16 && BPW > 1) {
                         if (!((blocks >> blk) & 1u)) continue;   // wave-uniform
-                    } else if (BPW > 1 && (r.x1 <= bx0 || r.x0 >= bx0 + 8 || r.y1 <= by0 || r.y0 >= by0 + 8)) {
-                        continue;   // wave-uniform
+                    } else if (BPW > 1) {
+                        // wave-uniform: the block's quadrant may be covered
+                        // and the block meets the primitive's box
+                        if (TILE == 32 && !((blocks >> ((blk / BW) / 2 * 2 + (blk % BW) / 2)) & 1u)) continue;
+                        if (r.x1 <= bx0 || r.x0 >= bx0 + 8 || r.y1 <= by0 || r.y0 >= by0 + 8) continue;
                     }
                     const int x = pxx[k], y = pxy[k];
                     if (x < r.x0 || x >= r.x1 || y < r.y0 || y >= r.y1) continue;

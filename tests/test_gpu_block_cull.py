@@ -1,12 +1,12 @@
 """Block coverage culling at its edge cases (GPU only).
 
-For 16x16 tiles the emit kernels leave out of a pair the 8x8 blocks where
-some rounded edge function is negative at all four corners of the block's
-part of the triangle's box.  The triangles here put edges through pixel
+For 16x16 and 32x32 tiles the emit kernels leave out of a pair the tile
+quarters (8x8 blocks, 16x16 quadrants) where some rounded edge function is
+negative at all four corners of the quarter's part of the triangle's box.  The triangles here put edges through pixel
 centres and along block rows and columns, make slivers a fraction of a
 pixel wide on block boundaries, and put vertices on block-corner pixel
-centres, under identity, offset and scaled viewports and all three 16x16
-tile configurations.  Colour, depth and the covered-fragment count must
+centres, under identity, offset and scaled viewports and every 16x16 and
+32x32 tile configuration.  Colour, depth and the covered-fragment count must
 equal the oracle's bit for bit."""
 import zlib
 
@@ -111,18 +111,18 @@ def ctx(gpu_ctx):
     gpu_ctx.apply_config(saved)
 
 
-@pytest.mark.parametrize("waves", [4, 2, 1])
+@pytest.mark.parametrize("tile,waves", [(16, 4), (16, 2), (16, 1), (32, 4), (32, 2), (32, 1)])
 @pytest.mark.parametrize("vp", VIEWPORTS)
 @pytest.mark.parametrize("case", list(CASES))
-def test_block_culling_matches_oracle(ctx, case, vp, waves):
-    label = f"block cull {case} vp={vp} 16x{waves}"
+def test_block_culling_matches_oracle(ctx, case, vp, tile, waves):
+    label = f"block cull {case} vp={vp} {tile}x{waves}"
     print(f"[block_cull] {label}", flush=True)
     s = W.soup_scene(10, WIDTH, HEIGHT, seed=1)
     s.draws[0].vertices[0] = CASES[case](seed=zlib.crc32(label.encode())).view(np.uint8).reshape(-1)
     s.draws[0].count = s.draws[0].vertices[0].size // 32
     s = S.with_viewport(s, vp)
     cfg = ctx.config()
-    cfg.tile_size, cfg.waves_per_tile = 16, waves
+    cfg.tile_size, cfg.waves_per_tile = tile, waves
     ctx.apply_config(cfg)
     ref = oracle_lib.render_scene(s)
     compare(s, render_gpu(ctx, s), ref, label)
