@@ -30,6 +30,7 @@
  */
 #include "cpvref.h"
 
+#include <stdint.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -802,6 +803,12 @@ typedef struct {
     fmt_t cfmt, dsfmt;
     int has_color, has_ds;
     uint64_t fragments;
+    /* Pixel window [wx0, wx1) x [wy0, wy1) the primitive loops are limited
+     * to (the whole plane for the reference's single-threaded replay; one
+     * tile per task in the tile-parallel mode, where every pixel still sees
+     * the same primitives in the same order with the same arithmetic). */
+    int32_t wx0, wy0, wx1, wy1;
+    int threads;   /* > 1: tile-parallel replay (OpenMP) */
 } ctx_t;
 
 static uint8_t* color_px(ctx_t* c, int x, int y)
@@ -968,95 +975,194 @@ static float set_datum(int mode, int n, const float* pw, const float* v, const f
     return num / den;
 }
 
-/* ProcessTriangles (CommandBuffer.Draw.cpp:1510-1594) over primitives
- * [pb, pe) of the assembled list. */
-static void process_triangles(ctx_t* c, const uint32_t (*prims)[4], uint64_t nprims, const float* vout,
-                              int vstride, int nvary, uint64_t pb, uint64_t pe)
+/* One triangle after the per-primitive part of ProcessTriangles
+ * (CommandBuffer.Draw.cpp:1526-1569): the CW swap, the perspective divide and
+ * the truncated pixel box [x0, x1) x [y0, y1). */
+typedef struct {
+    float P[3][4];
+    const float* q[3];
+    uint32_t provoking;
+    int32_t x0, y0, x1, y1;
+} tri_t;
+
+typedef struct {
+    float vpw, vph, half_x, half_y;
+    int32_t limit_x, limit_y;
+} tri_view_t;
+
+static void tri_view(const ctx_t* c, tri_view_t* v)
+{
+    const float* vp = c->draw->viewport;
+    v->vpw = vp[2];
+    v->vph = vp[3];
+    v->half_x = 1.0f / v->vpw * 0.5f;
+    v->half_y = 1.0f / v->vph * 0.5f;
+    v->limit_x = f2i_trunc(v->vpw);
+    v->limit_y = f2i_trunc(v->vph);
+    /* Guard the attachment extent (a viewport larger than the framebuffer
+     * writes out of bounds in the reference). */
+    if (v->limit_x > (int32_t)c->pass->width) v->limit_x = (int32_t)c->pass->width;
+    if (v->limit_y > (int32_t)c->pass->height) v->limit_y = (int32_t)c->pass->height;
+}
+
+static void tri_setup(const ctx_t* c, const tri_view_t* v, const uint32_t prim[4], const float* vout, int vstride, tri_t* t)
+{
+    t->provoking = prim[0];
+    uint32_t i0 = prim[1], i1 = prim[2], i2 = prim[3];
+    if (c->pipe->front_face == CPV_FRONT_FACE_CW) { uint32_t s = i0; i0 = i2; i2 = s; }
+    t->q[0] = vout + (uint64_t)i0 * vstride;
+    t->q[1] = vout + (uint64_t)i1 * vstride;
+    t->q[2] = vout + (uint64_t)i2 * vstride;
+    for (int k = 0; k < 3; k++) {
+        t->P[k][0] = t->q[k][0] / t->q[k][3];
+        t->P[k][1] = t->q[k][1] / t->q[k][3];
+        t->P[k][2] = t->q[k][2] / t->q[k][3];
+        t->P[k][3] = t->q[k][3];
+    }
+    int32_t sx[3], sy[3];
+    for (int k = 0; k < 3; k++) {
+        sx[k] = f2i_trunc((t->P[k][0] + 1.0f) * 0.5f * v->vpw);
+        sy[k] = f2i_trunc((t->P[k][1] + 1.0f) * 0.5f * v->vph);
+    }
+    int32_t mnx = sx[0], mny = sy[0], mxx = sx[0], mxy = sy[0];
+    for (int k = 1; k < 3; k++) {
+        if (sx[k] < mnx) mnx = sx[k];
+        if (sy[k] < mny) mny = sy[k];
+        if (sx[k] > mxx) mxx = sx[k];
+        if (sy[k] > mxy) mxy = sy[k];
+    }
+    t->x0 = mnx > 0 ? mnx : 0;
+    t->y0 = mny > 0 ? mny : 0;
+    t->x1 = (int32_t)((int64_t)mxx + 1 < v->limit_x ? (int64_t)mxx + 1 : v->limit_x);
+    t->y1 = (int32_t)((int64_t)mxy + 1 < v->limit_y ? (int64_t)mxy + 1 : v->limit_y);
+}
+
+/* The per-pixel part of ProcessTriangles over the triangle's box clipped to
+ * the context's window. */
+static void tri_raster(ctx_t* c, const tri_view_t* v, const tri_t* t, const float* vout, int vstride, int nvary)
 {
     const cpv_pipeline_desc* p = c->pipe;
     const float* vp = c->draw->viewport;
-    const float vpw = vp[2], vph = vp[3];
-    const float half_x = 1.0f / vpw * 0.5f, half_y = 1.0f / vph * 0.5f;
-    int32_t limit_x = f2i_trunc(vpw), limit_y = f2i_trunc(vph);
-    /* Guard the attachment extent (a viewport larger than the framebuffer
-     * writes out of bounds in the reference). */
-    if (limit_x > (int32_t)c->pass->width) limit_x = (int32_t)c->pass->width;
-    if (limit_y > (int32_t)c->pass->height) limit_y = (int32_t)c->pass->height;
-    if (pe > nprims) pe = nprims;
-
-    for (uint64_t pi = pb; pi < pe; pi++) {
-        uint32_t provoking = prims[pi][0];
-        uint32_t i0 = prims[pi][1], i1 = prims[pi][2], i2 = prims[pi][3];
-        if (p->front_face == CPV_FRONT_FACE_CW) { uint32_t t = i0; i0 = i2; i2 = t; }
-        const float* q[3] = {vout + (uint64_t)i0 * vstride, vout + (uint64_t)i1 * vstride, vout + (uint64_t)i2 * vstride};
-        float P[3][4];
-        for (int k = 0; k < 3; k++) {
-            P[k][0] = q[k][0] / q[k][3];
-            P[k][1] = q[k][1] / q[k][3];
-            P[k][2] = q[k][2] / q[k][3];
-            P[k][3] = q[k][3];
-        }
-        int32_t sx[3], sy[3];
-        for (int k = 0; k < 3; k++) {
-            sx[k] = f2i_trunc((P[k][0] + 1.0f) * 0.5f * vpw);
-            sy[k] = f2i_trunc((P[k][1] + 1.0f) * 0.5f * vph);
-        }
-        int32_t mnx = sx[0], mny = sy[0], mxx = sx[0], mxy = sy[0];
-        for (int k = 1; k < 3; k++) {
-            if (sx[k] < mnx) mnx = sx[k];
-            if (sy[k] < mny) mny = sy[k];
-            if (sx[k] > mxx) mxx = sx[k];
-            if (sy[k] > mxy) mxy = sy[k];
-        }
-        int32_t start_x = mnx > 0 ? mnx : 0, start_y = mny > 0 ? mny : 0;
-        int32_t end_x = (int32_t)((int64_t)mxx + 1 < limit_x ? (int64_t)mxx + 1 : limit_x);
-        int32_t end_y = (int32_t)((int64_t)mxy + 1 < limit_y ? (int64_t)mxy + 1 : limit_y);
-
-        for (int32_t y = start_y; y < end_y; y++) {
-            const float yf = ((float)y / vph + half_y) * 2.0f - 1.0f;
-            for (int32_t x = start_x; x < end_x; x++) {
-                const float xf = ((float)x / vpw + half_x) * 2.0f - 1.0f;
-                /* GetFragmentInput (:874-954) */
-                float area = edge(P[0], P[1], P[2][0], P[2][1]);
-                float w0, w1, w2;
-                int front;
-                if (area < 0) {
-                    area = -area;
-                    front = 0;
-                    w0 = edge(P[2], P[1], xf, yf);
-                    w1 = edge(P[0], P[2], xf, yf);
-                    w2 = edge(P[1], P[0], xf, yf);
-                } else {
-                    front = 1;
-                    w0 = edge(P[1], P[2], xf, yf);
-                    w1 = edge(P[2], P[0], xf, yf);
-                    w2 = edge(P[0], P[1], xf, yf);
-                }
-                if (w0 < 0 || w1 < 0 || w2 < 0 || ((p->cull_mode & CPV_CULL_BACK) && !front) ||
-                    ((p->cull_mode & CPV_CULL_FRONT) && front))
-                    continue;
-                c->fragments++;
-                w0 /= area;
-                w1 /= area;
-                w2 /= area;
-                float depth = P[0][2] * w0 + P[1][2] * w1 + P[2][2] * w2;
-                float in[16];
-                const float pw[3] = {P[0][3], P[1][3], P[2][3]};
-                const float w[3] = {w0, w1, w2};
-                const int mode = fs_interp(p->fs_kind);
-                for (int k = 0; k < nvary; k++) {
-                    const float v[3] = {q[0][4 + k], q[1][4 + k], q[2][4 + k]};
-                    /* Flat: the provoking vertex's value (:943-946) */
-                    in[k] = mode == 2 ? vout[(uint64_t)provoking * vstride + 4 + k] : set_datum(mode, 3, pw, v, w);
-                }
-                /* DrawPixel (:1300-1313) */
-                float dt = (vp[5] - vp[4]) * depth + vp[4];
-                float colour[4];
-                if (run_fs(p, c->draw, in, colour)) continue;
-                fragment_ops(c, x, y, dt, front, colour);
+    const int32_t ys = t->y0 > c->wy0 ? t->y0 : c->wy0, ye = t->y1 < c->wy1 ? t->y1 : c->wy1;
+    const int32_t xs = t->x0 > c->wx0 ? t->x0 : c->wx0, xe = t->x1 < c->wx1 ? t->x1 : c->wx1;
+    const float (*P)[4] = t->P;
+    for (int32_t y = ys; y < ye; y++) {
+        const float yf = ((float)y / v->vph + v->half_y) * 2.0f - 1.0f;
+        for (int32_t x = xs; x < xe; x++) {
+            const float xf = ((float)x / v->vpw + v->half_x) * 2.0f - 1.0f;
+            /* GetFragmentInput (:874-954) */
+            float area = edge(P[0], P[1], P[2][0], P[2][1]);
+            float w0, w1, w2;
+            int front;
+            if (area < 0) {
+                area = -area;
+                front = 0;
+                w0 = edge(P[2], P[1], xf, yf);
+                w1 = edge(P[0], P[2], xf, yf);
+                w2 = edge(P[1], P[0], xf, yf);
+            } else {
+                front = 1;
+                w0 = edge(P[1], P[2], xf, yf);
+                w1 = edge(P[2], P[0], xf, yf);
+                w2 = edge(P[0], P[1], xf, yf);
             }
+            if (w0 < 0 || w1 < 0 || w2 < 0 || ((p->cull_mode & CPV_CULL_BACK) && !front) ||
+                ((p->cull_mode & CPV_CULL_FRONT) && front))
+                continue;
+            c->fragments++;
+            w0 /= area;
+            w1 /= area;
+            w2 /= area;
+            float depth = P[0][2] * w0 + P[1][2] * w1 + P[2][2] * w2;
+            float in[16];
+            const float pw[3] = {P[0][3], P[1][3], P[2][3]};
+            const float w[3] = {w0, w1, w2};
+            const int mode = fs_interp(p->fs_kind);
+            for (int k = 0; k < nvary; k++) {
+                const float vv[3] = {t->q[0][4 + k], t->q[1][4 + k], t->q[2][4 + k]};
+                /* Flat: the provoking vertex's value (:943-946) */
+                in[k] = mode == 2 ? vout[(uint64_t)t->provoking * vstride + 4 + k] : set_datum(mode, 3, pw, vv, w);
+            }
+            /* DrawPixel (:1300-1313) */
+            float dt = (vp[5] - vp[4]) * depth + vp[4];
+            float colour[4];
+            if (run_fs(p, c->draw, in, colour)) continue;
+            fragment_ops(c, x, y, dt, front, colour);
         }
     }
+}
+
+/* ProcessTriangles (CommandBuffer.Draw.cpp:1510-1594) over primitives
+ * [pb, pe) of the assembled list, in order. */
+static void process_triangles(ctx_t* c, const uint32_t (*prims)[4], uint64_t nprims, const float* vout,
+                              int vstride, int nvary, uint64_t pb, uint64_t pe)
+{
+    tri_view_t v;
+    tri_view(c, &v);
+    if (pe > nprims) pe = nprims;
+    for (uint64_t pi = pb; pi < pe; pi++) {
+        tri_t t;
+        tri_setup(c, &v, prims[pi], vout, vstride, &t);
+        tri_raster(c, &v, &t, vout, vstride, nvary);
+    }
+}
+
+/* Tile-parallel ProcessTriangles: the framebuffer is cut into kParTile x
+ * kParTile pixel tiles; the triangles are set up once and binned to the
+ * tiles their boxes meet, in primitive order; each tile then runs the
+ * per-pixel loop of its triangles over its own pixels.  Pixels are
+ * independent and each sees the same triangles in the same order with the
+ * same arithmetic, so the result is bit-identical to process_triangles. */
+enum { kParTile = 64 };
+
+static void process_triangles_parallel(ctx_t* c, const uint32_t (*prims)[4], uint64_t nprims, const float* vout,
+                                       int vstride, int nvary, uint64_t pb, uint64_t pe)
+{
+    tri_view_t v;
+    tri_view(c, &v);
+    if (pe > nprims) pe = nprims;
+    if (pb >= pe) return;
+    const uint64_t n = pe - pb;
+    tri_t* tris = (tri_t*)malloc(sizeof(tri_t) * n);
+    const int32_t tx = (int32_t)((c->pass->width + kParTile - 1) / kParTile), ty = (int32_t)((c->pass->height + kParTile - 1) / kParTile);
+    const uint64_t ntiles = (uint64_t)tx * (uint64_t)ty;
+    uint64_t* start = (uint64_t*)calloc(ntiles + 1, sizeof(uint64_t));
+#pragma omp parallel for schedule(static) num_threads(c->threads)
+    for (int64_t i = 0; i < (int64_t)n; i++) tri_setup(c, &v, prims[pb + (uint64_t)i], vout, vstride, &tris[i]);
+    /* bin: count, scan, fill (primitive order within every tile) */
+    for (uint64_t i = 0; i < n; i++) {
+        const tri_t* t = &tris[i];
+        if (t->x0 >= t->x1 || t->y0 >= t->y1) continue;
+        for (int32_t b = t->y0 / kParTile; b <= (t->y1 - 1) / kParTile; b++)
+            for (int32_t a = t->x0 / kParTile; a <= (t->x1 - 1) / kParTile; a++) start[(uint64_t)b * tx + a + 1]++;
+    }
+    for (uint64_t k = 0; k < ntiles; k++) start[k + 1] += start[k];
+    uint64_t* fill = (uint64_t*)malloc(sizeof(uint64_t) * (ntiles ? ntiles : 1));
+    memcpy(fill, start, sizeof(uint64_t) * ntiles);
+    uint32_t* list = (uint32_t*)malloc(sizeof(uint32_t) * (start[ntiles] ? start[ntiles] : 1));
+    for (uint64_t i = 0; i < n; i++) {
+        const tri_t* t = &tris[i];
+        if (t->x0 >= t->x1 || t->y0 >= t->y1) continue;
+        for (int32_t b = t->y0 / kParTile; b <= (t->y1 - 1) / kParTile; b++)
+            for (int32_t a = t->x0 / kParTile; a <= (t->x1 - 1) / kParTile; a++) list[fill[(uint64_t)b * tx + a]++] = (uint32_t)i;
+    }
+    uint64_t frags = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : frags) num_threads(c->threads)
+    for (int64_t k = 0; k < (int64_t)ntiles; k++) {
+        ctx_t tc = *c;
+        tc.fragments = 0;
+        tc.wx0 = (int32_t)(k % tx) * kParTile;
+        tc.wy0 = (int32_t)(k / tx) * kParTile;
+        tc.wx1 = tc.wx0 + kParTile;
+        tc.wy1 = tc.wy0 + kParTile;
+        for (uint64_t j = start[k]; j < start[k + 1]; j++) tri_raster(&tc, &v, &tris[list[j]], vout, vstride, nvary);
+        frags += tc.fragments;
+    }
+    c->fragments += frags;
+    free(list);
+    free(fill);
+    free(start);
+    free(tris);
 }
 
 /* int32 sum with two's-complement wrap, as the reference's compiled int
@@ -1090,6 +1196,10 @@ static void process_points(ctx_t* c, uint64_t nverts, const float* vout, int vst
         if (sy < 0) sy = 0;
         if (ex > limit_x) ex = limit_x;
         if (ey > limit_y) ey = limit_y;
+        if (sx < c->wx0) sx = c->wx0;
+        if (sy < c->wy0) sy = c->wy0;
+        if (ex > c->wx1) ex = c->wx1;
+        if (ey > c->wy1) ey = c->wy1;
         for (int32_t y = sy; y < ey; y++) {
             for (int32_t x = sx; x < ex; x++) {
                 const float s = 0.5f + (float)wrap_sub(x, cx) / size;
@@ -1143,9 +1253,12 @@ static void process_lines(ctx_t* c, uint64_t nverts, const float* vout, int vstr
         const float p10[2] = {P1[0] + perp[0] * lwx, P1[1] + perp[1] * lwy};
         const float p11[2] = {P1[0] - perp[0] * lwx, P1[1] - perp[1] * lwy};
         const float len = sqrtf(d[0] * d[0] + d[1] * d[1]);
-        for (uint32_t y = 0; y < ly; y++) {
+        const uint32_t ys = c->wy0 > 0 ? (uint32_t)c->wy0 : 0u, xs = c->wx0 > 0 ? (uint32_t)c->wx0 : 0u;
+        const uint32_t ye = c->wy1 < (int32_t)ly ? (uint32_t)(c->wy1 > 0 ? c->wy1 : 0) : ly;
+        const uint32_t xe = c->wx1 < (int32_t)lx ? (uint32_t)(c->wx1 > 0 ? c->wx1 : 0) : lx;
+        for (uint32_t y = ys; y < ye; y++) {
             const float yf = ((float)y / vph + half_y) * 2.0f - 1.0f;
-            for (uint32_t x = 0; x < lx; x++) {
+            for (uint32_t x = xs; x < xe; x++) {
                 const float xf = ((float)x / vpw + half_x) * 2.0f - 1.0f;
                 if (!(edge(p00, p01, xf, yf) >= 0 && edge(p11, p10, xf, yf) >= 0 && edge(p10, p00, xf, yf) >= 0 &&
                       edge(p01, p11, xf, yf) >= 0))
@@ -1230,18 +1343,44 @@ static int draw_one(ctx_t* c, uint64_t pb, uint64_t pe)
         }
         /* ProcessVertexShader: one invocation per assembled vertex */
         float* vout = (float*)malloc(sizeof(float) * (size_t)vstride * (n ? n : 1));
-        for (uint64_t i = 0; i < n; i++) {
+        int vs_rc = 0;   /* the lowest (most negative) error code of any invocation */
+#pragma omp parallel for schedule(static) reduction(min : vs_rc) num_threads(c->threads > 1 ? c->threads : 1)
+        for (int64_t i = 0; i < (int64_t)n; i++) {
             int rc = run_vs(p, d, vid[i], d->first_instance + inst, vout + i * vstride, vout + i * vstride + 4);
-            if (rc) { free(vid); free(vout); return rc; }
+            if (rc < vs_rc) vs_rc = rc;
         }
-        if (p->topology == CPV_TOPOLOGY_POINT_LIST) {
+        if (vs_rc) { free(vid); free(vout); return vs_rc; }
+        if (p->topology < CPV_TOPOLOGY_TRIANGLE_LIST && c->threads > 1) {
+            /* points and lines: every tile runs the whole loop over its own pixels */
+            const int32_t tx = (int32_t)((c->pass->width + kParTile - 1) / kParTile);
+            const int64_t ntiles = (int64_t)tx * (int64_t)((c->pass->height + kParTile - 1) / kParTile);
+            uint64_t frags = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : frags) num_threads(c->threads)
+            for (int64_t k = 0; k < ntiles; k++) {
+                ctx_t tc = *c;
+                tc.fragments = 0;
+                tc.wx0 = (int32_t)(k % tx) * kParTile;
+                tc.wy0 = (int32_t)(k / tx) * kParTile;
+                tc.wx1 = tc.wx0 + kParTile;
+                tc.wy1 = tc.wy0 + kParTile;
+                if (p->topology == CPV_TOPOLOGY_POINT_LIST)
+                    process_points(&tc, n, vout, vstride, nvary, p->vs_kind == CPV_VS_PASSTHROUGH_POINT ? 4 : -1, pb, pe);
+                else
+                    process_lines(&tc, n, vout, vstride, nvary, pb, pe);
+                frags += tc.fragments;
+            }
+            c->fragments += frags;
+        } else if (p->topology == CPV_TOPOLOGY_POINT_LIST) {
             process_points(c, n, vout, vstride, nvary, p->vs_kind == CPV_VS_PASSTHROUGH_POINT ? 4 : -1, pb, pe);
         } else if (p->topology < CPV_TOPOLOGY_TRIANGLE_LIST) {
             process_lines(c, n, vout, vstride, nvary, pb, pe);
         } else {
             uint64_t nprims;
             uint32_t (*prims)[4] = assemble(p->topology, n, &nprims);
-            process_triangles(c, (const uint32_t (*)[4])prims, nprims, vout, vstride, nvary, pb, pe);
+            if (c->threads > 1)
+                process_triangles_parallel(c, (const uint32_t (*)[4])prims, nprims, vout, vstride, nvary, pb, pe);
+            else
+                process_triangles(c, (const uint32_t (*)[4])prims, nprims, vout, vstride, nvary, pb, pe);
             free(prims);
         }
         free(vout);
@@ -1280,10 +1419,10 @@ static int clear_pass(const cpv_pass_desc* pass)
     return CPV_OK;
 }
 
-int cpvref_render_pass_range(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
-                             const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
-                             uint32_t draw_count, uint64_t prim_begin, uint64_t prim_end,
-                             uint64_t* fragments)
+static int render_pass_impl(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                            const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
+                            uint32_t draw_count, uint64_t prim_begin, uint64_t prim_end, int threads,
+                            uint64_t* fragments)
 {
     if (!pass || (draw_count && (!pipelines || !draws))) return CPV_ERROR_INVALID;
     if (pass->color_count > CPV_MAX_COLOR_ATTACHMENTS) return CPV_ERROR_UNSUPPORTED;
@@ -1294,6 +1433,9 @@ int cpvref_render_pass_range(const cpv_pass_desc* pass, const cpv_pipeline_desc*
         ctx_t c;
         memset(&c, 0, sizeof(c));
         c.pass = pass;
+        c.wx0 = c.wy0 = INT32_MIN;
+        c.wx1 = c.wy1 = INT32_MAX;
+        c.threads = threads;
         c.pipe = &pipelines[draw_pipeline ? draw_pipeline[i] : 0];
         c.draw = &draws[i];
         c.has_color = pass->color_count > 0 && c.pipe->color_count > 0;
@@ -1308,11 +1450,26 @@ int cpvref_render_pass_range(const cpv_pass_desc* pass, const cpv_pipeline_desc*
     return CPV_OK;
 }
 
+int cpvref_render_pass_range(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                             const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
+                             uint32_t draw_count, uint64_t prim_begin, uint64_t prim_end,
+                             uint64_t* fragments)
+{
+    return render_pass_impl(pass, pipelines, draws, draw_pipeline, draw_count, prim_begin, prim_end, 1, fragments);
+}
+
 int cpvref_render_pass(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
                        const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
                        uint32_t draw_count, uint64_t* fragments)
 {
-    return cpvref_render_pass_range(pass, pipelines, draws, draw_pipeline, draw_count, 0, UINT64_MAX, fragments);
+    return render_pass_impl(pass, pipelines, draws, draw_pipeline, draw_count, 0, UINT64_MAX, 1, fragments);
+}
+
+int cpvref_render_pass_parallel(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                                const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
+                                uint32_t draw_count, int threads, uint64_t* fragments)
+{
+    return render_pass_impl(pass, pipelines, draws, draw_pipeline, draw_count, 0, UINT64_MAX, threads, fragments);
 }
 
 int cpvref_pixel_coverage(const float positions[12], uint32_t cull_mode, uint32_t front_face,

@@ -37,6 +37,15 @@ int cpvref_render_pass_range(const cpv_pass_desc* pass, const cpv_pipeline_desc*
                              uint32_t draw_count, uint64_t prim_begin, uint64_t prim_end,
                              uint64_t* fragments);
 
+/* Same result as cpvref_render_pass, bit for bit, computed tile-parallel
+ * with OpenMP over `threads` host threads: the framebuffer is cut into
+ * 64x64 tiles, the triangles are set up once and binned to the tiles in
+ * primitive order, and every tile runs the reference's per-pixel loop over
+ * its own pixels (threads <= 1: the single-threaded replay). */
+int cpvref_render_pass_parallel(const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                                const cpv_draw_desc* draws, const uint32_t* draw_pipeline,
+                                uint32_t draw_count, int threads, uint64_t* fragments);
+
 /* Format conversions (exposed for the format tests). */
 int cpvref_pack_f32(uint32_t format, const float rgba[4], uint8_t* dst);
 int cpvref_unpack_f32(uint32_t format, const uint8_t* src, float rgba[4]);

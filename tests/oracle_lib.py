@@ -31,6 +31,9 @@ def lib() -> C.CDLL:
         L.cpvref_render_pass_range.argtypes = [C.POINTER(abi.PassDesc), C.POINTER(abi.PipelineDesc),
                                                C.POINTER(abi.DrawDesc), C.POINTER(C.c_uint32), C.c_uint32,
                                                C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.cpvref_render_pass_parallel.argtypes = [C.POINTER(abi.PassDesc), C.POINTER(abi.PipelineDesc),
+                                                  C.POINTER(abi.DrawDesc), C.POINTER(C.c_uint32), C.c_uint32,
+                                                  C.c_int, C.POINTER(C.c_uint64)]
         L.cpvref_pack_f32.argtypes = [C.c_uint32, C.POINTER(C.c_float), P]
         L.cpvref_unpack_f32.argtypes = [C.c_uint32, P, C.POINTER(C.c_float)]
         L.cpvref_pack_u32.argtypes = [C.c_uint32, C.POINTER(C.c_uint32), P]
@@ -62,15 +65,30 @@ def lib() -> C.CDLL:
     return _lib
 
 
-def render(m: Materialized, prim_range: tuple[int, int] | None = None) -> int:
-    """Run the oracle on a host-materialised scene; returns the fragment count."""
+def default_threads() -> int:
+    """Host threads for the tile-parallel oracle: the cores this process may
+    use, at most 16 (the GPU boxes' CPU share)."""
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def render(m: Materialized, prim_range: tuple[int, int] | None = None, threads: int = 1) -> int:
+    """Run the oracle on a host-materialised scene; returns the fragment
+    count.  threads > 1 runs the tile-parallel replay, which is bit-identical
+    to the single-threaded one (tests/test_oracle_parallel.py checks it)."""
     L = lib()
     n = len(m.draws)
     parr = (abi.PipelineDesc * len(m.pipelines))(*m.pipelines)
     darr = (abi.DrawDesc * max(n, 1))(*m.draws)
     dp = (C.c_uint32 * max(n, 1))(*(m.draw_pipeline or [0] * n))
     frags = C.c_uint64()
-    if prim_range is None:
+    if prim_range is None and threads > 1:
+        rc = L.cpvref_render_pass_parallel(C.byref(m.pass_desc), parr, darr, dp, n, threads, C.byref(frags))
+    elif prim_range is None:
         rc = L.cpvref_render_pass(C.byref(m.pass_desc), parr, darr, dp, n, C.byref(frags))
     else:
         rc = L.cpvref_render_pass_range(C.byref(m.pass_desc), parr, darr, dp, n, prim_range[0], prim_range[1],
@@ -80,10 +98,10 @@ def render(m: Materialized, prim_range: tuple[int, int] | None = None) -> int:
     return frags.value
 
 
-def render_scene(scene: Scene) -> tuple[np.ndarray, np.ndarray, int]:
+def render_scene(scene: Scene, threads: int = 1) -> tuple[np.ndarray, np.ndarray, int]:
     """Oracle image of a scene: (colour bytes, depth/stencil bytes, fragments)."""
     m = materialize(scene, HostMemory())
-    frags = render(m)
+    frags = render(m, threads=threads)
     return m.read_color(), m.read_ds(), frags
 
 
