@@ -604,6 +604,17 @@ static float dot3(const float a[3], const float b[3])
     return (x + y) + z;
 }
 
+/* glm::normalize for vec3 (Samples/utils/glm/detail/func_geometric.inl:256-266,
+ * func_exponential.inl:55-60): sqr = (x*x + y*y) + z*z, then v * (1 / sqrt(sqr)). */
+static void normalize3(const float v[3], float out[3])
+{
+    const float sqr = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    const float inv = 1.0f / sqrtf(sqr);
+    out[0] = v[0] * inv;
+    out[1] = v[1] * inv;
+    out[2] = v[2] * inv;
+}
+
 typedef struct {
     int nout;               /* varying floats written by the VS, in declaration order */
     int nin_attr;           /* number of vertex attributes read */
@@ -763,10 +774,8 @@ static int run_fs(const cpv_pipeline_desc* p, const cpv_draw_desc* d, const floa
     case CPV_FS_TEX_LAMBERT: {
         float t[4];
         cpvref_sample_implicit(&d->textures[1], in[3], in[4], t);
-        /* glm::normalize(v) = v * inversesqrt(dot(v, v)), inversesqrt = 1 / sqrt */
-        float dd = dot3(in, in);
-        float inv = 1.0f / sqrtf(dd);
-        float n[3] = {in[0] * inv, in[1] * inv, in[2] * inv};
+        float n[3];
+        normalize3(in, n);
         const float* L = uniform_ptr(d, 2);
         float l = dot3(n, L);
         /* FMax -> std::max(l, 0.0f) (GlslFunctions.cpp:122-126) */
@@ -1622,4 +1631,35 @@ int cpvref_blit_image(const cpv_image_level* src, const cpv_image_level* dst, co
         }
     }
     return CPV_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Test hooks                                                          */
+/* ------------------------------------------------------------------ */
+
+int cpvref_fragment(const cpv_pipeline_desc* p, const cpv_pass_desc* pass, int32_t x, int32_t y, float depth, int front,
+                    const float colour[4])
+{
+    if (!p || !pass || !colour) return CPV_ERROR_INVALID;
+    ctx_t c;
+    memset(&c, 0, sizeof(c));
+    c.pass = pass;
+    c.pipe = p;
+    c.has_color = pass->color_count > 0 && p->color_count > 0;
+    if (c.has_color && !fmt_get(pass->color[0].format, &c.cfmt)) return CPV_ERROR_UNSUPPORTED;
+    c.has_ds = pass->has_depth_stencil != 0;
+    if (c.has_ds && !fmt_get(pass->depth_stencil.format, &c.dsfmt)) return CPV_ERROR_UNSUPPORTED;
+    if (x < 0 || y < 0 || (uint32_t)x >= pass->width || (uint32_t)y >= pass->height) return CPV_ERROR_INVALID;
+    fragment_ops(&c, x, y, depth, front, colour);
+    return CPV_OK;
+}
+
+void cpvref_glm(uint32_t op, const float* a, const float* b, float* out)
+{
+    switch (op) {
+    case 0: mat4_mul_vec4(a, b, out); break;
+    case 1: out[0] = dot3(a, b); break;
+    case 2: normalize3(a, out); break;
+    default: break;
+    }
 }

@@ -76,6 +76,18 @@ int cpvref_blit_image(const cpv_image_level* src, const cpv_image_level* dst, co
 void cpvref_apply_blend(const float src[4], const float dst[4], const float constant[4],
                         const cpv_blend_attachment* state, float out[4]);
 
+/* The fragment wrapper after the shader (PipelineCompiler.cpp:1087-1727:
+ * depth bounds, stencil, depth test and writes, colour write with blend)
+ * for one fragment of colour `colour` at pixel (x, y) of the pass's
+ * attachments, with the viewport-transformed depth `depth`. */
+int cpvref_fragment(const cpv_pipeline_desc* pipeline, const cpv_pass_desc* pass, int32_t x, int32_t y, float depth,
+                    int front, const float colour[4]);
+
+/* The shaders' vector math in glm 0.9.5.3's operation order: op 0
+ * out = a * b (a a column-major mat4, b a vec4), op 1 out[0] = dot(a, b)
+ * (vec3), op 2 out = normalize(a) (vec3). */
+void cpvref_glm(uint32_t op, const float* a, const float* b, float* out);
+
 /* One edge function value and the coverage decision for one pixel of one
  * triangle exactly as ProcessTriangles/GetFragmentInput compute them.
  * positions: 3 clip-space vec4 (post-VS, before any swap). Returns 1 if the

@@ -58,6 +58,10 @@ def lib() -> C.CDLL:
         L.cpvref_apply_blend.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                          C.POINTER(abi.BlendAttachment), C.POINTER(C.c_float)]
         L.cpvref_apply_blend.restype = None
+        L.cpvref_fragment.argtypes = [C.POINTER(abi.PipelineDesc), C.POINTER(abi.PassDesc), C.c_int32, C.c_int32,
+                                      C.c_float, C.c_int, C.POINTER(C.c_float)]
+        L.cpvref_glm.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.cpvref_glm.restype = None
         L.cpvref_pixel_coverage.argtypes = [C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_float, C.c_float,
                                             C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_int),
                                             C.POINTER(C.c_float)]
