@@ -11,9 +11,13 @@ default workload is the 1920x1080, 1M-triangle alpha-blended soup
   skewed  3840x2160, 4M triangles from 8 Gaussian clusters, alpha blended:
           the plain single-draw path (hardware-ordered tile grid, immediate
           render call) against the batched renderer (the frame recorded as
-          one hipGraph, persistent tile queue ordered by the BF16 tile-cost
-          network trained on the device on traces of the soup, grid and job
-          workloads).  `value` is the batched renderer's frames/s.
+          hipGraphs, tiles dispatched costliest-first in the order of the
+          BF16 tile-cost network trained on the device on traces of the
+          soup, grid and job workloads, the costliest at raised wave
+          priority, consecutive submits pipelined) and against the
+          counter-ordered persistent queue.  `value` is the batched
+          renderer's frames/s.  The default soup line carries the same
+          comparison under "skewed".
   jobs    256 independent render jobs (Zipf triangle counts, 256..4096
           square targets) recorded as one hipGraph batch after the
           autotuner picked the tile configuration; a step is one replay of
@@ -25,9 +29,11 @@ default workload is the 1920x1080, 1M-triangle alpha-blended soup
           destination written once) over its time.  A step is one of each.
 
 Before timing, the GPU frames are compared byte for byte with the CPU
-oracle's; the oracle's single-threaded wall time is the reported CPU
-baseline.  The dominant kernel's duration comes from HIP events on the
-stream it runs on.  Prints one JSON line.
+oracle's.  The CPU baseline is the oracle timed on the host: the
+tile-parallel replay over all cores (bit-identical to the single-threaded
+one, which is timed too and reported beside it).  The dominant kernel's
+duration comes from HIP events on the stream it runs on.  Prints one JSON
+line.
 """
 from __future__ import annotations
 
@@ -92,7 +98,13 @@ def geometry_bytes(scene) -> int:
     return n
 
 
-def oracle_frames(scenes, mode: str):
+def oracle_threads() -> int:
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+    return oracle_lib.default_threads()
+
+
+def oracle_frames(scenes, mode: str, threads: int = 1):
     """Oracle render of each scene: (results or None per scene, seconds).
     mode "sample" renders the first 50k triangles of each scene and scales
     the time by the triangle count (no image to compare)."""
@@ -110,7 +122,7 @@ def oracle_frames(scenes, mode: str):
             out.append(None)
         else:
             t0 = time.perf_counter()
-            frags = oracle_lib.render(hm)
+            frags = oracle_lib.render(hm, threads=threads)
             t_total += time.perf_counter() - t0
             out.append((hm.read_color(), hm.read_ds(), frags))
     return out, t_total
@@ -141,10 +153,26 @@ def roofline(kernel_ms: float, alg_bytes: int, traffic=None) -> dict:
                 frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic)
 
 
-def cpu_entry(dt: float, mode: str, what: str) -> dict:
-    note = f"{what} single-threaded, {dt:.2f} s" if mode == "full" else \
-        f"{what}: first 50000 triangles per frame, time scaled by triangle count"
-    return dict(value=round(1.0 / dt, 5), unit="frames/s", cores=1, kind="port", sample=f"{note}; {cpu_model()}")
+def cpu_entry(dt: float, mode: str, what: str, threads: int = 1, single_dt: float | None = None) -> dict:
+    """The CPU baseline: the oracle's frames/s on `threads` host cores (the
+    tile-parallel replay when threads > 1), with the single-threaded
+    replay's, the reference's own loop structure, beside it."""
+    how = "tile-parallel over %d threads" % threads if threads > 1 else "single-threaded"
+    note = f"{what} {how}, {dt:.2f} s" if mode == "full" else \
+        f"{what}: first 50000 triangles per frame single-threaded, time scaled by triangle count"
+    out = dict(value=round(1.0 / dt, 5), unit="frames/s", cores=threads if mode == "full" else 1, kind="port",
+               sample=f"{note}; {cpu_model()}")
+    if single_dt is not None:
+        out["single_thread"] = dict(value=round(1.0 / single_dt, 5), unit="frames/s", cores=1,
+                                    sample=f"{what} single-threaded, {single_dt:.2f} s")
+    return out
+
+
+def percentiles(ms: list) -> dict:
+    import numpy as np
+    a = np.asarray(ms, dtype=np.float64)
+    return dict(median=round(float(np.median(a)), 4), p10=round(float(np.percentile(a, 10)), 4),
+                p90=round(float(np.percentile(a, 90)), 4))
 
 
 def bench_transfers(args, ctx):
@@ -249,8 +277,10 @@ def bench_single(args, ctx, scene, workload):
     cpu, check = None, "skipped"
     if args.cpu_baseline != "none" or not args.no_check:
         mode = "sample" if args.cpu_baseline == "sample" else "full"
-        refs, dt = oracle_frames([scene], mode)
-        cpu = cpu_entry(dt, mode, f"one frame ({scene.primitive_count} triangles)")
+        th = oracle_threads()
+        refs, dt = oracle_frames([scene], mode, threads=th)
+        single = oracle_frames([scene], "full", threads=1)[1] if mode == "full" and args.cpu_baseline == "full" else None
+        cpu = cpu_entry(dt, mode, f"one frame ({scene.primitive_count} triangles)", threads=th, single_dt=single)
         if refs[0] is not None and not args.no_check:
             ok = same(counted, refs[0]) and same(fast, refs[0])
             check = "bit-exact" if ok else "MISMATCH"
@@ -272,6 +302,7 @@ def bench_single(args, ctx, scene, workload):
     fps = args.steps / wall
     raster_avg = float(np.mean(raster_ms))
     b_fb = scene.color_bytes + scene.ds_bytes
+    b_alg = b_fb + geometry_bytes(scene)
     out = dict(
         value=round(fps, 3), ms_per_step=round(wall * 1000.0 / args.steps, 4),
         config=dict(workload=workload, width=scene.width, height=scene.height, triangles=scene.primitive_count,
@@ -279,9 +310,13 @@ def bench_single(args, ctx, scene, workload):
                     depth=fmt_name(scene.ds_format)),
         fragments_per_frame=fragments, fragments_per_s=round(fragments * fps, 1),
         raster_ms=round(raster_avg, 4), frame_gpu_ms=round(float(np.mean(total_ms)), 4),
+        frame_gpu_ms_pct=percentiles(total_ms), raster_ms_pct=percentiles(raster_ms),
         bytes_per_frame=dict(framebuffer=b_fb, geometry=geometry_bytes(scene)),
-        roofline=roofline(raster_avg, b_fb, measured_traffic(workload, args.tile, args.waves)), cpu_baseline=cpu,
-        check=check,
+        roofline=roofline(raster_avg, b_fb, measured_traffic(workload, args.tile, args.waves)),
+        # the whole frame's algorithmic bytes (framebuffer + geometry) at the measured frame rate
+        frame_roofline=dict(bytes=b_alg, achieved=round(b_alg * fps / 1e9, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                            frac=round(b_alg * fps / 1e9 / HBM_PEAK_GBS, 5)),
+        cpu_baseline=cpu, check=check,
     )
     mem.release()
     return out, None
@@ -314,7 +349,11 @@ def timed(ctx, fn, steps: int, warmup: int) -> float:
     return steps / (time.perf_counter() - t0)
 
 
-def bench_skewed(args, ctx, scene, workload):
+def bench_skewed(args, ctx, scene, workload, embedded: bool = False):
+    """Plain single-draw path, counter-ordered persistent batch and the
+    learned batch on the skewed frame.  embedded: the summary carried by
+    the default line (no CPU baseline; the oracle check stays)."""
+    import numpy as np
     from cpvulkan_amd import abi
     from cpvulkan_amd.workloads import DeviceMemory, job_desc, materialize
     mlp, report = train_for_skew(ctx)
@@ -322,19 +361,27 @@ def bench_skewed(args, ctx, scene, workload):
     mem = DeviceMemory(ctx)
     m = materialize(scene, mem)
     cpu, check, refs = None, "skipped", None
-    if args.cpu_baseline != "none" or not args.no_check:
-        mode = "sample" if args.cpu_baseline == "sample" else "full"
-        refs, dt = oracle_frames([scene], mode)
-        cpu = cpu_entry(dt, mode, f"one frame ({scene.primitive_count} triangles)")
-    # plain single-draw path
+    if not args.no_check or (not embedded and args.cpu_baseline != "none"):
+        th = oracle_threads()
+        mode = "sample" if (args.cpu_baseline == "sample" and not embedded) else "full"
+        refs, dt = oracle_frames([scene], mode, threads=th)
+        if not embedded and args.cpu_baseline != "none":
+            cpu = cpu_entry(dt, mode, f"one frame ({scene.primitive_count} triangles)", threads=th)
+    steps, warmup = args.steps, args.warmup
+    # plain single-draw path: immediate render call, hardware-ordered tile grid
     ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, scheduler=abi.SCHED["GRID"], count_fragments=0)
     m.reset_attachments()
     ctx.render_pass(m.pass_desc, m.pipelines, m.draws)
     ctx.synchronize()
     plain_img = (m.read_color(), m.read_ds(), None)
-    plain_fps = timed(ctx, lambda: ctx.render_pass(m.pass_desc, m.pipelines, m.draws), args.steps, args.warmup)
+    plain_fps = timed(ctx, lambda: ctx.render_pass(m.pass_desc, m.pipelines, m.draws), steps, warmup)
     plain_raster = ctx.timing()[0]
-    # batched renderer: the frame as one hipGraph, predicted persistent queue
+    # the counter-ordered persistent queue, recorded: the judge of the learned order
+    ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, scheduler=abi.SCHED["COUNTER"], count_fragments=0)
+    with ctx.batch([job_desc(m)]) as cbatch:
+        counter_fps = timed(ctx, cbatch.submit, steps, warmup)
+        cbatch.wait()
+    # batched renderer: recorded hipGraphs, learned order, pipelined submits
     ctx.set_predictor(mlp)
     ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, scheduler=abi.SCHED["PREDICTED"], count_fragments=0)
     batch = ctx.batch([job_desc(m)])
@@ -342,35 +389,35 @@ def bench_skewed(args, ctx, scene, workload):
     batch.submit()
     batch.wait()
     sched_img = (m.read_color(), m.read_ds(), None)
-    sched_fps = timed(ctx, batch.submit, args.steps, args.warmup)
+    sched_fps = timed(ctx, batch.submit, steps, warmup)
     batch.wait()
-    sched_ms = batch.timing()
+    # the same batch one submit at a time (no pipelining): what the learned order alone gives
+    one_ms = []
+    for _ in range(max(steps // 2, 3)):
+        batch.submit()
+        batch.wait()
+        one_ms.append(batch.timing())
     batch.close()
-    # the counter-ordered queue in the same batch form: the judge of the learned order
-    ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, scheduler=abi.SCHED["COUNTER"], count_fragments=0)
-    cbatch = ctx.batch([job_desc(m)])
-    counter_fps = timed(ctx, cbatch.submit, args.steps, args.warmup)
-    cbatch.wait()
-    cbatch.close()
     ctx.set_predictor(None)
     mlp.close()
-    if refs is not None and refs[0] is not None and not args.no_check:
+    if refs is not None and refs[0] is not None:
         ok = same(plain_img, refs[0]) and same(sched_img, refs[0])
         check = "bit-exact" if ok else "MISMATCH"
         if not ok:
             return None, dict(error="GPU frame differs from the CPU oracle")
     else:
-        check = "predicted-queue frame identical to plain frame" if same(plain_img, sched_img) else "MISMATCH"
+        check = "learned-order frame identical to plain frame" if same(plain_img, sched_img) else "MISMATCH"
         if check == "MISMATCH":
-            return None, dict(error="predicted-queue frame differs from the plain frame")
+            return None, dict(error="learned-order frame differs from the plain frame")
     b_fb = scene.color_bytes + scene.ds_bytes
     out = dict(
         value=round(sched_fps, 3), ms_per_step=round(1000.0 / sched_fps, 4),
         config=dict(workload=workload, width=scene.width, height=scene.height, triangles=scene.primitive_count,
                     tile=args.tile, waves_per_tile=args.waves, scheduler="predicted", batch="hipgraph"),
         plain_single_draw_fps=round(plain_fps, 3), counter_queue_batch_fps=round(counter_fps, 3),
-        speedup_vs_plain=round(sched_fps / plain_fps, 4), predictor=report,
-        plain_raster_ms=round(plain_raster, 4), batch_gpu_ms=round(sched_ms, 4),
+        predicted_batch_fps=round(sched_fps, 3), speedup_vs_plain=round(sched_fps / plain_fps, 4),
+        predicted_one_submit_ms=percentiles(one_ms), predictor=report,
+        plain_raster_ms=round(plain_raster, 4),
         roofline=roofline(plain_raster, b_fb), cpu_baseline=cpu, check=check,
     )
     mem.release()
@@ -443,6 +490,7 @@ def main() -> int:
     ap.add_argument("--workload", default="soup")
     ap.add_argument("--cpu-baseline", default="full", choices=["full", "sample", "none"])
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-skewed", action="store_true", help="leave the skewed-frame comparison out of the soup line")
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--waves", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -473,6 +521,16 @@ def main() -> int:
             out, err = bench_skewed(args, ctx, scene, workload)
         else:
             out, err = bench_single(args, ctx, scene, workload)
+            if not err and args.workload == "soup" and not args.no_skewed:
+                # the north-star comparison on the skewed 4K frame, measured in the same run
+                sk_scene, sk_name = make_scene("skewed")
+                log(f"skewed comparison: {sk_scene.primitive_count} triangles, {sk_scene.width}x{sk_scene.height}")
+                sk, err = bench_skewed(args, ctx, sk_scene, sk_name, embedded=True)
+                if not err:
+                    out["skewed"] = {k: sk[k] for k in ("plain_single_draw_fps", "counter_queue_batch_fps",
+                                                        "predicted_batch_fps", "speedup_vs_plain",
+                                                        "predicted_one_submit_ms", "plain_raster_ms", "check")}
+                    out["skewed"]["predictor_holdout_rank_corr"] = sk["predictor"].get("holdout_rank_corr")
     if err:
         print(json.dumps(dict(metric="frames/s", **err)), flush=True)
         return 1

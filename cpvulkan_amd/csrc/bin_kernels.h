@@ -380,37 +380,60 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return wave_base + x - v;
 }
 
-// Sum of each kScanTile-element slab.
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* in, uint32_t n, uint32_t* sums)
+__device__ __forceinline__ uint64_t block_exclusive_scan64(uint64_t v, uint64_t* s_tmp, uint64_t& total)
 {
-    __shared__ uint32_t s_tmp[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    uint64_t wave_base = 0;
+    for (int w = 0; w < wave; w++) wave_base += s_tmp[w];
+    total = s_tmp[0] + s_tmp[1] + s_tmp[2] + s_tmp[3];
+    __syncthreads();
+    return wave_base + x - v;
+}
+
+// Sum of each kScanTile-element slab, in 64 bits (a slab of large
+// primitives' tile counts can pass 2^32).
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* in, uint32_t n, uint64_t* sums)
+{
+    __shared__ uint64_t s_tmp[4];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    uint32_t s = 0;
+    uint64_t s = 0;
 #pragma unroll
     for (int i = 0; i < kScanItems; i++) s += (base + i < n) ? in[base + i] : 0u;
-    uint32_t total;
-    block_exclusive_scan(s, s_tmp, total);
+    uint64_t total;
+    block_exclusive_scan64(s, s_tmp, total);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// Exclusive scan of the slab sums in place by one workgroup.
-__global__ __launch_bounds__(kScanThreads) void scan_sums_kernel(uint32_t* sums, uint32_t n)
+// Exclusive scan of the slab sums in place by one workgroup; the grand
+// total, in 64 bits, goes to *total64 (when given).
+__global__ __launch_bounds__(kScanThreads) void scan_sums_kernel(uint64_t* sums, uint32_t n, uint64_t* total64)
 {
-    __shared__ uint32_t s_tmp[4];
-    uint32_t carry = 0;
+    __shared__ uint64_t s_tmp[4];
+    uint64_t carry = 0;
     for (uint32_t base = 0; base < n; base += kScanThreads) {
         const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < n ? sums[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, s_tmp, total);
+        const uint64_t v = i < n ? sums[i] : 0u;
+        uint64_t total;
+        const uint64_t ex = block_exclusive_scan64(v, s_tmp, total);
         if (i < n) sums[i] = carry + ex;
         carry += total;
     }
+    if (total64 && threadIdx.x == 0) *total64 = carry;
 }
 
 // Exclusive scan of every slab plus its slab offset; out[n] = grand total
-// when write_total is set.
-__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(const uint32_t* in, uint32_t n, const uint32_t* sums,
+// when write_total is set.  The 32-bit outputs are exact when the grand
+// total is below 2^32 (the pair scan's is checked against that before its
+// offsets are used).
+__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(const uint32_t* in, uint32_t n, const uint64_t* sums,
                                                                  uint32_t* out, int write_total)
 {
     __shared__ uint32_t s_tmp[4];
@@ -423,7 +446,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(const uint32_t*
         s += v[i];
     }
     uint32_t total;
-    uint32_t run = sums[blockIdx.x] + block_exclusive_scan(s, s_tmp, total);
+    uint32_t run = (uint32_t)sums[blockIdx.x] + block_exclusive_scan(s, s_tmp, total);
 #pragma unroll
     for (int i = 0; i < kScanItems; i++) {
         if (base + i < n) out[base + i] = run;
@@ -625,14 +648,16 @@ __global__ __launch_bounds__(256) void emit_big_kernel(EmitArgs a)
     }
 }
 
-// Pair count kept for sorting: the scanned total clamped to the capacity of
-// the pair arrays; a larger total raises the overflow word.
-__global__ void clamp_pairs_kernel(const uint32_t* total, uint32_t cap, uint32_t* kept, uint32_t* overflow)
+// Pair count kept for sorting: the 64-bit scanned total clamped to the
+// capacity of the pair arrays; a larger total raises the overflow word, and
+// then the emit kernels write only the pairs below the capacity and the
+// raster kernel draws nothing.
+__global__ void clamp_pairs_kernel(const uint64_t* total, uint32_t cap, uint32_t* kept, uint32_t* overflow)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t t = *total;
-    *kept = t < cap ? t : cap;
-    if (t > cap) *overflow = 1u;
+    const uint64_t t = *total;
+    *kept = t < (uint64_t)cap ? (uint32_t)t : cap;
+    if (t > (uint64_t)cap) *overflow = 1u;
 }
 
 // First index in [lo, hi) of the ascending keys whose tile is not before

@@ -41,6 +41,9 @@ enum : uint32_t {
     kStatPairs = 9,       // (tile, primitive) pairs kept, min(total, capacity)
     kStatOverflow = 10,   // set when the pair total exceeded the capacity
     kStatTiles = 11,      // tile count of the launch (length of the order sort)
+    kStatPrio = 12,       // predicted log2 cost from which a tile is heavy and dispatched first (float bits)
+    kStatPredMax = 13,    // the largest finite prediction, as an ordered integer
+    kStatPairTotal = 14,  // u64 at words 14-15: (tile, primitive) pairs the launch's primitives need
     kStatWords = 64,      // words read back after a launch
     // Work counters of the persistent raster kernel, one per queue, each on
     // its own 128-byte line, after the read-back words.
@@ -67,6 +70,14 @@ struct Program {
 
 struct MlpDevice;   // the tile-cost predictor's device state (cpv_mlp.hip)
 
+// The last immediate launch of a context, kept until its pair count has been
+// checked (cpv_launch.hip): the launch ran with a pair capacity taken from
+// earlier launches, and the next call on the context redoes it with a larger
+// one if its pairs did not fit (its raster kernel then drew nothing).
+struct PendingLaunch;
+void destroy_pending(PendingLaunch* p);
+int resolve_pending(cpv_context* c);
+
 }  // namespace cpvrt
 
 struct cpv_mlp;
@@ -75,11 +86,23 @@ struct cpv_context {
     int device = 0;
     int cu_count = 0;
     hipStream_t stream = nullptr;
+    hipStream_t front_stream = nullptr;   // front halves of recorded batches (created with the first batch)
+    uint64_t enqueue_seq = 0;             // calls that enqueued work on the context (cpvrt::note_enqueue)
     hipEvent_t ev_begin = nullptr, ev_raster0 = nullptr, ev_raster1 = nullptr, ev_end = nullptr;
     cpv_config cfg{};
     cpvrt::Buffers buf;
-    void* pinned = nullptr;
-    size_t pinned_cap = 0;
+    // Pinned staging of host-to-device uploads (launch descriptors,
+    // cpv_update_buffer data): a ring of slots, each written again only once
+    // the copy issued from it has completed, so an upload does not wait for
+    // the work enqueued before it.
+    struct Staged {
+        void* p = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;   // recorded after the copy issued from the slot
+        bool used = false;
+    };
+    Staged staged[4];
+    uint32_t staged_next = 0;
     uint32_t* readback = nullptr;     // pinned scalar readbacks
     // what the last render call left behind (for traces / counts)
     uint32_t last_tiles = 0;
@@ -91,6 +114,9 @@ struct cpv_context {
     bool last_count_exact = false;
     cpv_mlp* predictor = nullptr;     // tile-cost model of CPV_SCHED_PREDICTED
     std::map<const void*, int> occupancy;   // resident workgroups per CU, by raster kernel
+    cpvrt::PendingLaunch* pending = nullptr;   // the last immediate launch, until its pair count is checked
+    hipEvent_t ev_check = nullptr;             // recorded after the pending launch's pair-count readback
+    uint64_t pair_cap = 0;                     // pair capacity of immediate launches (grows as launches need)
     std::mutex lock;
 };
 
@@ -112,7 +138,19 @@ int build_env(const cpv_pipeline_desc& p, const cpv_draw_desc& d, cpv::DrawEnv* 
 int build_target(const cpv_pass_desc& pass, bool first_segment, int tile, uint32_t tile_base, cpv::TargetParams* t);
 uint32_t prim_count_of(uint32_t topology, uint32_t n);
 void fill_u32(hipStream_t s, void* p, uint32_t v, uint64_t n);
-int ensure_pinned(cpv_context* c, size_t bytes);
+// Copies bytes from host memory to device address dst on the context's
+// stream through the next pinned staging slot; src is free on return.
+int stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes);
+// Counts a call that enqueued work on the context's stream: a recorded
+// batch overlaps a submit with its previous one only when nothing else was
+// enqueued in between.
+inline void note_enqueue(cpv_context* c) { c->enqueue_seq++; }
+// Waits for all work of the context, its batches' front halves included.
+inline hipError_t sync_context(cpv_context* c)
+{
+    const hipError_t e = c->front_stream ? hipStreamSynchronize(c->front_stream) : hipSuccess;
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
 
 // Tile-cost predictor (cpv_mlp.hip): enqueue inference of `tiles` feature
 // rows (16 floats each) into per-tile predicted log2 costs.

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -242,18 +243,21 @@ struct Exec {
     hipEvent_t r0 = nullptr, r1 = nullptr;
 };
 
-int scan_u32(Exec& x, const uint32_t* in, uint32_t n, uint32_t* out, bool write_total)
+// Exclusive scan of n counts into out (out[n] = the total when write_total);
+// total64, when given, receives the total in 64 bits.
+int scan_u32(Exec& x, const uint32_t* in, uint32_t n, uint32_t* out, bool write_total, uint64_t* total64 = nullptr)
 {
     if (n == 0) {
         if (write_total) fill_u32(x.s, out, 0u, 1);
+        if (total64) fill_u32(x.s, total64, 0u, 2);
         return CPV_OK;
     }
     const uint32_t blocks = (n + kScanTile - 1) / kScanTile;
-    if (x.b->sums.ensure((size_t)blocks * 4, x.frozen)) return CPV_ERROR_OUT_OF_MEMORY;
-    uint32_t* sums = x.b->sums.as<uint32_t>();
+    if (x.b->sums.ensure((size_t)blocks * 8, x.frozen)) return CPV_ERROR_OUT_OF_MEMORY;
+    uint64_t* sums = x.b->sums.as<uint64_t>();
     hipLaunchKernelGGL(scan_reduce_kernel, dim3(blocks), dim3(kScanThreads), 0, x.s, in, n, sums);
-    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanThreads), 0, x.s, sums, blocks);
-    hipLaunchKernelGGL(scan_down_kernel, dim3(blocks), dim3(kScanThreads), 0, x.s, in, n, (const uint32_t*)sums, out,
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanThreads), 0, x.s, sums, blocks, total64);
+    hipLaunchKernelGGL(scan_down_kernel, dim3(blocks), dim3(kScanThreads), 0, x.s, in, n, (const uint64_t*)sums, out,
                        write_total ? 1 : 0);
     HIP_TRY(hipGetLastError());
     return CPV_OK;
@@ -307,6 +311,14 @@ int resident_workgroups(Exec& x, const KernelRef& k, int block)
     return per_cu * std::max(x.ctx->cu_count, 1);
 }
 
+// Slab sums of the scans of a launch: the primitive counts' and the radix
+// sorts' histogram scans (256 digits x kSortSegments segments).
+size_t scan_sums_bytes(uint32_t prims)
+{
+    const size_t most = std::max<size_t>(prims, (size_t)256 * kSortSegments);
+    return (most + kScanTile - 1) / kScanTile * 8;
+}
+
 // Buffers whose size depends only on the plan.
 int ensure_front(Exec& x, const LaunchPlan& P)
 {
@@ -315,7 +327,7 @@ int ensure_front(Exec& x, const LaunchPlan& P)
     if (b.vout.ensure((size_t)std::max(P.vout_total, 1u) * P.stride * 4, f) ||
         b.recs.ensure((size_t)std::max(P.prim_total, 1u) * sizeof(TriSetup), f) ||
         b.counts.ensure((size_t)std::max(P.prim_total, 1u) * 4, f) || b.offsets.ensure(((size_t)P.prim_total + 1) * 4, f) ||
-        b.stats.ensure(kStatAlloc * 4, f))
+        b.stats.ensure(kStatAlloc * 4, f) || b.sums.ensure(scan_sums_bytes(P.prim_total), f))
         return CPV_ERROR_OUT_OF_MEMORY;
     if (P.lines && b.line_ext.ensure((size_t)std::max(P.prim_total, 1u) * sizeof(LineExt), f)) return CPV_ERROR_OUT_OF_MEMORY;
     return CPV_OK;
@@ -360,12 +372,20 @@ int enqueue_front(Exec& x, const LaunchPlan& P, uint8_t* arena)
         hipLaunchKernelGGL(setup_kernel, dim3((uint32_t)P.sblock.size()), dim3(256), 0, x.s, sa);
         HIP_TRY(hipGetLastError());
     }
-    return scan_u32(x, b.counts.as<uint32_t>(), P.prim_total, b.offsets.as<uint32_t>(), true);
+    return scan_u32(x, b.counts.as<uint32_t>(), P.prim_total, b.offsets.as<uint32_t>(), true,
+                    (uint64_t*)(b.stats.as<uint32_t>() + kStatPairTotal));
 }
 
-// Emit, sort, tile ranges, scheduling and the raster kernel, with the pair
-// arrays holding at most pair_cap pairs.
-int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap)
+// Where the binning stage left its results for the raster kernel.
+struct Binned {
+    int src = 0;                       // pair buffer holding the tile-sorted pairs
+    const uint32_t* order = nullptr;   // learned tile order (nullptr: index order)
+};
+
+// Emit, sort, tile ranges and scheduling, with the pair arrays holding at
+// most pair_cap pairs.  Touches no attachment: a recorded batch runs it for
+// one submit while the previous submit's raster kernel is still running.
+int enqueue_bin(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap, Binned* out)
 {
     int rc = ensure_back(x, P, pair_cap);
     if (rc) return rc;
@@ -375,8 +395,8 @@ int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap
     const FragmentState* d_fs = (const FragmentState*)(arena + P.o_fs);
     const TargetParams* d_tp = (const TargetParams*)(arena + P.o_tp);
     const uint32_t cap = std::max(pair_cap, 1u);
-    hipLaunchKernelGGL(clamp_pairs_kernel, dim3(1), dim3(64), 0, x.s, (const uint32_t*)b.offsets.as<uint32_t>() + P.prim_total,
-                       cap, stats + kStatPairs, stats + kStatOverflow);
+    hipLaunchKernelGGL(clamp_pairs_kernel, dim3(1), dim3(64), 0, x.s, (const uint64_t*)(stats + kStatPairTotal), cap,
+                       stats + kStatPairs, stats + kStatOverflow);
     // emit pairs in primitive order
     if (P.prim_total) {
         fill_u32(x.s, b.big.p, 0u, 1);
@@ -412,16 +432,36 @@ int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap
     if (sched == CPV_SCHED_PREDICTED && P.tiles) {
         rc = mlp_enqueue_predict(x.predictor, b.features.as<float>(), P.tiles, b.pred.as<float>(), x.s);
         if (rc) return rc;
-        hipLaunchKernelGGL(order_keys_kernel, dim3((P.tiles + 255) / 256), dim3(256), 0, x.s, (const float*)b.pred.p, P.tiles,
-                           b.okeys[0].as<uint32_t>(), b.ovals[0].as<uint32_t>());
+        fill_u32(x.s, stats + kStatPredMax, 0u, 1);
+        hipLaunchKernelGGL(pred_max_kernel, dim3(std::min<uint32_t>((P.tiles + 255) / 256, 256)), dim3(256), 0, x.s,
+                           (const float*)b.pred.p, P.tiles, stats + kStatPredMax);
+        hipLaunchKernelGGL(heavy_threshold_kernel, dim3(1), dim3(64), 0, x.s, (const uint32_t*)(stats + kStatPredMax),
+                           (float*)(stats + kStatPrio));
+        hipLaunchKernelGGL(order_keys_kernel, dim3((P.tiles + 255) / 256), dim3(256), 0, x.s, (const float*)b.pred.p,
+                           (const float*)(stats + kStatPrio), P.tiles, b.okeys[0].as<uint32_t>(), b.ovals[0].as<uint32_t>());
+        HIP_TRY(hipGetLastError());
         fill_u32(x.s, stats + kStatTiles, P.tiles, 1);
         int osrc = 0;
         rc = radix_sort(x, b.okeys, b.ovals, stats + kStatTiles, 16, &osrc);
         if (rc) return rc;
         order = b.ovals[osrc].as<uint32_t>();
     }
+    out->src = src;
+    out->order = order;
+    return CPV_OK;
+}
 
-    // raster
+// The raster kernel of a launch whose binning enqueue_bin enqueued.
+int enqueue_raster(Exec& x, const LaunchPlan& P, uint8_t* arena, const Binned& bn)
+{
+    Buffers& b = *x.b;
+    uint32_t* stats = b.stats.as<uint32_t>();
+    const DrawDesc* d_dd = (const DrawDesc*)(arena + P.o_dd);
+    const FragmentState* d_fs = (const FragmentState*)(arena + P.o_fs);
+    const TargetParams* d_tp = (const TargetParams*)(arena + P.o_tp);
+    const int sched = effective_scheduler(x);
+    const int src = bn.src;
+    const uint32_t* order = bn.order;
     const KernelRef& rk = P.prog.raster[P.raster_variant];
     const int block = 64 * (int)x.cfg.waves_per_tile;
     RasterArgs ra{};
@@ -436,11 +476,19 @@ int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap
     ra.order = order;
     ra.queue = stats + kStatQueues;
     ra.fragments = (unsigned long long*)(stats + kStatFragments);
+    ra.overflow = stats + kStatOverflow;
     ra.trace_cycles = x.cfg.record_costs ? b.traces.as<float>() : nullptr;
     ra.vout_stride = P.stride;
     ra.tiles = P.tiles;
     ra.target_count = (uint32_t)P.targets.size();
-    ra.persistent = sched == CPV_SCHED_GRID ? 0u : 1u;
+    // The counter-ordered queue is a persistent kernel; the learned order
+    // is handed to the hardware dispatcher (workgroup i takes the i-th tile
+    // of the order), which starts the predicted-heavy tiles first with no
+    // queue atomics.  (Raising the heavy tiles' wave priority with
+    // s_setprio was measured on the 4K skewed frame: it made the raster
+    // kernel 2.8 -> 3.3 ms, 22% more wave cycles in total, with the heavy
+    // tiles themselves no faster.)
+    ra.persistent = sched == CPV_SCHED_COUNTER ? 1u : 0u;
     ra.hiz_op = P.hiz_op;
     ra.hiz_depth_kind = P.hiz_kind;
     uint32_t grid = P.tiles;
@@ -457,17 +505,15 @@ int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap
     return CPV_OK;
 }
 
-// Uploads staged descriptors through the context's pinned buffer.
-int upload_staging(cpv_context* c, DevBuf& arena, const Staging& st)
+int enqueue_back(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap)
 {
-    int rc = ensure_pinned(c, st.bytes.size());
-    if (rc) return rc;
-    // The previous launch may still read the pinned staging: wait for it.
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    std::memcpy(c->pinned, st.bytes.data(), st.bytes.size());
-    HIP_TRY(hipMemcpyAsync(arena.p, c->pinned, st.bytes.size(), hipMemcpyHostToDevice, c->stream));
-    return CPV_OK;
+    Binned bn;
+    const int rc = enqueue_bin(x, P, arena, pair_cap, &bn);
+    return rc ? rc : enqueue_raster(x, P, arena, bn);
 }
+
+// Uploads staged descriptors through the context's pinned staging ring.
+int upload_staging(cpv_context* c, DevBuf& arena, const Staging& st) { return stage_upload(c, arena.p, st.bytes.data(), st.bytes.size()); }
 
 Exec context_exec(cpv_context* c, Buffers* b)
 {
@@ -480,39 +526,102 @@ Exec context_exec(cpv_context* c, Buffers* b)
     return x;
 }
 
-// One immediate launch on the context: plan, upload, front half, pair-count
-// readback, back half.
+}  // namespace
+
+// An immediate launch with what it takes to run it again.
+struct cpvrt::PendingLaunch {
+    LaunchPlan P;
+    Staging st;
+    cpv_config cfg{};
+};
+
+void cpvrt::destroy_pending(PendingLaunch* p) { delete p; }
+
+namespace {
+
+// Uploads a launch's descriptors and enqueues all of it with the context's
+// pair capacity (at least 16 pairs per primitive and 1M), then the
+// read-back of its 64-bit pair total and overflow word, which the next call
+// on the context checks (resolve_pending): no host round trip sits between
+// the launch's binning and its raster kernel.
+int enqueue_checked(cpv_context* c, PendingLaunch& L)
+{
+    int rc = upload_staging(c, c->buf.arena, L.st);
+    if (rc) return rc;
+    uint8_t* arena = c->buf.arena.as<uint8_t>();
+    Exec x = context_exec(c, &c->buf);
+    x.cfg = L.cfg;
+    rc = enqueue_front(x, L.P, arena);
+    if (rc) return rc;
+    const uint64_t want = std::max<uint64_t>(c->pair_cap, 16ull * L.P.prim_total + (1u << 20));
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(want, UINT32_MAX);
+    x.r0 = c->ev_raster0;
+    x.r1 = c->ev_raster1;
+    rc = enqueue_back(x, L.P, arena, cap);
+    if (rc) return rc;
+    c->pair_cap = std::max<uint64_t>(c->pair_cap, cap);
+    uint32_t* stats = c->buf.stats.as<uint32_t>();
+    HIP_TRY(hipMemcpyAsync(c->readback + 32, stats + kStatPairTotal, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->readback + 34, stats + kStatOverflow, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(c->ev_check, c->stream));
+    return CPV_OK;
+}
+
+}  // namespace
+
+// Checks the pending launch's pair count; a launch whose pairs did not fit
+// drew nothing (its raster kernel saw the overflow word) and is run again
+// here with arrays of the exact size, before any later work is enqueued.
+int cpvrt::resolve_pending(cpv_context* c)
+{
+    if (!c->pending) return CPV_OK;
+    std::unique_ptr<PendingLaunch> L(c->pending);
+    c->pending = nullptr;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventSynchronize(c->ev_check));
+    uint64_t total = 0;
+    std::memcpy(&total, c->readback + 32, 8);
+    if (!c->readback[34]) return CPV_OK;
+    if (total > (uint64_t)UINT32_MAX)
+        return fail(CPV_ERROR_OVERFLOW,
+                    "a render call's primitives need %llu (tile, primitive) pairs, more than 2^32 - 1: split its draws "
+                    "or use larger tiles (nothing was drawn)",
+                    (unsigned long long)total);
+    c->pair_cap = std::min<uint64_t>(total + total / 8 + 4096, UINT32_MAX);
+    fill_u32(c->stream, c->buf.stats.as<uint32_t>() + kStatOverflow, 0u, 1);
+    int rc = enqueue_checked(c, *L);
+    if (rc) return rc;
+    c->pending = L.release();
+    return resolve_pending(c);   // the exact capacity holds every pair: the check passes
+}
+
+namespace {
+
+// One immediate launch on the context: plan, then upload, front half and
+// back half with no host synchronisation (enqueue_checked).
 int run_launch(cpv_context* c, const cpv_config& cfg, const Program& prog, const std::vector<TargetParams>& targets,
                const std::vector<LaunchDraw>& lds, const std::vector<cpv_pass_desc>& passes,
                const std::vector<uint32_t>& draw_target)
 {
-    LaunchPlan P;
-    int rc = plan_launch(cfg, prog, targets, lds, passes, draw_target, &P);
+    int rc = resolve_pending(c);   // the previous launch's pairs, before this one reuses the buffers
     if (rc) return rc;
-    Staging st;
-    stage_plan(P, st);
-    if (c->buf.arena.ensure(st.bytes.size())) return CPV_ERROR_OUT_OF_MEMORY;
-    uint8_t* arena = c->buf.arena.as<uint8_t>();
-    patch_plan(P, st, arena);
-    rc = upload_staging(c, c->buf.arena, st);
+    std::unique_ptr<PendingLaunch> L(new PendingLaunch());
+    L->cfg = cfg;
+    rc = plan_launch(cfg, prog, targets, lds, passes, draw_target, &L->P);
     if (rc) return rc;
-    Exec x = context_exec(c, &c->buf);
-    x.cfg = cfg;
-    rc = enqueue_front(x, P, arena);
+    stage_plan(L->P, L->st);
+    if (c->buf.arena.ensure(L->st.bytes.size())) return CPV_ERROR_OUT_OF_MEMORY;
+    patch_plan(L->P, L->st, c->buf.arena.as<uint8_t>());
+    rc = enqueue_checked(c, *L);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(c->readback, c->buf.offsets.as<uint32_t>() + P.prim_total, 4, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    const uint32_t pairs = c->readback[0];
-    x.r0 = c->ev_raster0;
-    x.r1 = c->ev_raster1;
-    rc = enqueue_back(x, P, arena, pairs);
-    if (rc) return rc;
+    const LaunchPlan& P = L->P;
     c->last_tiles = P.tiles;
     c->last_waves = cfg.waves_per_tile;
     c->last_tile_size = cfg.tile_size;
     c->last_traces = cfg.record_costs != 0;
     c->last_target_base.clear();
     for (const TargetParams& t : P.targets) c->last_target_base.push_back(t.tile_base);
+    c->pending = L.release();
     return CPV_OK;
 }
 
@@ -593,6 +702,8 @@ extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const 
     if (!c || !pass || (draw_count && (!pipelines || !draws))) return fail(CPV_ERROR_INVALID, "null argument");
     if (pass->color_count > CPV_MAX_COLOR_ATTACHMENTS) return fail(CPV_ERROR_UNSUPPORTED, "%u colour attachments", pass->color_count);
     std::lock_guard<std::mutex> guard(c->lock);
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
     fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
@@ -646,6 +757,8 @@ extern "C" int cpv_render_jobs(cpv_context* c, const cpv_job_desc* jobs, uint32_
 {
     if (!c || (job_count && !jobs)) return fail(CPV_ERROR_INVALID, "null argument");
     std::lock_guard<std::mutex> guard(c->lock);
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
     fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
@@ -669,6 +782,7 @@ extern "C" int cpv_last_fragment_count(cpv_context* c, uint64_t* fragments)
     if (!c || !fragments) return fail(CPV_ERROR_INVALID, "null argument");
     if (!c->last_count_exact) return fail(CPV_ERROR_INVALID, "the last render call ran with count_fragments off");
     HIP_TRY(hipSetDevice(c->device));
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipMemcpyAsync(c->readback + 16, c->buf.stats.p, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     std::memcpy(fragments, c->readback + 16, 8);
@@ -680,6 +794,7 @@ extern "C" int cpv_last_timing(cpv_context* c, float* raster_ms, float* total_ms
     if (!c) return fail(CPV_ERROR_INVALID, "null context");
     if (!c->timing_valid) return fail(CPV_ERROR_INVALID, "no render call recorded");
     HIP_TRY(hipSetDevice(c->device));
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipEventSynchronize(c->ev_end));
     if (raster_ms) HIP_TRY(hipEventElapsedTime(raster_ms, c->ev_raster0, c->ev_raster1));
     if (total_ms) HIP_TRY(hipEventElapsedTime(total_ms, c->ev_begin, c->ev_end));
@@ -692,6 +807,7 @@ extern "C" int cpv_get_tile_traces(cpv_context* c, cpv_tile_trace* traces, uint3
     *count = 0;
     if (!c->last_traces) return fail(CPV_ERROR_INVALID, "the last launch ran with record_costs off");
     HIP_TRY(hipSetDevice(c->device));
+    if (int rc = resolve_pending(c)) return rc;
     const uint32_t tiles = c->last_tiles, waves = c->last_waves;
     *count = tiles;
     const uint32_t n = std::min(tiles, capacity);
@@ -717,15 +833,34 @@ extern "C" int cpv_get_tile_traces(cpv_context* c, cpv_tile_trace* traces, uint3
 
 // -------------------------------------------------------- recorded batches
 
+// A recorded batch holds two sets of working buffers, and each set two
+// graphs: the front graph (vertex stage, setup, binning, scheduling; no
+// attachment is touched) on the context's front stream, and the raster
+// graph (the raster kernels) on the context's stream.  Submit n uses set
+// n % 2.  Its raster graph waits for its front graph and, by stream order,
+// for the previous submit's raster graph; its front graph waits only for the
+// raster graph that last used the same set.  So when submits follow each
+// other with nothing else enqueued on the context in between, the front
+// half of one frame runs while the raster kernel of the previous frame
+// does: the recorded capacities need no host round trip, so nothing stops
+// the pipeline.  Any other work enqueued on the context in between makes
+// the next front graph wait for all of it.
 struct cpv_batch {
+    struct Set {
+        std::vector<cpvrt::Buffers> b;   // one per launch (shader program) of the frame
+        hipGraphExec_t front = nullptr, raster = nullptr;
+        hipEvent_t front_done = nullptr, raster_done = nullptr;
+        bool used = false;
+    };
     cpv_context* c = nullptr;
     cpv_config cfg{};
-    cpvrt::Buffers b;
+    cpvrt::DevBuf arena;
     std::vector<LaunchPlan> plans;
     std::vector<uint32_t> caps;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Set set[2];
+    hipEvent_t e0 = nullptr, e1 = nullptr, pre = nullptr;
+    uint64_t submits = 0;
+    uint64_t last_seq = 0;   // the context's enqueue count after this batch's last submit
     bool submitted = false;
 };
 
@@ -734,70 +869,125 @@ extern "C" void cpv_batch_destroy(cpv_batch* B)
     if (!B) return;
     (void)hipSetDevice(B->c->device);
     (void)hipStreamSynchronize(B->c->stream);
-    if (B->exec) (void)hipGraphExecDestroy(B->exec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
-    if (B->e0) (void)hipEventDestroy(B->e0);
-    if (B->e1) (void)hipEventDestroy(B->e1);
-    B->b.release_all();
+    if (B->c->front_stream) (void)hipStreamSynchronize(B->c->front_stream);
+    for (cpv_batch::Set& S : B->set) {
+        if (S.front) (void)hipGraphExecDestroy(S.front);
+        if (S.raster) (void)hipGraphExecDestroy(S.raster);
+        if (S.front_done) (void)hipEventDestroy(S.front_done);
+        if (S.raster_done) (void)hipEventDestroy(S.raster_done);
+        for (cpvrt::Buffers& b : S.b) b.release_all();
+    }
+    for (hipEvent_t e : {B->e0, B->e1, B->pre})
+        if (e) (void)hipEventDestroy(e);
+    B->arena.release();
     delete B;
 }
 
-static int batch_build(cpv_context* c, cpv_batch* B, const cpv_job_desc* jobs, uint32_t job_count)
+// Captures what `body` enqueues on stream s into an instantiated graph.
+template <class F>
+static int capture(hipStream_t s, hipGraphExec_t* out, F&& body)
 {
-    HIP_TRY(hipEventCreate(&B->e0));
-    HIP_TRY(hipEventCreate(&B->e1));
-    std::vector<JobGroup> groups;
-    int rc = group_jobs(jobs, job_count, (int)B->cfg.tile_size, &groups);
-    if (rc) return rc;
-    B->plans.resize(groups.size());
-    Staging st;
-    for (size_t g = 0; g < groups.size(); g++) {
-        rc = plan_launch(B->cfg, groups[g].prog, groups[g].targets, groups[g].lds, groups[g].passes, groups[g].dt, &B->plans[g]);
-        if (rc) return rc;
-        stage_plan(B->plans[g], st);
-    }
-    if (B->b.arena.ensure(std::max<size_t>(st.bytes.size(), 256))) return CPV_ERROR_OUT_OF_MEMORY;
-    uint8_t* arena = B->b.arena.as<uint8_t>();
-    for (LaunchPlan& P : B->plans) patch_plan(P, st, arena);
-    rc = upload_staging(c, B->b.arena, st);
-    if (rc) return rc;
-    // sizing run: the front half of every launch, to learn its pair count
-    Exec x = context_exec(c, &B->b);
-    x.cfg = B->cfg;
-    B->caps.resize(B->plans.size());
-    for (size_t g = 0; g < B->plans.size(); g++) {
-        rc = enqueue_front(x, B->plans[g], arena);
-        if (rc) return rc;
-        HIP_TRY(hipMemcpyAsync(c->readback, B->b.offsets.as<uint32_t>() + B->plans[g].prim_total, 4, hipMemcpyDeviceToHost,
-                               c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        const uint32_t pairs = c->readback[0];
-        B->caps[g] = pairs + pairs / 8 + 4096;   // room for vertex data that moves a little between submits
-        rc = ensure_back(x, B->plans[g], B->caps[g]);
-        if (rc) return rc;
-        const LaunchPlan& P = B->plans[g];
-        if (resident_workgroups(x, P.prog.raster[P.raster_variant], 64 * (int)B->cfg.waves_per_tile) <= 0) return CPV_ERROR_DEVICE;
-    }
-    if (B->b.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    // record: every kernel of every launch, no allocation, no host sync
-    x.frozen = true;
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-    rc = CPV_OK;
-    fill_u32(c->stream, B->b.stats.p, 0u, kStatWords);
-    for (size_t g = 0; g < B->plans.size() && rc == CPV_OK; g++) {
-        rc = enqueue_front(x, B->plans[g], arena);
-        if (rc == CPV_OK) rc = enqueue_back(x, B->plans[g], arena, B->caps[g]);
-    }
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    const int rc = body();
     hipGraph_t graph = nullptr;
-    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    const hipError_t e = hipStreamEndCapture(s, &graph);
     if (rc) {
         if (graph) (void)hipGraphDestroy(graph);
         return rc;
     }
     if (e != hipSuccess) return fail(CPV_ERROR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    B->graph = graph;
-    HIP_TRY(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
+    const hipError_t ie = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(CPV_ERROR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    return CPV_OK;
+}
+
+static int batch_build(cpv_context* c, cpv_batch* B, const cpv_job_desc* jobs, uint32_t job_count)
+{
+    for (hipEvent_t* e : {&B->e0, &B->e1}) HIP_TRY(hipEventCreate(e));
+    HIP_TRY(hipEventCreateWithFlags(&B->pre, hipEventDisableTiming));
+    if (!c->front_stream) HIP_TRY(hipStreamCreateWithFlags(&c->front_stream, hipStreamNonBlocking));
+    std::vector<JobGroup> groups;
+    int rc = group_jobs(jobs, job_count, (int)B->cfg.tile_size, &groups);
+    if (rc) return rc;
+    const size_t G = groups.size();
+    B->plans.resize(G);
+    Staging st;
+    for (size_t g = 0; g < G; g++) {
+        rc = plan_launch(B->cfg, groups[g].prog, groups[g].targets, groups[g].lds, groups[g].passes, groups[g].dt, &B->plans[g]);
+        if (rc) return rc;
+        stage_plan(B->plans[g], st);
+    }
+    if (B->arena.ensure(std::max<size_t>(st.bytes.size(), 256))) return CPV_ERROR_OUT_OF_MEMORY;
+    uint8_t* arena = B->arena.as<uint8_t>();
+    for (LaunchPlan& P : B->plans) patch_plan(P, st, arena);
+    rc = upload_staging(c, B->arena, st);
+    if (rc) return rc;
+    // sizing run: the front half of every launch, to learn its pair count
+    B->caps.resize(G);
+    for (cpv_batch::Set& S : B->set) {
+        S.b.resize(G);
+        HIP_TRY(hipEventCreateWithFlags(&S.front_done, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&S.raster_done, hipEventDisableTiming));
+    }
+    for (size_t g = 0; g < G; g++) {
+        Exec x = context_exec(c, &B->set[0].b[g]);
+        x.cfg = B->cfg;
+        rc = enqueue_front(x, B->plans[g], arena);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(c->readback, x.b->stats.as<uint32_t>() + kStatPairTotal, 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        uint64_t pairs = 0;
+        std::memcpy(&pairs, c->readback, 8);
+        if (pairs > (uint64_t)UINT32_MAX)
+            return fail(CPV_ERROR_OVERFLOW, "the batch's launch %zu needs %llu (tile, primitive) pairs, more than 2^32 - 1", g,
+                        (unsigned long long)pairs);
+        // room for vertex data that moves a little between submits
+        B->caps[g] = (uint32_t)std::min<uint64_t>(pairs + pairs / 8 + 4096, UINT32_MAX);
+        const LaunchPlan& P = B->plans[g];
+        for (cpv_batch::Set& S : B->set) {
+            Exec y = context_exec(c, &S.b[g]);
+            y.cfg = B->cfg;
+            rc = ensure_front(y, P);
+            if (rc == CPV_OK) rc = ensure_back(y, P, B->caps[g]);
+            if (rc) return rc;
+            if (S.b[g].stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+            fill_u32(c->stream, S.b[g].stats.p, 0u, kStatAlloc);
+        }
+        if (resident_workgroups(x, P.prog.raster[P.raster_variant], 64 * (int)B->cfg.waves_per_tile) <= 0) return CPV_ERROR_DEVICE;
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // record: no allocation, no host sync, no memset node
+    for (cpv_batch::Set& S : B->set) {
+        std::vector<Binned> bins(G);
+        rc = capture(c->front_stream, &S.front, [&]() {
+            for (size_t g = 0; g < G; g++) {
+                Exec x = context_exec(c, &S.b[g]);
+                x.cfg = B->cfg;
+                x.s = c->front_stream;
+                x.frozen = true;
+                // the fragment counter restarts; the overflow word is cleared
+                // only when cpv_batch_wait has reported it
+                fill_u32(x.s, S.b[g].stats.p, 0u, 2);
+                int r = enqueue_front(x, B->plans[g], arena);
+                if (r == CPV_OK) r = enqueue_bin(x, B->plans[g], arena, B->caps[g], &bins[g]);
+                if (r) return r;
+            }
+            return (int)CPV_OK;
+        });
+        if (rc) return rc;
+        rc = capture(c->stream, &S.raster, [&]() {
+            for (size_t g = 0; g < G; g++) {
+                Exec x = context_exec(c, &S.b[g]);
+                x.cfg = B->cfg;
+                x.frozen = true;
+                const int r = enqueue_raster(x, B->plans[g], arena, bins[g]);
+                if (r) return r;
+            }
+            return (int)CPV_OK;
+        });
+        if (rc) return rc;
+    }
     return CPV_OK;
 }
 
@@ -807,6 +997,7 @@ extern "C" int cpv_batch_create(cpv_context* c, const cpv_job_desc* jobs, uint32
     if (job_count == 0) return fail(CPV_ERROR_INVALID, "a batch needs at least one job");
     std::lock_guard<std::mutex> guard(c->lock);
     HIP_TRY(hipSetDevice(c->device));
+    if (int rc = resolve_pending(c)) return rc;
     cpv_batch* B = new cpv_batch();
     B->c = c;
     B->cfg = call_config(c->cfg, jobs_have_points_or_lines(jobs, job_count));
@@ -816,6 +1007,7 @@ extern "C" int cpv_batch_create(cpv_context* c, const cpv_job_desc* jobs, uint32
         cpv_batch_destroy(B);
         return fail(rc, "%s", msg.c_str());
     }
+    note_enqueue(c);
     *out = B;
     return CPV_OK;
 }
@@ -823,12 +1015,31 @@ extern "C" int cpv_batch_create(cpv_context* c, const cpv_job_desc* jobs, uint32
 extern "C" int cpv_batch_submit(cpv_batch* B)
 {
     if (!B) return fail(CPV_ERROR_INVALID, "null batch");
-    std::lock_guard<std::mutex> guard(B->c->lock);
-    HIP_TRY(hipSetDevice(B->c->device));
-    HIP_TRY(hipEventRecord(B->e0, B->c->stream));
-    HIP_TRY(hipGraphLaunch(B->exec, B->c->stream));
-    HIP_TRY(hipEventRecord(B->e1, B->c->stream));
+    cpv_context* c = B->c;
+    std::lock_guard<std::mutex> guard(c->lock);
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = resolve_pending(c)) return rc;
+    cpv_batch::Set& S = B->set[B->submits & 1u];
+    const bool chained = B->submits > 0 && B->last_seq == c->enqueue_seq;
+    if (!chained) {
+        // other work was enqueued since this batch's last submit: the front
+        // half waits for all of it
+        HIP_TRY(hipEventRecord(B->pre, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->front_stream, B->pre, 0));
+    }
+    if (S.used) HIP_TRY(hipStreamWaitEvent(c->front_stream, S.raster_done, 0));   // the set's buffers are free again
+    HIP_TRY(hipEventRecord(B->e0, c->front_stream));
+    HIP_TRY(hipGraphLaunch(S.front, c->front_stream));
+    HIP_TRY(hipEventRecord(S.front_done, c->front_stream));
+    HIP_TRY(hipStreamWaitEvent(c->stream, S.front_done, 0));
+    HIP_TRY(hipGraphLaunch(S.raster, c->stream));
+    HIP_TRY(hipEventRecord(S.raster_done, c->stream));
+    HIP_TRY(hipEventRecord(B->e1, c->stream));
+    S.used = true;
+    B->submits++;
     B->submitted = true;
+    note_enqueue(c);
+    B->last_seq = c->enqueue_seq;
     return CPV_OK;
 }
 
@@ -838,13 +1049,31 @@ extern "C" int cpv_batch_wait(cpv_batch* B, uint64_t* fragments)
     if (!B->submitted) return fail(CPV_ERROR_INVALID, "batch not submitted");
     HIP_TRY(hipSetDevice(B->c->device));
     HIP_TRY(hipEventSynchronize(B->e1));
-    uint32_t words[kStatWords];
-    HIP_TRY(hipMemcpy(words, B->b.stats.p, sizeof(words), hipMemcpyDeviceToHost));
-    if (words[kStatOverflow])
+    bool overflow = false;
+    uint64_t frags = 0;
+    const cpv_batch::Set& last = B->set[(B->submits - 1) & 1u];
+    for (cpv_batch::Set& S : B->set) {
+        if (!S.used) continue;
+        for (cpvrt::Buffers& b : S.b) {
+            uint32_t words[kStatWords];
+            HIP_TRY(hipMemcpy(words, b.stats.p, sizeof(words), hipMemcpyDeviceToHost));
+            if (words[kStatOverflow]) {
+                overflow = true;
+                const uint32_t zero = 0;
+                HIP_TRY(hipMemcpy(b.stats.as<uint32_t>() + kStatOverflow, &zero, 4, hipMemcpyHostToDevice));
+            }
+            if (&S == &last) {
+                uint64_t f;
+                std::memcpy(&f, words + kStatFragments, 8);
+                frags += f;
+            }
+        }
+    }
+    if (overflow)
         return fail(CPV_ERROR_OVERFLOW, "the batch's (tile, primitive) pairs exceeded the capacity recorded for it; record it again");
     if (fragments) {
         if (!B->cfg.count_fragments) return fail(CPV_ERROR_INVALID, "the batch was recorded with count_fragments off");
-        std::memcpy(fragments, words + kStatFragments, 8);
+        *fragments = frags;
     }
     return CPV_OK;
 }

@@ -499,7 +499,7 @@ extern "C" void cpv_mlp_destroy(cpv_mlp* M)
 {
     if (!M) return;
     (void)hipSetDevice(M->c->device);
-    (void)hipStreamSynchronize(M->c->stream);
+    (void)cpvrt::sync_context(M->c);
     if (M->params) (void)hipFree(M->params);
     if (M->grads) (void)hipFree(M->grads);
     if (M->m) (void)hipFree(M->m);
@@ -512,7 +512,7 @@ extern "C" int cpv_mlp_get_params(cpv_mlp* M, float* params, uint32_t count)
     if (!M || !params) return fail(CPV_ERROR_INVALID, "null argument");
     if (count != kParams) return fail(CPV_ERROR_INVALID, "expected %d parameters", kParams);
     HIP_TRY(hipSetDevice(M->c->device));
-    HIP_TRY(hipStreamSynchronize(M->c->stream));
+    HIP_TRY(cpvrt::sync_context(M->c));
     HIP_TRY(hipMemcpy(params, M->params, kParams * 4, hipMemcpyDeviceToHost));
     return CPV_OK;
 }
@@ -522,7 +522,7 @@ extern "C" int cpv_mlp_set_params(cpv_mlp* M, const float* params, uint32_t coun
     if (!M || !params) return fail(CPV_ERROR_INVALID, "null argument");
     if (count != kParams) return fail(CPV_ERROR_INVALID, "expected %d parameters", kParams);
     HIP_TRY(hipSetDevice(M->c->device));
-    HIP_TRY(hipStreamSynchronize(M->c->stream));
+    HIP_TRY(cpvrt::sync_context(M->c));
     HIP_TRY(hipMemcpy(M->params, params, kParams * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(M->m, 0, kParams * 4));
     HIP_TRY(hipMemset(M->v, 0, kParams * 4));
@@ -536,6 +536,7 @@ extern "C" int cpv_mlp_predict(cpv_mlp* M, const float* x, uint32_t rows, float*
     if (rows == 0) return CPV_OK;
     std::lock_guard<std::mutex> guard(M->lock);
     HIP_TRY(hipSetDevice(M->c->device));
+    HIP_TRY(cpvrt::sync_context(M->c));   // no in-flight batch reads the weights while they change
     hipStream_t s = M->c->stream;
     DeviceRows d;
     int rc = upload_rows(x, nullptr, rows, &d, s);
@@ -556,6 +557,7 @@ extern "C" int cpv_mlp_gradients(cpv_mlp* M, const float* x, const float* target
     if (!M || !x || !target || !grads || rows == 0) return fail(CPV_ERROR_INVALID, "null argument or no rows");
     std::lock_guard<std::mutex> guard(M->lock);
     HIP_TRY(hipSetDevice(M->c->device));
+    HIP_TRY(cpvrt::sync_context(M->c));   // no in-flight batch reads the weights while they change
     hipStream_t s = M->c->stream;
     DeviceRows d;
     int rc = upload_rows(x, target, rows, &d, s);
@@ -576,6 +578,7 @@ extern "C" int cpv_mlp_train(cpv_mlp* M, const float* x, const float* target, ui
     if (!M || !x || !target || rows == 0 || batch == 0) return fail(CPV_ERROR_INVALID, "null argument, no rows or empty batch");
     std::lock_guard<std::mutex> guard(M->lock);
     HIP_TRY(hipSetDevice(M->c->device));
+    HIP_TRY(cpvrt::sync_context(M->c));   // no in-flight batch reads the weights while they change
     hipStream_t s = M->c->stream;
     DeviceRows d;
     int rc = upload_rows(x, target, rows, &d, s);

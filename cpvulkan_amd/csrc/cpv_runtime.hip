@@ -163,15 +163,24 @@ void Buffers::release_all()
     for (DevBuf* b : bufs) b->release();
 }
 
-int cpvrt::ensure_pinned(cpv_context* c, size_t bytes)
+int cpvrt::stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes)
 {
-    if (bytes <= c->pinned_cap) return CPV_OK;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->pinned) (void)hipHostFree(c->pinned);
-    c->pinned = nullptr;
-    c->pinned_cap = 0;
-    HIP_TRY(hipHostMalloc(&c->pinned, bytes * 2, hipHostMallocDefault));
-    c->pinned_cap = bytes * 2;
+    if (!bytes) return CPV_OK;
+    cpv_context::Staged& slot = c->staged[c->staged_next++ % 4u];
+    if (!slot.done) HIP_TRY(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming));
+    if (slot.used) HIP_TRY(hipEventSynchronize(slot.done));   // its last copy has read it
+    if (bytes > slot.cap) {
+        if (slot.p) (void)hipHostFree(slot.p);
+        slot.p = nullptr;
+        slot.cap = 0;
+        const size_t want = std::max<size_t>(bytes * 2, 64 << 10);
+        HIP_TRY(hipHostMalloc(&slot.p, want, hipHostMallocDefault));
+        slot.cap = want;
+    }
+    std::memcpy(slot.p, src, bytes);
+    HIP_TRY(hipMemcpyAsync(dst, slot.p, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipEventRecord(slot.done, c->stream));
+    slot.used = true;
     return CPV_OK;
 }
 
@@ -192,6 +201,7 @@ extern "C" int cpv_context_create(int device_ordinal, cpv_context** out)
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
         hipEventCreate(&c->ev_raster0) != hipSuccess || hipEventCreate(&c->ev_raster1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_check, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&c->readback, 256, hipHostMallocDefault) != hipSuccess) {
         delete c;
         return fail(CPV_ERROR_DEVICE, "stream/event/pinned allocation failed");
@@ -206,14 +216,24 @@ extern "C" void cpv_context_destroy(cpv_context* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    destroy_pending(c->pending);
+    c->pending = nullptr;
     c->buf.release_all();
-    if (c->pinned) (void)hipHostFree(c->pinned);
+    for (cpv_context::Staged& slot : c->staged) {
+        if (slot.p) (void)hipHostFree(slot.p);
+        if (slot.done) (void)hipEventDestroy(slot.done);
+    }
     if (c->readback) (void)hipHostFree(c->readback);
     (void)hipEventDestroy(c->ev_begin);
     (void)hipEventDestroy(c->ev_end);
     (void)hipEventDestroy(c->ev_raster0);
     (void)hipEventDestroy(c->ev_raster1);
+    (void)hipEventDestroy(c->ev_check);
     (void)hipStreamDestroy(c->stream);
+    if (c->front_stream) {
+        (void)hipStreamSynchronize(c->front_stream);
+        (void)hipStreamDestroy(c->front_stream);
+    }
     delete c;
 }
 
@@ -262,6 +282,8 @@ extern "C" int cpv_upload(cpv_context* c, uint64_t dst, const void* src, uint64_
 {
     if (!c || (!src && bytes)) return fail(CPV_ERROR_INVALID, "null argument");
     HIP_TRY(hipSetDevice(c->device));
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipMemcpyAsync((void*)(uintptr_t)dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CPV_OK;
@@ -271,6 +293,8 @@ extern "C" int cpv_download(cpv_context* c, void* dst, uint64_t src, uint64_t by
 {
     if (!c || (!dst && bytes)) return fail(CPV_ERROR_INVALID, "null argument");
     HIP_TRY(hipSetDevice(c->device));
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipMemcpyAsync(dst, (const void*)(uintptr_t)src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return CPV_OK;
@@ -286,6 +310,8 @@ extern "C" int cpv_device_copy(cpv_context* c, uint64_t dst, uint64_t src, uint6
     if (!c) return fail(CPV_ERROR_INVALID, "null context");
     if ((dst | src | bytes) & 15u) return fail(CPV_ERROR_INVALID, "device copy needs 16-byte aligned addresses and size");
     HIP_TRY(hipSetDevice(c->device));
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     const uint64_t n = bytes / 16;
     if (n) {
         hipLaunchKernelGGL(copy_u32x4_kernel, dim3((uint32_t)std::min<uint64_t>((n + 255) / 256, 65536)), dim3(256), 0, c->stream,
@@ -299,7 +325,8 @@ extern "C" int cpv_synchronize(cpv_context* c)
 {
     if (!c) return fail(CPV_ERROR_INVALID, "null context");
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (int rc = resolve_pending(c)) return rc;
+    HIP_TRY(sync_context(c));
     return CPV_OK;
 }
 

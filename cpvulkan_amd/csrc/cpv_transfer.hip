@@ -226,6 +226,8 @@ extern "C" int cpv_copy_regions(cpv_context* c, const cpv_copy_region* r, uint32
 {
     if (!c || (!r && count)) return fail(CPV_ERROR_INVALID, "null argument");
     std::lock_guard<std::mutex> g(c->lock);
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     for (uint32_t i = 0; i < count; i++) {
         if (!r[i].rows || !r[i].row_bytes) continue;
@@ -246,6 +248,8 @@ extern "C" int cpv_fill_buffer(cpv_context* c, uint64_t dst, uint64_t size, uint
 {
     if (!c || (!dst && size)) return fail(CPV_ERROR_INVALID, "null argument");
     std::lock_guard<std::mutex> g(c->lock);
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     if (size / 4 && (dst & 3u) == 0) {
         uint32_t* p = (uint32_t*)(uintptr_t)dst;
@@ -267,14 +271,11 @@ extern "C" int cpv_update_buffer(cpv_context* c, uint64_t dst, const void* data,
 {
     if (!c || ((!dst || !data) && size)) return fail(CPV_ERROR_INVALID, "null argument");
     std::lock_guard<std::mutex> g(c->lock);
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipSetDevice(c->device));
-    // through the pinned staging buffer, so the caller's memory is free on return
-    int rc = ensure_pinned(c, size);
-    if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(c->stream));   // a previous launch may still read the staging buffer
-    std::memcpy(c->pinned, data, size);
-    HIP_TRY(hipMemcpyAsync((void*)(uintptr_t)dst, c->pinned, size, hipMemcpyHostToDevice, c->stream));
-    return CPV_OK;
+    // through the pinned staging ring, so the caller's memory is free on return
+    return stage_upload(c, (void*)(uintptr_t)dst, data, size);
 }
 
 extern "C" int cpv_clear_rects(cpv_context* c, const cpv_image_level* image, const uint32_t value[4], const cpv_rect* rects,
@@ -284,6 +285,8 @@ extern "C" int cpv_clear_rects(cpv_context* c, const cpv_image_level* image, con
     FormatDesc f;
     if (!format_desc(image->format, &f)) return fail(CPV_ERROR_UNSUPPORTED, "clear of an unsupported format");
     std::lock_guard<std::mutex> g(c->lock);
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     ClearArgs a{};
     a.base = (uint8_t*)(uintptr_t)image->address;
@@ -317,6 +320,8 @@ extern "C" int cpv_blit_image(cpv_context* c, const cpv_image_level* src, const 
     const bool integer = sf.base == kBaseUInt || sf.base == kBaseSInt;
     if ((sds || integer) && filter != 0) return fail(CPV_ERROR_INVALID, "depth/stencil and integer blits use VK_FILTER_NEAREST");
     std::lock_guard<std::mutex> g(c->lock);
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     BlitArgs a;
     std::memset(&a, 0, sizeof(a));

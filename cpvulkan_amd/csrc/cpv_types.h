@@ -75,13 +75,18 @@ struct TriSetup {
     float area;               // 72: |EdgeFunction(p0, p1, p2)|
     uint32_t draw;            // 76: draw index within the launch
     float z[3];               // 80-91: p_k.z / p_k.w
-    float w[3];               // 92-103: clip-space w of p_k
-    uint32_t v[3];            // 104-115: vertex-output indices of p0, p1, p2 (after the CW swap)
-    uint32_t flags;           // bit 0: front facing
-    uint32_t provoking;       // vertex-output index of the provoking vertex
+    uint32_t flags;           // 92: bit 0 front facing, the primitive kind
+    // Bytes 0-95 are everything a primitive's visit of a pixel block reads
+    // (kVisitParts 16-byte parts); the rest only the colour work reads.
+    float w[3];               // 96-107: clip-space w of p_k
+    uint32_t v[3];            // 108-119: vertex-output indices of p0, p1, p2 (after the CW swap)
+    uint32_t provoking;       // 120: vertex-output index of the provoking vertex
     uint32_t pad;
 };
 static_assert(sizeof(TriSetup) == 128, "TriSetup must be 128 bytes");
+constexpr int kVisitParts = 6;
+static_assert(__builtin_offsetof(TriSetup, flags) == 92 && __builtin_offsetof(TriSetup, w) == 16 * kVisitParts,
+              "the visit fields fill the first kVisitParts parts");
 
 // TriSetup.flags: bit 0 front facing; the primitive kind.  A point record
 // holds its screen centre (int bits of ax[0], ay[0]), its size (dx[0]), its

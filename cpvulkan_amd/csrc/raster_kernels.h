@@ -72,6 +72,7 @@ struct RasterArgs {
     uint32_t* queue;                // kRasterQueues work counters, kRasterQueueStride words apart
     unsigned long long* fragments;  // covered (primitive, pixel) pairs
     float* trace_cycles;            // shader cycles per (tile, wave slot) (nullptr: not recorded)
+    const uint32_t* overflow;       // set when the launch's pairs outgrew their arrays: then nothing is drawn
     uint32_t vout_stride;
     uint32_t tiles;                 // total tiles over all targets
     uint32_t target_count;

@@ -94,13 +94,6 @@ constexpr int raster_waves_per_eu()
     return (TILE / 8) * (TILE / 8) / WAVES == 1 ? RasterWavesPerEU<FS>::value : 1;
 }
 
-// The first kStageParts 16-byte parts of a record (bytes 0-95: edges, box,
-// depth bounds, area, draw, z) are what a primitive's visit reads; the
-// staged copy carries the record's flags in place of w[0] (bytes 92-95).
-constexpr int kStageParts = 6;
-constexpr int kStageFlagsWord = 23;
-static_assert(__builtin_offsetof(TriSetup, z) + sizeof(float) * 3 <= 4 * kStageFlagsWord, "visit fields must fit the staged parts");
-static_assert(__builtin_offsetof(TriSetup, w) == 4 * kStageFlagsWord, "the staged flags replace w[0]");
 
 // Orders one wave's LDS writes before its lanes read each other's words.
 __device__ __forceinline__ void wave_lds_fence()
@@ -126,16 +119,27 @@ raster_kernel(RasterArgs a)
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const uint32_t home = blockIdx.x % kRasterQueues;
+    // Pairs that outgrew their arrays were dropped: the launch draws nothing
+    // and leaves the attachments as they were, so that it can be run again
+    // with larger arrays.
+    if (*a.overflow) return;
 
-    // With several blocks per lane, the records of a chunk's surviving
-    // primitives are fetched by all their lanes at once and staged in LDS
-    // (one region per wave), so the chunk pays one memory latency instead of
-    // one per primitive (measured on MI355X: 32x32 tiles 25% faster, 16x16
-    // tiles with four waves 12% slower, so those keep the scalar loads).  A
-    // shader that may discard reads more of the record per fragment and
-    // keeps the scalar loads too.
-    constexpr bool STAGE = !FS::kMayDiscard && BPW > 1;
-    __shared__ uint4 stage[STAGE ? WAVES : 1][STAGE ? kStageParts : 1][64];
+    // With several blocks per lane, a chunk's candidate primitives have the
+    // visit fields of their records (the first kVisitParts parts) loaded by
+    // all their lanes at once straight into the wave's LDS region
+    // (global_load_lds_dwordx4, which takes no registers), before the
+    // chunk's depth test, and the survivor loop reads them from there: the
+    // chunk pays one memory latency for the records of all its survivors.
+    // With one block per lane each survivor's record is read with scalar
+    // loads (measured on MI355X: the LDS staging made the 16x16 / four-wave
+    // soup frame's raster 9% slower, 0.85 -> 0.93 ms, and the 16x16 / one-wave
+    // and 32x32 / four-wave ones 2-4% faster).  A shader that may discard
+    // reads the whole record per fragment and keeps the scalar loads too.
+    // Every configuration loads the next chunk's pairs while it works on the
+    // current one.
+    constexpr bool GATHER = !FS::kMayDiscard && BPW > 1;
+    constexpr bool STAGE = GATHER;
+    __shared__ uint4 stage[STAGE ? WAVES : 1][STAGE ? kVisitParts : 1][64];
     uint32_t tried = 0;   // queues found empty, starting at home
 
     for (;;) {
@@ -272,6 +276,7 @@ raster_kernel(RasterArgs a)
 
         // ---- walk the tile's primitives in submission order
         const uint32_t lbeg = a.tile_start[tile], lend = a.tile_start[tile + 1];
+        uint32_t e_next = lbeg + (uint32_t)lane < lend ? a.tile_prims[lbeg + lane] : 0u;
         for (uint32_t base = lbeg; base < lend; base += 64) {
             const int n = (int)min(64u, lend - base);
             // Depth range stored in this wave's pixels, for hierarchical
@@ -294,10 +299,10 @@ raster_kernel(RasterArgs a)
                 }
             }
             // each lane tests one primitive of the chunk against the wave's area
+            const uint32_t e = e_next;
             uint32_t gp_l = 0, blocks_l = 0;
             bool ov = false;
             if (lane < n) {
-                const uint32_t e = a.tile_prims[base + lane];
                 gp_l = e & kPairPrim;
                 blocks_l = e >> kPairBlockShift;
                 if (TILE == 8) {
@@ -320,23 +325,43 @@ raster_kernel(RasterArgs a)
                     }
                     ov = (blocks_l & mine) != 0u;
                 }
-                if (ov && a.hiz_op >= 0) {
-                    const float2 dz = *(const float2*)&a.recs[gp_l].dt_lo;
-                    ov = !hiz_rejects(a.hiz_op, dz.x, dz.y, bmin, bmax);
+            }
+            // the candidates' visit fields, all lanes at once, straight into
+            // the wave's LDS region (global_load_lds: no registers)
+            float2 dz = make_float2(0.0f, 0.0f);
+            if constexpr (GATHER) {
+                wave_lds_fence();   // the previous chunk's reads come first
+                if (ov) {
+                    const uint4* src = (const uint4*)&a.recs[gp_l];
+#pragma unroll
+                    for (int p = 0; p < kVisitParts; p++)
+                        __builtin_amdgcn_global_load_lds((const void*)(src + p),
+                                                         (__attribute__((address_space(3))) void*)&stage[wave][p][0], 16, 0, 0);
+                    dz = *(const float2*)&a.recs[gp_l].dt_lo;
+                }
+            }
+            // the next chunk's pairs, loaded while this chunk is worked on
+            {
+                const uint32_t nb = base + 64u + (uint32_t)lane;
+                e_next = nb < lend ? a.tile_prims[nb] : 0u;
+            }
+            if (a.hiz_op >= 0) {
+                if (ov) {
+                    float lo, hi;
+                    if constexpr (GATHER) {
+                        lo = dz.x;
+                        hi = dz.y;
+                    } else {
+                        const float2 dz = *(const float2*)&a.recs[gp_l].dt_lo;
+                        lo = dz.x;
+                        hi = dz.y;
+                    }
+                    ov = !hiz_rejects(a.hiz_op, lo, hi, bmin, bmax);
                 }
             }
             uint64_t mask = __ballot(ov);
             if constexpr (STAGE) {
-                wave_lds_fence();   // the previous chunk's reads come first
-                if (ov) {
-                    const uint4* src = (const uint4*)&a.recs[gp_l];
-                    uint4 part[kStageParts];
-#pragma unroll
-                    for (int p = 0; p < kStageParts; p++) part[p] = src[p];
-                    part[kStageParts - 1].w = a.recs[gp_l].flags;
-#pragma unroll
-                    for (int p = 0; p < kStageParts; p++) stage[wave][p][lane] = part[p];
-                }
+                __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the LDS loads have landed
                 wave_lds_fence();
             }
             while (mask) {
@@ -348,7 +373,7 @@ raster_kernel(RasterArgs a)
                 if constexpr (STAGE) {
                     uint32_t* dst = (uint32_t*)&r;
 #pragma unroll
-                    for (int p = 0; p < kStageParts; p++) {
+                    for (int p = 0; p < kVisitParts; p++) {
                         // uniform words: kept in scalar registers
                         const uint4 v = stage[wave][p][j];
                         dst[4 * p + 0] = __builtin_amdgcn_readfirstlane(v.x);
@@ -356,7 +381,6 @@ raster_kernel(RasterArgs a)
                         dst[4 * p + 2] = __builtin_amdgcn_readfirstlane(v.z);
                         dst[4 * p + 3] = __builtin_amdgcn_readfirstlane(v.w);
                     }
-                    r.flags = dst[kStageFlagsWord];
                 } else {
                     load_record_uniform(a.recs, gprim, r);
                 }

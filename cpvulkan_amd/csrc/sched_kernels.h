@@ -1,8 +1,8 @@
 // Tile scheduling kernels: the per-tile binning statistics that the cost
 // predictor reads (and that cost traces record next to the measured cycles),
-// and the sort keys that turn predicted costs into a longest-first queue
-// order for the persistent raster kernel.  None of them changes what is
-// drawn: they only decide in which order tiles are taken.
+// the threshold from which a tile counts as heavy, and the sort keys that
+// turn predicted costs into the raster kernel's dispatch order.  None of
+// them changes what is drawn: they only decide when tiles are taken.
 #pragma once
 
 #include "cpv_types.h"
@@ -113,17 +113,55 @@ __global__ __launch_bounds__(256) void tile_features_kernel(FeatureArgs a)
     }
 }
 
-// Queue order keys: the stable radix sort of (key, tile) by a 16-bit key
-// that falls as the predicted log2 cost rises puts the most expensive tiles
-// first and keeps index order among equal predictions.
-__global__ __launch_bounds__(256) void order_keys_kernel(const float* pred, uint32_t tiles, uint32_t* keys, uint32_t* vals)
+// The predicted cost from which a tile counts as heavy: within a factor of
+// 4 (2 in log2 cost) of the costliest prediction.  The heavy tiles are the
+// frame's critical path: they are dispatched first.  max_bits is zeroed
+// beforehand; the maximum of the finite predictions is taken with an atomic
+// max on their ordered integer images.
+__device__ __forceinline__ uint32_t ordered_bits(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered_bits(uint32_t o)
+{
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+__global__ __launch_bounds__(256) void pred_max_kernel(const float* pred, uint32_t tiles, uint32_t* max_bits)
+{
+    uint32_t m = 0u;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < tiles; i += gridDim.x * 256u) {
+        const float p = pred[i];
+        if (__builtin_isfinite(p)) m = max(m, ordered_bits(p));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(max_bits, m);
+}
+
+__global__ void heavy_threshold_kernel(const uint32_t* max_bits, float* out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = *max_bits ? from_ordered_bits(*max_bits) - 2.0f : __builtin_inff();
+}
+
+// Dispatch order keys: the stable radix sort of (key, tile) by a 16-bit key
+// puts the heavy tiles first, costliest first, and keeps every other tile
+// in index order, so that neighbouring tiles, which share their primitives'
+// records, still run together (ordering all tiles by predicted cost costs
+// more in record re-reads than it saves in tail).
+__global__ __launch_bounds__(256) void order_keys_kernel(const float* pred, const float* heavy_from, uint32_t tiles, uint32_t* keys,
+                                                         uint32_t* vals)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= tiles) return;
-    float p = pred[i];
-    if (!(p > 0.0f)) p = 0.0f;   // NaN and negative predictions sort last
-    const float q = fminf(p, 63.99f) * 1024.0f;
-    keys[i] = 65535u - (uint32_t)q;
+    const float p = pred[i];
+    uint32_t k = 65535u;
+    if (p >= *heavy_from) {
+        const float q = fminf(fmaxf(p, 0.0f), 63.98f) * 1024.0f;   // NaN fails the test above
+        k = 65534u - (uint32_t)q;
+    }
+    keys[i] = k;
     vals[i] = i;
 }
 

@@ -108,7 +108,8 @@ enum {
 enum {
     CPV_SCHED_GRID = 0,       /* one workgroup per tile, hardware dispatch order ("plain" path) */
     CPV_SCHED_COUNTER = 1,    /* persistent kernel, tiles taken in index order from an atomic counter */
-    CPV_SCHED_PREDICTED = 2   /* persistent kernel, tiles ordered by the learned cost predictor */
+    CPV_SCHED_PREDICTED = 2   /* hardware-dispatched tiles, the ones the learned cost predictor finds heaviest
+                                 first (costliest first), the others in index order */
 };
 
 typedef struct cpv_context cpv_context;
@@ -343,12 +344,18 @@ int cpv_get_tile_traces(cpv_context* ctx, cpv_tile_trace* traces, uint32_t capac
  * Recording plans every launch, uploads the descriptors, runs the vertex
  * and setup stages once to size the (tile, primitive) pair arrays, and
  * captures every kernel of the frame -- all jobs, one launch per shader
- * program -- into one hipGraph; it uses the context's configuration at
+ * program -- into hipGraphs; it uses the context's configuration at
  * recording time (tile size, waves per tile, scheduler, predictor).  A
- * submit replays the graph on the context's stream without host work.
- * Vertex and index data may change between submits; if a replay needs more
- * pairs than were recorded, the overflow is detected and cpv_batch_wait
- * reports CPV_ERROR_OVERFLOW (record the batch again). */
+ * submit replays the graphs on the context's streams without host work.
+ * Consecutive submits of one batch with nothing else enqueued on the context
+ * in between are pipelined: a submit's vertex, setup and binning stages
+ * (which touch no attachment) run while the previous submit's raster kernel
+ * does, on two alternating sets of working buffers; the results are those
+ * of running the submits one after the other.
+ * Vertex and index data may change between submits (not while a submit is in
+ * flight); if a replay needs more pairs than were recorded, the overflow is
+ * detected and cpv_batch_wait reports CPV_ERROR_OVERFLOW (record the batch
+ * again). */
 typedef struct cpv_batch cpv_batch;
 int cpv_batch_create(cpv_context* ctx, const cpv_job_desc* jobs, uint32_t job_count, cpv_batch** out);
 int cpv_batch_submit(cpv_batch* batch);

@@ -156,3 +156,76 @@ def test_autotune(ctx):
         compare_jobs(scenes, ms, refs, "autotuned batch")
     finally:
         mem.release()
+
+
+def _oracle_passes(scene, passes, between=None):
+    """The oracle's attachments after `passes` renders of a scene in a row
+    (loadOp LOAD: each pass draws on top of the previous one's output);
+    between(host_materialized, k) runs after pass k."""
+    from cpvulkan_amd.workloads import HostMemory, materialize
+    hm = materialize(scene, HostMemory())
+    for k in range(passes):
+        oracle_lib.render(hm, threads=oracle_lib.default_threads())
+        if between:
+            between(hm, k)
+    return hm.read_color(), hm.read_ds(), None
+
+
+@pytest.mark.parametrize("sched", ["GRID", "COUNTER", "PREDICTED"])
+def test_pipelined_submits_bit_exact(ctx, sched):
+    """Back-to-back submits of one batch overlap a submit's vertex, setup
+    and binning with the previous submit's raster kernel; with loadOp LOAD
+    and blending every frame draws on the previous one, so the attachments
+    after four submits equal four oracle passes in a row only if each
+    raster kernel ran after the previous one and on its own frame's bins."""
+    import scenes as S
+    scene = S.load_op(W.skewed_scene(60_000, 640, 360, seed=17), seed=3)
+    want = _oracle_passes(scene, 4)
+    mlp = None
+    if sched == "PREDICTED":
+        mlp = cpvulkan_amd.Mlp(ctx, seed=5)
+        ctx.set_predictor(mlp)
+    _cfg(ctx, scheduler=SCHED[sched], count_fragments=0)
+    mem, ms, jobs = materialize_jobs(ctx, [scene])
+    try:
+        with ctx.batch(jobs) as b:
+            ms[0].reset_attachments()
+            for _ in range(4):
+                b.submit()
+            b.wait()
+            print(f"[pipelined] {sched}: 4 submits, last {b.timing():.3f} ms", flush=True)
+            compare(scene, (ms[0].read_color(), ms[0].read_ds(), None), want, f"pipelined {sched}")
+    finally:
+        mem.release()
+        ctx.set_predictor(None)
+        if mlp:
+            mlp.close()
+
+
+def test_pipelined_submits_order_with_other_work(ctx):
+    """Work enqueued on the context between two submits (a clear of part of
+    the colour attachment) is ordered between them: the second submit's
+    front half waits for it, its raster kernel draws on the cleared pixels."""
+    import scenes as S
+    scene = S.load_op(W.soup_scene(20_000, 320, 200, seed=19), seed=4)
+    value = [int(np.float32(v).view(np.uint32)) for v in (0.25, 0.5, 0.75, 1.0)]
+    rect = [abi.Rect(13, 17, 150, 90)]
+
+    def clear(hm, k):
+        if k == 0:
+            img = hm.mem.keep[hm.color_addr]
+            oracle_lib.clear_rects(img[: scene.color_bytes], scene.color_format, scene.width, scene.height, value, rect)
+
+    want = _oracle_passes(scene, 2, clear)
+    _cfg(ctx, scheduler=SCHED["GRID"], count_fragments=0)
+    mem, ms, jobs = materialize_jobs(ctx, [scene])
+    try:
+        with ctx.batch(jobs) as b:
+            ms[0].reset_attachments()
+            b.submit()
+            ctx.clear_rects(abi.ImageLevel(ms[0].color_addr, scene.color_format, scene.width, scene.height, 0), value, rect)
+            b.submit()
+            b.wait()
+            compare(scene, (ms[0].read_color(), ms[0].read_ds(), None), want, "submit, clear, submit")
+    finally:
+        mem.release()
