@@ -46,6 +46,7 @@ def _declare(lib: C.CDLL) -> None:
     lib.cpv_get_config.argtypes = [P, C.POINTER(abi.Config)]
     lib.cpv_render_pass.argtypes = [P, C.POINTER(abi.PassDesc), C.POINTER(abi.PipelineDesc), C.POINTER(abi.DrawDesc),
                                     C.POINTER(C.c_uint32), C.c_uint32]
+    lib.cpv_render_pass_noalloc.argtypes = lib.cpv_render_pass.argtypes
     lib.cpv_render_jobs.argtypes = [P, C.POINTER(abi.JobDesc), C.c_uint32]
     lib.cpv_last_fragment_count.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.cpv_last_timing.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -193,6 +194,16 @@ class Context:
         dp = (C.c_uint32 * max(n, 1))(*(draw_pipeline or [0] * n))
         rc = self.lib.cpv_render_pass(self.handle, C.byref(pass_desc), parr, darr, dp, n)
         _check(self.lib, rc, "cpv_render_pass")
+
+    def render_pass_noalloc(self, pass_desc, pipelines, draws, draw_pipeline=None) -> None:
+        """cpv_render_pass_noalloc: render_pass with every working buffer in
+        the GPU's fixed scratch array (no allocation)."""
+        n = len(draws)
+        parr = (abi.PipelineDesc * max(len(pipelines), 1))(*pipelines)
+        darr = (abi.DrawDesc * max(n, 1))(*draws)
+        dp = (C.c_uint32 * max(n, 1))(*(draw_pipeline or [0] * n))
+        rc = self.lib.cpv_render_pass_noalloc(self.handle, C.byref(pass_desc), parr, darr, dp, n)
+        _check(self.lib, rc, "cpv_render_pass_noalloc")
 
     def render_jobs(self, jobs) -> None:
         arr = (abi.JobDesc * max(len(jobs), 1))(*jobs)

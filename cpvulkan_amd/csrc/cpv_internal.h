@@ -76,7 +76,12 @@ struct MlpDevice;   // the tile-cost predictor's device state (cpv_mlp.hip)
 // one if its pairs did not fit (its raster kernel then drew nothing).
 struct PendingLaunch;
 void destroy_pending(PendingLaunch* p);
-int resolve_pending(cpv_context* c);
+int resolve_pending(cpv_context* c, bool may_allocate = true);
+
+// Takes the device address of this GPU's scratch array (the working memory
+// of cpv_render_pass_noalloc) into c->scratch; once per device.
+int scratch_setup(cpv_context* c);
+size_t scratch_bytes();
 
 }  // namespace cpvrt
 
@@ -117,6 +122,8 @@ struct cpv_context {
     cpvrt::PendingLaunch* pending = nullptr;   // the last immediate launch, until its pair count is checked
     hipEvent_t ev_check = nullptr;             // recorded after the pending launch's pair-count readback
     uint64_t pair_cap = 0;                     // pair capacity of immediate launches (grows as launches need)
+    uint8_t* scratch = nullptr;                // device address of this GPU's scratch array (cpvrt::scratch_setup)
+    uint32_t* last_stats = nullptr;            // control words of the last render call (fragment count)
     std::mutex lock;
 };
 
@@ -140,7 +147,9 @@ uint32_t prim_count_of(uint32_t topology, uint32_t n);
 void fill_u32(hipStream_t s, void* p, uint32_t v, uint64_t n);
 // Copies bytes from host memory to device address dst on the context's
 // stream through the next pinned staging slot; src is free on return.
-int stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes);
+// Without may_grow a slot too small for the bytes is an error, not an
+// allocation.
+int stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes, bool may_grow = true);
 // Counts a call that enqueued work on the context's stream: a recorded
 // batch overlaps a submit with its previous one only when nothing else was
 // enqueued in between.
@@ -155,6 +164,7 @@ inline hipError_t sync_context(cpv_context* c)
 // Tile-cost predictor (cpv_mlp.hip): enqueue inference of `tiles` feature
 // rows (16 floats each) into per-tile predicted log2 costs.
 int mlp_enqueue_predict(cpv_mlp* m, const float* features, uint32_t tiles, float* out, hipStream_t s);
+constexpr size_t kStagingSlotBytes = 256u << 10;   // initial size of each pinned staging slot
 constexpr int kFeatures = 12;       // features per tile the predictor reads
 constexpr int kFeatureStride = 16;  // floats per feature row (padded to the MFMA depth)
 

@@ -533,6 +533,8 @@ struct cpvrt::PendingLaunch {
     LaunchPlan P;
     Staging st;
     cpv_config cfg{};
+    bool in_scratch = false;     // ran in the scratch array (cpv_render_pass_noalloc)
+    uint32_t scratch_cap = 0;    // its pair capacity there
 };
 
 void cpvrt::destroy_pending(PendingLaunch* p) { delete p; }
@@ -569,10 +571,124 @@ int enqueue_checked(cpv_context* c, PendingLaunch& L)
 
 }  // namespace
 
+// ------------------------------------------------ allocation-free launches
+//
+// cpv_render_pass_noalloc runs a render pass with every working buffer in
+// one file-scope device array per GPU: the control words, the descriptors,
+// the vertex outputs, the triangle records, the scans, the tile ranges and
+// the four pair arrays.  On the host the array's name is no device address;
+// host code uses only the address hipGetSymbolAddress returns for it with
+// the device current (cpv_context::scratch, taken when the context is
+// created).  Every call on a GPU shares that GPU's array: calls that overlap
+// on one GPU, from two contexts or two threads, race.
+
+constexpr size_t kScratchBytes = 64ull << 20;
+
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+int cpvrt::scratch_setup(cpv_context* c)
+{
+    static std::mutex lock;
+    static std::map<int, uint8_t*> by_device;
+    std::lock_guard<std::mutex> g(lock);
+    auto it = by_device.find(c->device);
+    if (it == by_device.end()) {
+        HIP_TRY(hipSetDevice(c->device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(CPV_ERROR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        it = by_device.emplace(c->device, (uint8_t*)p).first;
+    }
+    c->scratch = it->second;
+    return CPV_OK;
+}
+
+size_t cpvrt::scratch_bytes() { return kScratchBytes; }
+
+namespace {
+
+// Places every buffer of launch P in the scratch array at base, with the
+// sizes ensure_front and ensure_back ask for: the control words first (at
+// the same address for every launch), then the descriptors, vertex outputs,
+// records, scans and tile ranges; the four pair arrays share what is left
+// and *pair_cap receives their capacity.  Touches no memory.
+int carve_scratch(uint8_t* base, const LaunchPlan& P, size_t arena_bytes, Buffers* b, uint32_t* pair_cap)
+{
+    size_t at = 0;
+    auto take = [&](DevBuf& d, size_t bytes) {
+        at = (at + 255) & ~(size_t)255;
+        d.p = base + at;
+        d.cap = bytes;
+        at += bytes;
+    };
+    const size_t prims = std::max(P.prim_total, 1u), tiles = std::max(P.tiles, 1u);
+    take(b->stats, kStatAlloc * 4);
+    take(b->arena, std::max<size_t>(arena_bytes, 256));
+    take(b->vout, (size_t)std::max(P.vout_total, 1u) * P.stride * 4);
+    take(b->recs, prims * sizeof(TriSetup));
+    take(b->counts, prims * 4);
+    take(b->offsets, ((size_t)P.prim_total + 1) * 4);
+    take(b->sums, scan_sums_bytes(P.prim_total));
+    if (P.lines) take(b->line_ext, prims * sizeof(LineExt));
+    take(b->tile_start, (tiles + 1) * 4);
+    take(b->hist, (size_t)256 * kSortSegments * 4);
+    take(b->hist_base, ((size_t)256 * kSortSegments + 1) * 4);
+    take(b->big, ((size_t)P.prim_total + 1) * 4);
+    // each pair array starts on a 256-byte boundary: at most 4 x 255 bytes of padding
+    const size_t left = at + 1024 < kScratchBytes ? kScratchBytes - at - 1024 : 0;
+    const uint64_t cap = std::min<uint64_t>(left / 16, UINT32_MAX);
+    if (cap == 0)
+        return fail(CPV_ERROR_UNSUPPORTED,
+                    "a launch of %u primitives over %u tiles needs more than the %zu MiB of the allocation-free entry point",
+                    P.prim_total, P.tiles, kScratchBytes >> 20);
+    for (DevBuf* d : {&b->keys[0], &b->keys[1], &b->vals[0], &b->vals[1]}) take(*d, (size_t)cap * 4);
+    *pair_cap = (uint32_t)cap;
+    return CPV_OK;
+}
+
+// Enqueues launch L with every buffer in the scratch array: its descriptors
+// through a staging slot that holds them without growing, then the front
+// half, the binning with the pair capacity left in the array, and the raster
+// kernel, then the read-back of its pair total and overflow word, which the
+// next call on the context checks.  Allocates nothing.
+int enqueue_scratch(cpv_context* c, PendingLaunch& L)
+{
+    Buffers b;
+    uint32_t cap = 0;
+    int rc = carve_scratch(c->scratch, L.P, L.st.bytes.size(), &b, &cap);
+    if (rc) return rc;
+    uint8_t* arena = b.arena.as<uint8_t>();
+    patch_plan(L.P, L.st, arena);
+    rc = stage_upload(c, arena, L.st.bytes.data(), L.st.bytes.size(), false);
+    if (rc) return rc;
+    Exec x = context_exec(c, &b);
+    x.cfg = L.cfg;
+    x.cfg.record_costs = 0;
+    x.predictor = nullptr;   // the learned order needs the predictor's buffers: the counter queue runs instead
+    x.frozen = true;         // a buffer that does not fit is an error, never an allocation
+    rc = enqueue_front(x, L.P, arena);
+    if (rc) return rc;
+    x.r0 = c->ev_raster0;
+    x.r1 = c->ev_raster1;
+    rc = enqueue_back(x, L.P, arena, cap);
+    if (rc) return rc;
+    L.in_scratch = true;
+    L.scratch_cap = cap;
+    uint32_t* stats = b.stats.as<uint32_t>();
+    HIP_TRY(hipMemcpyAsync(c->readback + 32, stats + kStatPairTotal, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->readback + 34, stats + kStatOverflow, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(c->ev_check, c->stream));
+    return CPV_OK;
+}
+
+}  // namespace
+
 // Checks the pending launch's pair count; a launch whose pairs did not fit
 // drew nothing (its raster kernel saw the overflow word) and is run again
 // here with arrays of the exact size, before any later work is enqueued.
-int cpvrt::resolve_pending(cpv_context* c)
+// Without may_allocate the launch runs again in the scratch array, if its
+// pairs fit there; a launch that already ran there cannot grow.
+int cpvrt::resolve_pending(cpv_context* c, bool may_allocate)
 {
     if (!c->pending) return CPV_OK;
     std::unique_ptr<PendingLaunch> L(c->pending);
@@ -587,12 +703,35 @@ int cpvrt::resolve_pending(cpv_context* c)
                     "a render call's primitives need %llu (tile, primitive) pairs, more than 2^32 - 1: split its draws "
                     "or use larger tiles (nothing was drawn)",
                     (unsigned long long)total);
-    c->pair_cap = std::min<uint64_t>(total + total / 8 + 4096, UINT32_MAX);
-    fill_u32(c->stream, c->buf.stats.as<uint32_t>() + kStatOverflow, 0u, 1);
-    int rc = enqueue_checked(c, *L);
+    if (L->in_scratch)
+        return fail(CPV_ERROR_OVERFLOW,
+                    "a render call's primitives need %llu (tile, primitive) pairs; the allocation-free entry point had room "
+                    "for %u (nothing was drawn from that launch on)",
+                    (unsigned long long)total, L->scratch_cap);
+    int rc;
+    if (may_allocate) {
+        c->pair_cap = std::min<uint64_t>(total + total / 8 + 4096, UINT32_MAX);
+        fill_u32(c->stream, c->buf.stats.as<uint32_t>() + kStatOverflow, 0u, 1);
+        rc = enqueue_checked(c, *L);
+    } else {
+        Buffers probe;
+        uint32_t cap = 0;
+        rc = carve_scratch(c->scratch, L->P, L->st.bytes.size(), &probe, &cap);
+        if (rc == CPV_OK && cap < total)
+            rc = fail(CPV_ERROR_OVERFLOW,
+                      "the previous render call needs %llu (tile, primitive) pairs, more than the allocation-free entry "
+                      "point holds (%u): call cpv_synchronize after it (nothing was drawn)",
+                      (unsigned long long)total, cap);
+        if (rc == CPV_OK) {
+            fill_u32(c->stream, c->scratch, 0u, kStatWords);
+            rc = enqueue_scratch(c, *L);
+        }
+        c->last_traces = false;
+        c->last_stats = (uint32_t*)c->scratch;
+    }
     if (rc) return rc;
     c->pending = L.release();
-    return resolve_pending(c);   // the exact capacity holds every pair: the check passes
+    return resolve_pending(c, may_allocate);   // the exact capacity holds every pair: the check passes
 }
 
 namespace {
@@ -696,11 +835,73 @@ int group_jobs(const cpv_job_desc* jobs, uint32_t job_count, int tile, std::vect
 
 // ------------------------------------------------------------ render calls
 
-extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
-                               const cpv_draw_desc* draws, const uint32_t* draw_pipeline, uint32_t draw_count)
+namespace {
+
+// The launches of one render pass: one per run of consecutive draws that
+// share a shader program (the first applies the pass's load ops, the rest
+// load what it stored); a pass with no draw is one empty launch, which still
+// applies the load ops to every tile.
+struct PassLaunch {
+    Program prog;
+    std::vector<LaunchDraw> lds;
+    std::vector<uint32_t> dt;
+    bool first = true;
+};
+
+int split_pass(const cpv_pipeline_desc* pipelines, const cpv_draw_desc* draws, const uint32_t* draw_pipeline, uint32_t draw_count,
+               std::vector<PassLaunch>* out)
+{
+    if (draw_count == 0) {
+        PassLaunch L;
+        if (!builtin_program(CPV_VS_PASSTHROUGH, CPV_FS_COLOR, &L.prog)) return fail(CPV_ERROR_DEVICE, "built-in table");
+        out->push_back(std::move(L));
+        return CPV_OK;
+    }
+    uint32_t i = 0;
+    while (i < draw_count) {
+        PassLaunch L;
+        L.first = out->empty();
+        const uint32_t pi0 = draw_pipeline ? draw_pipeline[i] : 0;
+        int rc = resolve_program(pipelines[pi0], &L.prog);
+        if (rc) return rc;
+        uint32_t j = i;
+        while (j < draw_count) {
+            const uint32_t pj = draw_pipeline ? draw_pipeline[j] : 0;
+            if (!same_program(pipelines[pj], pipelines[pi0])) break;
+            for (uint32_t inst = 0; inst < draws[j].instance_count; inst++) {
+                L.lds.push_back({&pipelines[pj], &draws[j], inst});
+                L.dt.push_back(0);
+            }
+            j++;
+        }
+        i = j;
+        out->push_back(std::move(L));
+    }
+    return CPV_OK;
+}
+
+int check_pass_args(cpv_context* c, const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines, const cpv_draw_desc* draws,
+                    uint32_t draw_count)
 {
     if (!c || !pass || (draw_count && (!pipelines || !draws))) return fail(CPV_ERROR_INVALID, "null argument");
     if (pass->color_count > CPV_MAX_COLOR_ATTACHMENTS) return fail(CPV_ERROR_UNSUPPORTED, "%u colour attachments", pass->color_count);
+    return CPV_OK;
+}
+
+bool pass_has_points_or_lines(const cpv_pipeline_desc* pipelines, const uint32_t* draw_pipeline, uint32_t draw_count)
+{
+    bool any = false;
+    for (uint32_t k = 0; k < draw_count; k++)
+        any = any || pipelines[draw_pipeline ? draw_pipeline[k] : 0].topology < CPV_TOPOLOGY_TRIANGLE_LIST;
+    return any;
+}
+
+}  // namespace
+
+extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                               const cpv_draw_desc* draws, const uint32_t* draw_pipeline, uint32_t draw_count)
+{
+    if (int rc = check_pass_args(c, pass, pipelines, draws, draw_count)) return rc;
     std::lock_guard<std::mutex> guard(c->lock);
     note_enqueue(c);
     if (int rc = resolve_pending(c)) return rc;
@@ -708,48 +909,85 @@ extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const 
     if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
     fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
     HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
-    bool any_prim = false;
-    for (uint32_t k = 0; k < draw_count; k++)
-        any_prim = any_prim || pipelines[draw_pipeline ? draw_pipeline[k] : 0].topology < CPV_TOPOLOGY_TRIANGLE_LIST;
-    const cpv_config cfg = call_config(c->cfg, any_prim);
-    const int tile = (int)cfg.tile_size;
-    std::vector<cpv_pass_desc> passes(1, *pass);
-    bool first = true;
-    uint32_t i = 0;
-    do {
-        // a run of draws sharing one shader program
-        std::vector<LaunchDraw> lds;
-        std::vector<uint32_t> dt;
-        Program prog;
-        if (draw_count == 0) {
-            // clear-only pass: an empty launch still initialises every tile
-            if (!builtin_program(CPV_VS_PASSTHROUGH, CPV_FS_COLOR, &prog)) return fail(CPV_ERROR_DEVICE, "built-in table");
-        } else {
-            const uint32_t pi0 = draw_pipeline ? draw_pipeline[i] : 0;
-            int rc = resolve_program(pipelines[pi0], &prog);
-            if (rc) return rc;
-            uint32_t j = i;
-            while (j < draw_count) {
-                const uint32_t pj = draw_pipeline ? draw_pipeline[j] : 0;
-                if (!same_program(pipelines[pj], pipelines[pi0])) break;
-                for (uint32_t inst = 0; inst < draws[j].instance_count; inst++) {
-                    lds.push_back({&pipelines[pj], &draws[j], inst});
-                    dt.push_back(0);
-                }
-                j++;
-            }
-            i = j;
-        }
+    const cpv_config cfg = call_config(c->cfg, pass_has_points_or_lines(pipelines, draw_pipeline, draw_count));
+    const std::vector<cpv_pass_desc> passes(1, *pass);
+    std::vector<PassLaunch> launches;
+    int rc = split_pass(pipelines, draws, draw_pipeline, draw_count, &launches);
+    if (rc) return rc;
+    for (const PassLaunch& L : launches) {
         std::vector<TargetParams> targets(1);
-        int rc = build_target(*pass, first, tile, 0, &targets[0]);
+        rc = build_target(*pass, L.first, (int)cfg.tile_size, 0, &targets[0]);
         if (rc) return rc;
-        rc = run_launch(c, cfg, prog, targets, lds, passes, dt);
+        rc = run_launch(c, cfg, L.prog, targets, L.lds, passes, L.dt);
         if (rc) return rc;
-        first = false;
-    } while (i < draw_count);
+    }
     HIP_TRY(hipEventRecord(c->ev_end, c->stream));
     c->timing_valid = true;
     c->last_count_exact = c->cfg.count_fragments != 0;
+    c->last_stats = c->buf.stats.as<uint32_t>();
+    return CPV_OK;
+}
+
+// cpv_render_pass with every working buffer in this GPU's scratch array:
+// every launch of the pass is planned and checked against the array and the
+// staging slot first, so a pass that does not fit is refused before any of
+// it is enqueued; then the launches run one after the other in the array.
+// No hipMalloc, hipFree, hipHostMalloc or other allocating call is made.
+extern "C" int cpv_render_pass_noalloc(cpv_context* c, const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
+                                       const cpv_draw_desc* draws, const uint32_t* draw_pipeline, uint32_t draw_count)
+{
+    if (int rc = check_pass_args(c, pass, pipelines, draws, draw_count)) return rc;
+    std::lock_guard<std::mutex> guard(c->lock);
+    if (!c->scratch) {
+        if (int rc = scratch_setup(c)) return rc;   // reports why the context could not take the array's address
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    cpv_config cfg = call_config(c->cfg, pass_has_points_or_lines(pipelines, draw_pipeline, draw_count));
+    cfg.record_costs = 0;
+    const std::vector<cpv_pass_desc> passes(1, *pass);
+    std::vector<PassLaunch> split;
+    int rc = split_pass(pipelines, draws, draw_pipeline, draw_count, &split);
+    if (rc) return rc;
+    std::vector<std::unique_ptr<PendingLaunch>> launches;
+    for (const PassLaunch& S : split) {
+        std::vector<TargetParams> targets(1);
+        rc = build_target(*pass, S.first, (int)cfg.tile_size, 0, &targets[0]);
+        if (rc) return rc;
+        std::unique_ptr<PendingLaunch> L(new PendingLaunch());
+        L->cfg = cfg;
+        rc = plan_launch(cfg, S.prog, targets, S.lds, passes, S.dt, &L->P);
+        if (rc) return rc;
+        stage_plan(L->P, L->st);
+        if (L->st.bytes.size() > kStagingSlotBytes)
+            return fail(CPV_ERROR_UNSUPPORTED, "the pass's descriptors take %zu bytes; the allocation-free entry point stages at most %zu",
+                        L->st.bytes.size(), kStagingSlotBytes);
+        Buffers probe;
+        uint32_t cap = 0;
+        rc = carve_scratch(c->scratch, L->P, L->st.bytes.size(), &probe, &cap);
+        if (rc) return rc;
+        launches.push_back(std::move(L));
+    }
+    note_enqueue(c);
+    rc = resolve_pending(c, false);
+    if (rc) return rc;
+    fill_u32(c->stream, c->scratch, 0u, kStatWords);   // fragment counter, pair counts, overflow word
+    HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+    for (std::unique_ptr<PendingLaunch>& L : launches) {
+        rc = resolve_pending(c, false);   // the previous launch of the pass, before this one reuses the array
+        if (rc) return rc;
+        rc = enqueue_scratch(c, *L);
+        if (rc) return rc;
+        c->last_tiles = L->P.tiles;
+        c->last_waves = cfg.waves_per_tile;
+        c->last_tile_size = cfg.tile_size;
+        c->last_traces = false;
+        c->last_target_base.assign(1, 0u);
+        c->pending = L.release();
+    }
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    c->timing_valid = true;
+    c->last_count_exact = cfg.count_fragments != 0;
+    c->last_stats = (uint32_t*)c->scratch;
     return CPV_OK;
 }
 
@@ -774,6 +1012,7 @@ extern "C" int cpv_render_jobs(cpv_context* c, const cpv_job_desc* jobs, uint32_
     HIP_TRY(hipEventRecord(c->ev_end, c->stream));
     c->timing_valid = true;
     c->last_count_exact = c->cfg.count_fragments != 0;
+    c->last_stats = c->buf.stats.as<uint32_t>();
     return CPV_OK;
 }
 
@@ -783,7 +1022,7 @@ extern "C" int cpv_last_fragment_count(cpv_context* c, uint64_t* fragments)
     if (!c->last_count_exact) return fail(CPV_ERROR_INVALID, "the last render call ran with count_fragments off");
     HIP_TRY(hipSetDevice(c->device));
     if (int rc = resolve_pending(c)) return rc;
-    HIP_TRY(hipMemcpyAsync(c->readback + 16, c->buf.stats.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->readback + 16, c->last_stats, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     std::memcpy(fragments, c->readback + 16, 8);
     return CPV_OK;

@@ -163,12 +163,14 @@ void Buffers::release_all()
     for (DevBuf* b : bufs) b->release();
 }
 
-int cpvrt::stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes)
+int cpvrt::stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes, bool may_grow)
 {
     if (!bytes) return CPV_OK;
     cpv_context::Staged& slot = c->staged[c->staged_next++ % 4u];
     if (!slot.done) HIP_TRY(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming));
     if (slot.used) HIP_TRY(hipEventSynchronize(slot.done));   // its last copy has read it
+    if (bytes > slot.cap && !may_grow)
+        return fail(CPV_ERROR_UNSUPPORTED, "%zu bytes of descriptors exceed the %zu-byte staging slot", bytes, slot.cap);
     if (bytes > slot.cap) {
         if (slot.p) (void)hipHostFree(slot.p);
         slot.p = nullptr;
@@ -207,6 +209,19 @@ extern "C" int cpv_context_create(int device_ordinal, cpv_context** out)
         return fail(CPV_ERROR_DEVICE, "stream/event/pinned allocation failed");
     }
     if (c->buf.stats.ensure(kStatAlloc * 4)) { delete c; return CPV_ERROR_OUT_OF_MEMORY; }
+    c->last_stats = c->buf.stats.as<uint32_t>();
+    // the staging slots start large enough for the allocation-free entry point
+    for (cpv_context::Staged& slot : c->staged) {
+        if (hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc(&slot.p, kStagingSlotBytes, hipHostMallocDefault) != hipSuccess) {
+            cpv_context_destroy(c);
+            return fail(CPV_ERROR_DEVICE, "staging slot allocation failed");
+        }
+        slot.cap = kStagingSlotBytes;
+    }
+    // the scratch array's address on this device; a failure here is reported
+    // by cpv_render_pass_noalloc, the only user of the array
+    if (scratch_setup(c) != CPV_OK) c->scratch = nullptr;
     *out = c;
     return CPV_OK;
 }

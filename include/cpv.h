@@ -324,6 +324,27 @@ int cpv_render_pass(cpv_context* ctx, const cpv_pass_desc* pass,
                     const uint32_t* draw_pipeline, uint32_t draw_count);
 int cpv_render_jobs(cpv_context* ctx, const cpv_job_desc* jobs, uint32_t job_count);
 
+/* cpv_render_pass without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory during a frame
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and asynchrony as cpv_render_pass;
+ * every working buffer lives in one 64 MiB device array per GPU, set up
+ * when the context is created and kept until the process ends.  No
+ * hipMalloc, hipFree, hipHostMalloc or other allocating call is made.
+ * A pass whose buffers (vertex outputs, triangle records, scans, tile
+ * ranges) leave no room for pairs, or whose descriptors exceed 256 KiB, is
+ * refused with CPV_ERROR_UNSUPPORTED before anything is enqueued.  A launch
+ * whose (tile, primitive) pairs exceed the room left in the array draws
+ * nothing, nor do the launches of the pass after it, and the next call on
+ * the context (cpv_synchronize, for one) returns CPV_ERROR_OVERFLOW.
+ * The CPV_SCHED_PREDICTED scheduler runs as CPV_SCHED_COUNTER here, and no
+ * tile costs are recorded.  All calls on one GPU share its array, which
+ * keeps the last call's contents: calls that overlap on one GPU (two
+ * contexts, or two threads) race. */
+int cpv_render_pass_noalloc(cpv_context* ctx, const cpv_pass_desc* pass,
+                            const cpv_pipeline_desc* pipelines, const cpv_draw_desc* draws,
+                            const uint32_t* draw_pipeline, uint32_t draw_count);
+
 /* Number of (primitive, pixel) pairs that passed coverage and culling in the
  * last cpv_render_pass / cpv_render_jobs call (synchronises). */
 int cpv_last_fragment_count(cpv_context* ctx, uint64_t* fragments);
