@@ -58,6 +58,19 @@ def _declare(lib: C.CDLL) -> None:
     lib.cpv_abi_sizes.argtypes = [C.POINTER(C.c_uint32), C.c_uint32]
     lib.cpv_get_tile_traces.argtypes = [P, C.POINTER(abi.TileTrace), C.c_uint32, C.POINTER(C.c_uint32)]
     lib.cpv_batch_create.argtypes = [P, C.POINTER(abi.JobDesc), C.c_uint32, C.POINTER(P)]
+    U32P = C.POINTER(C.c_uint32)
+    lib.cpv_render_passes.argtypes = [P, C.POINTER(abi.PassDesc), C.c_uint32, C.POINTER(abi.PipelineDesc),
+                                      C.POINTER(abi.DrawDesc), U32P, U32P]
+    lib.cpv_batch_create_passes.argtypes = [P, C.POINTER(abi.PassDesc), C.c_uint32, C.POINTER(abi.PipelineDesc),
+                                            C.POINTER(abi.DrawDesc), U32P, U32P, C.POINTER(P)]
+    lib.cpv_launch_count.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.cpv_fence_create.argtypes = [P, C.c_int, C.POINTER(P)]
+    lib.cpv_fence_signal.argtypes = [P]
+    lib.cpv_fence_reset.argtypes = [P]
+    lib.cpv_fence_status.argtypes = [P]
+    lib.cpv_fence_wait.argtypes = [P, C.c_uint64]
+    lib.cpv_fence_destroy.argtypes = [P]
+    lib.cpv_fence_destroy.restype = None
     lib.cpv_batch_submit.argtypes = [P]
     lib.cpv_batch_wait.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.cpv_batch_timing.argtypes = [P, C.POINTER(C.c_float)]
@@ -243,6 +256,25 @@ class Context:
     def batch(self, jobs) -> "Batch":
         return Batch(self, jobs)
 
+    def batch_passes(self, passes) -> "Batch":
+        """A recorded batch of render passes: passes is a list of
+        (pass_desc, pipelines, draws, draw_pipeline) as for render_pass."""
+        return Batch(self, None, passes=passes)
+
+    def render_passes(self, passes) -> None:
+        """cpv_render_passes over a list of (pass_desc, pipelines, draws,
+        draw_pipeline) tuples, each as render_pass takes them."""
+        args = _pass_arrays(passes)
+        _check(self.lib, self.lib.cpv_render_passes(self.handle, *args), "cpv_render_passes")
+
+    def launch_count(self) -> int:
+        v = C.c_uint64()
+        _check(self.lib, self.lib.cpv_launch_count(self.handle, C.byref(v)), "cpv_launch_count")
+        return v.value
+
+    def fence(self, signaled: bool = False) -> "Fence":
+        return Fence(self, signaled)
+
     def autotune(self, jobs, repeats: int = 3):
         """Sweeps the tile configurations over a recorded batch of jobs;
         returns ([(tile, waves, ms)], best abi.Config)."""
@@ -312,14 +344,76 @@ class Program:
             self.handle = C.c_void_p()
 
 
-class Batch:
-    """A recorded set of render jobs replayed as one hipGraph (cpv_batch_*)."""
+def _pass_arrays(passes):
+    """The flat arrays of cpv_render_passes: every pass's pipelines and draws
+    concatenated, draw_pipeline re-indexed, first_draw offsets."""
+    pds, pipes, draws, dp, first = [], [], [], [], [0]
+    for pass_desc, pipelines, pass_draws, draw_pipeline in passes:
+        base = len(pipes)
+        pds.append(pass_desc)
+        pipes.extend(pipelines)
+        draws.extend(pass_draws)
+        dp.extend(base + k for k in (draw_pipeline or [0] * len(pass_draws)))
+        first.append(len(draws))
+    return ((abi.PassDesc * max(len(pds), 1))(*pds), len(pds), (abi.PipelineDesc * max(len(pipes), 1))(*pipes),
+            (abi.DrawDesc * max(len(draws), 1))(*draws), (C.c_uint32 * max(len(dp), 1))(*dp),
+            (C.c_uint32 * len(first))(*first))
 
-    def __init__(self, ctx: Context, jobs):
+
+class Fence:
+    """cpv_fence_*: the end of the work enqueued on a context up to signal()."""
+
+    def __init__(self, ctx: Context, signaled: bool = False):
+        self.lib = ctx.lib
+        self.handle = C.c_void_p()
+        _check(self.lib, self.lib.cpv_fence_create(ctx.handle, 1 if signaled else 0, C.byref(self.handle)), "cpv_fence_create")
+
+    def signal(self) -> None:
+        _check(self.lib, self.lib.cpv_fence_signal(self.handle), "cpv_fence_signal")
+
+    def reset(self) -> None:
+        _check(self.lib, self.lib.cpv_fence_reset(self.handle), "cpv_fence_reset")
+
+    def status(self) -> int:
+        """abi.OK, or abi.NOT_READY."""
+        rc = self.lib.cpv_fence_status(self.handle)
+        if rc < 0:
+            _check(self.lib, rc, "cpv_fence_status")
+        return rc
+
+    def wait(self, timeout_ns: int = (1 << 64) - 1) -> int:
+        """abi.OK, or abi.TIMEOUT."""
+        rc = self.lib.cpv_fence_wait(self.handle, timeout_ns)
+        if rc < 0:
+            _check(self.lib, rc, "cpv_fence_wait")
+        return rc
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.cpv_fence_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """A recorded set of render jobs, or of render passes, replayed as
+    hipGraphs (cpv_batch_*)."""
+
+    def __init__(self, ctx: Context, jobs, passes=None):
         self.ctx = ctx
         self.lib = ctx.lib
-        self._jobs = (abi.JobDesc * max(len(jobs), 1))(*jobs)
         self.handle = C.c_void_p()
+        if passes is not None:
+            self._arrays = _pass_arrays(passes)
+            _check(self.lib, self.lib.cpv_batch_create_passes(ctx.handle, *self._arrays, C.byref(self.handle)),
+                   "cpv_batch_create_passes")
+            return
+        self._jobs = (abi.JobDesc * max(len(jobs), 1))(*jobs)
         _check(self.lib, self.lib.cpv_batch_create(ctx.handle, self._jobs, len(jobs), C.byref(self.handle)), "cpv_batch_create")
 
     def submit(self) -> None:

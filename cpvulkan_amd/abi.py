@@ -37,6 +37,8 @@ ADDRESS = dict(REPEAT=0, MIRRORED_REPEAT=1, CLAMP_TO_EDGE=2, CLAMP_TO_BORDER=3, 
 VS = dict(PASSTHROUGH=1, MVP_COLOR=2, MVP_LAMBERT=3, PASSTHROUGH_POINT=4)
 FS = dict(COLOR=1, TEX_LAMBERT=2, COLOR_DISCARD=3, COLOR_FLAT=4, COLOR_NOPERSPECTIVE=5)
 SCHED = dict(GRID=0, COUNTER=1, PREDICTED=2)
+# non-error status codes (fences)
+OK, NOT_READY, TIMEOUT = 0, 1, 2
 COLOR_RGBA = 15
 
 FORMAT_BYTES = {9: 1, 37: 4, 38: 4, 41: 4, 42: 4, 44: 4, 91: 8, 100: 4, 103: 8, 106: 12, 109: 16,

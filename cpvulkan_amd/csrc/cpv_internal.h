@@ -77,6 +77,11 @@ struct MlpDevice;   // the tile-cost predictor's device state (cpv_mlp.hip)
 struct PendingLaunch;
 void destroy_pending(PendingLaunch* p);
 int resolve_pending(cpv_context* c, bool may_allocate = true);
+// Records a recorded batch's submit again after it outgrew its pair arrays
+// (cpv_launch.hip), and re-records the fences signalled after the redrawn
+// work.
+int batch_redo(cpv_context* c, cpv_batch* B, uint32_t set, bool may_allocate);
+void rerecord_fences(cpv_context* c);
 
 // Takes the device address of this GPU's scratch array (the working memory
 // of cpv_render_pass_noalloc) into c->scratch; once per device.
@@ -124,6 +129,10 @@ struct cpv_context {
     uint64_t pair_cap = 0;                     // pair capacity of immediate launches (grows as launches need)
     uint8_t* scratch = nullptr;                // device address of this GPU's scratch array (cpvrt::scratch_setup)
     uint32_t* last_stats = nullptr;            // control words of the last render call (fragment count)
+    uint64_t launches = 0;                     // raster launches enqueued: render calls and batch submits
+    cpv_batch* pending_batch = nullptr;        // the last batch submit, until its overflow word is checked
+    uint32_t pending_set = 0;                  // the set of working buffers that submit used
+    std::vector<cpv_fence*> fences_after_pending;   // fences signalled since the pending launch or submit
     std::mutex lock;
 };
 

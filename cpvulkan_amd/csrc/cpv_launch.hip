@@ -18,7 +18,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <memory>
+#include <thread>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -241,6 +243,7 @@ struct Exec {
     cpv_context* ctx = nullptr;   // CU count and the occupancy cache
     bool frozen = false;          // recording a graph: no allocation
     hipEvent_t r0 = nullptr, r1 = nullptr;
+    uint32_t* overflow = nullptr; // overflow word shared by several launches (nullptr: the launch's own)
 };
 
 // Exclusive scan of n counts into out (out[n] = the total when write_total);
@@ -396,7 +399,7 @@ int enqueue_bin(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap,
     const TargetParams* d_tp = (const TargetParams*)(arena + P.o_tp);
     const uint32_t cap = std::max(pair_cap, 1u);
     hipLaunchKernelGGL(clamp_pairs_kernel, dim3(1), dim3(64), 0, x.s, (const uint64_t*)(stats + kStatPairTotal), cap,
-                       stats + kStatPairs, stats + kStatOverflow);
+                       stats + kStatPairs, x.overflow ? x.overflow : stats + kStatOverflow);
     // emit pairs in primitive order
     if (P.prim_total) {
         fill_u32(x.s, b.big.p, 0u, 1);
@@ -476,7 +479,7 @@ int enqueue_raster(Exec& x, const LaunchPlan& P, uint8_t* arena, const Binned& b
     ra.order = order;
     ra.queue = stats + kStatQueues;
     ra.fragments = (unsigned long long*)(stats + kStatFragments);
-    ra.overflow = stats + kStatOverflow;
+    ra.overflow = x.overflow ? x.overflow : stats + kStatOverflow;
     ra.trace_cycles = x.cfg.record_costs ? b.traces.as<float>() : nullptr;
     ra.vout_stride = P.stride;
     ra.tiles = P.tiles;
@@ -690,26 +693,50 @@ int enqueue_scratch(cpv_context* c, PendingLaunch& L)
 // pairs fit there; a launch that already ran there cannot grow.
 int cpvrt::resolve_pending(cpv_context* c, bool may_allocate)
 {
-    if (!c->pending) return CPV_OK;
+    if (c->pending_batch) {
+        cpv_batch* B = c->pending_batch;
+        const uint32_t si = c->pending_set;
+        c->pending_batch = nullptr;
+        HIP_TRY(hipSetDevice(c->device));
+        HIP_TRY(hipEventSynchronize(c->ev_check));
+        if (!c->readback[36]) {
+            c->fences_after_pending.clear();
+            return CPV_OK;
+        }
+        const int rc = batch_redo(c, B, si, may_allocate);
+        if (rc) {
+            c->fences_after_pending.clear();
+            return rc;
+        }
+        rerecord_fences(c);
+        return resolve_pending(c, may_allocate);
+    }
+    if (!c->pending) {
+        c->fences_after_pending.clear();
+        return CPV_OK;
+    }
     std::unique_ptr<PendingLaunch> L(c->pending);
     c->pending = nullptr;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipEventSynchronize(c->ev_check));
     uint64_t total = 0;
     std::memcpy(&total, c->readback + 32, 8);
-    if (!c->readback[34]) return CPV_OK;
+    if (!c->readback[34]) {
+        c->fences_after_pending.clear();
+        return CPV_OK;
+    }
+    int rc = CPV_OK;
     if (total > (uint64_t)UINT32_MAX)
-        return fail(CPV_ERROR_OVERFLOW,
-                    "a render call's primitives need %llu (tile, primitive) pairs, more than 2^32 - 1: split its draws "
-                    "or use larger tiles (nothing was drawn)",
-                    (unsigned long long)total);
-    if (L->in_scratch)
-        return fail(CPV_ERROR_OVERFLOW,
-                    "a render call's primitives need %llu (tile, primitive) pairs; the allocation-free entry point had room "
-                    "for %u (nothing was drawn from that launch on)",
-                    (unsigned long long)total, L->scratch_cap);
-    int rc;
-    if (may_allocate) {
+        rc = fail(CPV_ERROR_OVERFLOW,
+                  "a render call's primitives need %llu (tile, primitive) pairs, more than 2^32 - 1: split its draws "
+                  "or use larger tiles (nothing was drawn)",
+                  (unsigned long long)total);
+    else if (L->in_scratch)
+        rc = fail(CPV_ERROR_OVERFLOW,
+                  "a render call's primitives need %llu (tile, primitive) pairs; the allocation-free entry point had room "
+                  "for %u (nothing was drawn from that launch on)",
+                  (unsigned long long)total, L->scratch_cap);
+    else if (may_allocate) {
         c->pair_cap = std::min<uint64_t>(total + total / 8 + 4096, UINT32_MAX);
         fill_u32(c->stream, c->buf.stats.as<uint32_t>() + kStatOverflow, 0u, 1);
         rc = enqueue_checked(c, *L);
@@ -729,7 +756,12 @@ int cpvrt::resolve_pending(cpv_context* c, bool may_allocate)
         c->last_traces = false;
         c->last_stats = (uint32_t*)c->scratch;
     }
-    if (rc) return rc;
+    if (rc) {
+        c->fences_after_pending.clear();
+        return rc;
+    }
+    c->launches++;
+    rerecord_fences(c);
     c->pending = L.release();
     return resolve_pending(c, may_allocate);   // the exact capacity holds every pair: the check passes
 }
@@ -761,6 +793,7 @@ int run_launch(cpv_context* c, const cpv_config& cfg, const Program& prog, const
     c->last_target_base.clear();
     for (const TargetParams& t : P.targets) c->last_target_base.push_back(t.tile_base);
     c->pending = L.release();
+    c->launches++;
     return CPV_OK;
 }
 
@@ -790,9 +823,9 @@ bool same_program(const cpv_pipeline_desc& a, const cpv_pipeline_desc& b)
     return a.vs_kind == b.vs_kind && a.fs_kind == b.fs_kind;
 }
 
-// Groups jobs by shader program: one launch per group, every job its own
-// render target (tile ranges laid end to end).
-struct JobGroup {
+// One launch of a frame: a shader program and the draws of one or more
+// independent render targets, whose tile ranges are laid end to end.
+struct LaunchSpec {
     Program prog;
     std::vector<TargetParams> targets;
     std::vector<cpv_pass_desc> passes;
@@ -800,12 +833,14 @@ struct JobGroup {
     std::vector<uint32_t> dt;
 };
 
-int group_jobs(const cpv_job_desc* jobs, uint32_t job_count, int tile, std::vector<JobGroup>* groups)
+// Groups jobs by shader program: one launch per group, every job its own
+// render target.
+int group_jobs(const cpv_job_desc* jobs, uint32_t job_count, int tile, std::vector<LaunchSpec>* groups)
 {
     std::vector<bool> done(job_count, false);
     for (uint32_t g = 0; g < job_count; g++) {
         if (done[g]) continue;
-        JobGroup G;
+        LaunchSpec G;
         int rc = resolve_program(jobs[g].pipeline, &G.prog);
         if (rc) return rc;
         uint32_t tile_base = 0;
@@ -896,6 +931,128 @@ bool pass_has_points_or_lines(const cpv_pipeline_desc* pipelines, const uint32_t
     return any;
 }
 
+// Byte ranges of device memory, for the hazards between render passes.
+struct Span {
+    uint64_t lo = 0, hi = 0;
+    bool meets(const Span& o) const { return lo < o.hi && o.lo < hi; }
+};
+
+void attachment_spans(const cpv_pass_desc& p, std::vector<Span>* out)
+{
+    auto add = [&](const cpv_attachment_desc& a) {
+        FormatDesc f;
+        if (!a.address || !format_desc(a.format, &f)) return;
+        out->push_back({a.address, a.address + (uint64_t)p.width * p.height * (uint64_t)f.total});
+    };
+    for (uint32_t k = 0; k < p.color_count && k < CPV_MAX_COLOR_ATTACHMENTS; k++) add(p.color[k]);
+    if (p.has_depth_stencil) add(p.depth_stencil);
+}
+
+void texture_spans(const cpv_draw_desc& d, std::vector<Span>* out)
+{
+    for (const cpv_texture_desc& t : d.textures) {
+        FormatDesc f;
+        if (!t.address || !format_desc(t.format, &f)) continue;
+        // every level of a mip chain fits in twice the base level
+        out->push_back({t.address, t.address + 2ull * t.width * t.height * (uint64_t)f.total});
+    }
+}
+
+bool spans_meet(const std::vector<Span>& a, const std::vector<Span>& b)
+{
+    for (const Span& x : a)
+        for (const Span& y : b)
+            if (x.meets(y)) return true;
+    return false;
+}
+
+// The launches of several render passes, each with its own draws
+// (draws first_draw[p] .. first_draw[p + 1] - 1 belong to pass p).  The
+// result is that of running the passes one after the other.  Consecutive
+// passes that neither write one another's attachments nor sample them form
+// a group; inside a group, the k-th launch of every pass (its k-th run of
+// draws that share a shader program, split_pass) is merged with those of
+// the other passes that use the same program into one launch with one
+// render target per pass.
+int plan_passes(const cpv_pass_desc* passes, uint32_t pass_count, const cpv_pipeline_desc* pipelines, const cpv_draw_desc* draws,
+                const uint32_t* draw_pipeline, const uint32_t* first_draw, int tile, std::vector<LaunchSpec>* out)
+{
+    uint32_t a = 0;
+    while (a < pass_count) {
+        // the group [a, b): each pass independent of the ones before it in the group
+        std::vector<Span> written, sampled;
+        uint32_t b = a;
+        for (; b < pass_count; b++) {
+            std::vector<Span> w, r;
+            attachment_spans(passes[b], &w);
+            for (uint32_t k = first_draw[b]; k < first_draw[b + 1]; k++) texture_spans(draws[k], &r);
+            if (b > a && (spans_meet(w, written) || spans_meet(w, sampled) || spans_meet(r, written))) break;
+            written.insert(written.end(), w.begin(), w.end());
+            sampled.insert(sampled.end(), r.begin(), r.end());
+        }
+        std::vector<std::vector<PassLaunch>> runs(b - a);
+        size_t steps = 0;
+        for (uint32_t p = a; p < b; p++) {
+            const uint32_t f = first_draw[p], n = first_draw[p + 1] - f;
+            int rc = split_pass(pipelines, draws + f, draw_pipeline ? draw_pipeline + f : nullptr, n, &runs[p - a]);
+            if (rc) return rc;
+            steps = std::max(steps, runs[p - a].size());
+        }
+        for (size_t k = 0; k < steps; k++) {
+            std::vector<bool> taken(b - a, false);
+            for (uint32_t p = a; p < b; p++) {
+                if (taken[p - a] || runs[p - a].size() <= k) continue;
+                // runs without draws (clear-only passes) join any launch
+                const PassLaunch* by = nullptr;   // the run whose program the launch runs
+                LaunchSpec G;
+                uint32_t tile_base = 0;
+                for (uint32_t q = p; q < b; q++) {
+                    if (taken[q - a] || runs[q - a].size() <= k) continue;
+                    const PassLaunch& R = runs[q - a][k];
+                    if (!R.lds.empty()) {
+                        if (by && !same_program(*R.lds[0].pipe, *by->lds[0].pipe)) continue;
+                        if (!by) by = &R;
+                    }
+                    taken[q - a] = true;
+                    TargetParams t;
+                    int rc = build_target(passes[q], R.first, tile, tile_base, &t);
+                    if (rc) return rc;
+                    if ((uint64_t)tile_base + (uint64_t)t.tiles_x * t.tiles_y >= (1ull << 31))
+                        return fail(CPV_ERROR_UNSUPPORTED, "more than 2^31 tiles in one launch");
+                    tile_base += t.tiles_x * t.tiles_y;
+                    const uint32_t ti = (uint32_t)G.targets.size();
+                    G.targets.push_back(t);
+                    G.passes.push_back(passes[q]);
+                    for (const LaunchDraw& d : R.lds) {
+                        G.lds.push_back(d);
+                        G.dt.push_back(ti);
+                    }
+                }
+                G.prog = by ? by->prog : runs[p - a][k].prog;
+                out->push_back(std::move(G));
+            }
+        }
+        a = b;
+    }
+    return CPV_OK;
+}
+
+int check_passes_args(cpv_context* c, const cpv_pass_desc* passes, uint32_t pass_count, const cpv_pipeline_desc* pipelines,
+                      const cpv_draw_desc* draws, const uint32_t* draw_pipeline, const uint32_t* first_draw)
+{
+    if (!c || (pass_count && (!passes || !first_draw))) return fail(CPV_ERROR_INVALID, "null argument");
+    if (pass_count && first_draw[0] != 0) return fail(CPV_ERROR_INVALID, "first_draw[0] is %u, not 0", first_draw[0]);
+    for (uint32_t p = 0; p < pass_count; p++) {
+        if (first_draw[p + 1] < first_draw[p]) return fail(CPV_ERROR_INVALID, "first_draw decreases at pass %u", p + 1);
+        if (passes[p].color_count > CPV_MAX_COLOR_ATTACHMENTS)
+            return fail(CPV_ERROR_UNSUPPORTED, "%u colour attachments", passes[p].color_count);
+    }
+    const uint32_t n = pass_count ? first_draw[pass_count] : 0;
+    if (n && (!pipelines || !draws)) return fail(CPV_ERROR_INVALID, "null argument");
+    (void)draw_pipeline;
+    return CPV_OK;
+}
+
 }  // namespace
 
 extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const cpv_pipeline_desc* pipelines,
@@ -983,6 +1140,7 @@ extern "C" int cpv_render_pass_noalloc(cpv_context* c, const cpv_pass_desc* pass
         c->last_traces = false;
         c->last_target_base.assign(1, 0u);
         c->pending = L.release();
+        c->launches++;
     }
     HIP_TRY(hipEventRecord(c->ev_end, c->stream));
     c->timing_valid = true;
@@ -1002,10 +1160,46 @@ extern "C" int cpv_render_jobs(cpv_context* c, const cpv_job_desc* jobs, uint32_
     fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
     HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
     const cpv_config cfg = call_config(c->cfg, jobs_have_points_or_lines(jobs, job_count));
-    std::vector<JobGroup> groups;
+    std::vector<LaunchSpec> groups;
     int rc = group_jobs(jobs, job_count, (int)cfg.tile_size, &groups);
     if (rc) return rc;
-    for (JobGroup& G : groups) {
+    for (LaunchSpec& G : groups) {
+        rc = run_launch(c, cfg, G.prog, G.targets, G.lds, G.passes, G.dt);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    c->timing_valid = true;
+    c->last_count_exact = c->cfg.count_fragments != 0;
+    c->last_stats = c->buf.stats.as<uint32_t>();
+    return CPV_OK;
+}
+
+namespace {
+
+bool passes_have_points_or_lines(const cpv_pipeline_desc* pipelines, const uint32_t* draw_pipeline, uint32_t draw_count)
+{
+    return draw_count && pass_has_points_or_lines(pipelines, draw_pipeline, draw_count);
+}
+
+}  // namespace
+
+extern "C" int cpv_render_passes(cpv_context* c, const cpv_pass_desc* passes, uint32_t pass_count, const cpv_pipeline_desc* pipelines,
+                                 const cpv_draw_desc* draws, const uint32_t* draw_pipeline, const uint32_t* first_draw)
+{
+    if (int rc = check_passes_args(c, passes, pass_count, pipelines, draws, draw_pipeline, first_draw)) return rc;
+    std::lock_guard<std::mutex> guard(c->lock);
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+    fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
+    HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+    const uint32_t n = pass_count ? first_draw[pass_count] : 0;
+    const cpv_config cfg = call_config(c->cfg, passes_have_points_or_lines(pipelines, draw_pipeline, n));
+    std::vector<LaunchSpec> specs;
+    int rc = plan_passes(passes, pass_count, pipelines, draws, draw_pipeline, first_draw, (int)cfg.tile_size, &specs);
+    if (rc) return rc;
+    for (LaunchSpec& G : specs) {
         rc = run_launch(c, cfg, G.prog, G.targets, G.lds, G.passes, G.dt);
         if (rc) return rc;
     }
@@ -1081,15 +1275,23 @@ extern "C" int cpv_get_tile_traces(cpv_context* c, cpv_tile_trace* traces, uint3
 // raster graph that last used the same set.  So when submits follow each
 // other with nothing else enqueued on the context in between, the front
 // half of one frame runs while the raster kernel of the previous frame
-// does: the recorded capacities need no host round trip, so nothing stops
-// the pipeline.  Any other work enqueued on the context in between makes
-// the next front graph wait for all of it.
+// does.  Any other work enqueued on the context in between makes the next
+// front graph wait for all of it.
+//
+// The launches of a set share one overflow word, cleared at the start of
+// its front graph and raised by any launch whose pairs exceed the capacity
+// recorded for it; every raster kernel of the set then draws nothing, so a
+// submit either draws all of its frame or none of it.  The word is read
+// back after the front graph, and the next call on the context checks it
+// (resolve_pending): a submit that drew nothing is recorded again with the
+// pair counts it needed and replayed before anything else is enqueued.
 struct cpv_batch {
     struct Set {
-        std::vector<cpvrt::Buffers> b;   // one per launch (shader program) of the frame
+        std::vector<cpvrt::Buffers> b;   // one per launch of the frame
         hipGraphExec_t front = nullptr, raster = nullptr;
         hipEvent_t front_done = nullptr, raster_done = nullptr;
         bool used = false;
+        uint32_t* overflow() const { return b[0].stats.as<uint32_t>() + kStatOverflow; }
     };
     cpv_context* c = nullptr;
     cpv_config cfg{};
@@ -1103,15 +1305,23 @@ struct cpv_batch {
     bool submitted = false;
 };
 
-extern "C" void cpv_batch_destroy(cpv_batch* B)
+static void release_graphs(cpv_batch* B)
 {
-    if (!B) return;
-    (void)hipSetDevice(B->c->device);
-    (void)hipStreamSynchronize(B->c->stream);
-    if (B->c->front_stream) (void)hipStreamSynchronize(B->c->front_stream);
     for (cpv_batch::Set& S : B->set) {
         if (S.front) (void)hipGraphExecDestroy(S.front);
         if (S.raster) (void)hipGraphExecDestroy(S.raster);
+        S.front = S.raster = nullptr;
+    }
+}
+
+// Frees a batch; the caller holds the context's lock and has resolved any
+// pending check of the batch.
+static void free_batch(cpv_batch* B)
+{
+    (void)hipSetDevice(B->c->device);
+    (void)sync_context(B->c);
+    release_graphs(B);
+    for (cpv_batch::Set& S : B->set) {
         if (S.front_done) (void)hipEventDestroy(S.front_done);
         if (S.raster_done) (void)hipEventDestroy(S.raster_done);
         for (cpvrt::Buffers& b : S.b) b.release_all();
@@ -1120,6 +1330,14 @@ extern "C" void cpv_batch_destroy(cpv_batch* B)
         if (e) (void)hipEventDestroy(e);
     B->arena.release();
     delete B;
+}
+
+extern "C" void cpv_batch_destroy(cpv_batch* B)
+{
+    if (!B) return;
+    std::lock_guard<std::mutex> guard(B->c->lock);
+    if (B->c->pending_batch == B) (void)resolve_pending(B->c);   // a submit that drew nothing is redrawn first
+    free_batch(B);
 }
 
 // Captures what `body` enqueues on stream s into an instantiated graph.
@@ -1141,26 +1359,77 @@ static int capture(hipStream_t s, hipGraphExec_t* out, F&& body)
     return CPV_OK;
 }
 
-static int batch_build(cpv_context* c, cpv_batch* B, const cpv_job_desc* jobs, uint32_t job_count)
+// Sizes both sets' buffers for the batch's capacities, clears their control
+// words, and records the four graphs: no allocation, no host sync and no
+// memset node inside a capture.
+static int batch_record(cpv_context* c, cpv_batch* B)
+{
+    const size_t G = B->plans.size();
+    uint8_t* arena = B->arena.as<uint8_t>();
+    for (cpv_batch::Set& S : B->set) {
+        for (size_t g = 0; g < G; g++) {
+            Exec y = context_exec(c, &S.b[g]);
+            y.cfg = B->cfg;
+            int rc = ensure_front(y, B->plans[g]);
+            if (rc == CPV_OK) rc = ensure_back(y, B->plans[g], B->caps[g]);
+            if (rc) return rc;
+            if (S.b[g].stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+            fill_u32(c->stream, S.b[g].stats.p, 0u, kStatAlloc);
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (cpv_batch::Set& S : B->set) {
+        std::vector<Binned> bins(G);
+        int rc = capture(c->front_stream, &S.front, [&]() {
+            fill_u32(c->front_stream, S.overflow(), 0u, 1);
+            for (size_t g = 0; g < G; g++) {
+                Exec x = context_exec(c, &S.b[g]);
+                x.cfg = B->cfg;
+                x.s = c->front_stream;
+                x.frozen = true;
+                x.overflow = S.overflow();
+                fill_u32(x.s, S.b[g].stats.p, 0u, 2);   // the fragment counter restarts
+                int r = enqueue_front(x, B->plans[g], arena);
+                if (r == CPV_OK) r = enqueue_bin(x, B->plans[g], arena, B->caps[g], &bins[g]);
+                if (r) return r;
+            }
+            return (int)CPV_OK;
+        });
+        if (rc) return rc;
+        rc = capture(c->stream, &S.raster, [&]() {
+            for (size_t g = 0; g < G; g++) {
+                Exec x = context_exec(c, &S.b[g]);
+                x.cfg = B->cfg;
+                x.frozen = true;
+                x.overflow = S.overflow();
+                const int r = enqueue_raster(x, B->plans[g], arena, bins[g]);
+                if (r) return r;
+            }
+            return (int)CPV_OK;
+        });
+        if (rc) return rc;
+    }
+    return CPV_OK;
+}
+
+static int batch_build(cpv_context* c, cpv_batch* B, std::vector<LaunchSpec>& specs)
 {
     for (hipEvent_t* e : {&B->e0, &B->e1}) HIP_TRY(hipEventCreate(e));
     HIP_TRY(hipEventCreateWithFlags(&B->pre, hipEventDisableTiming));
     if (!c->front_stream) HIP_TRY(hipStreamCreateWithFlags(&c->front_stream, hipStreamNonBlocking));
-    std::vector<JobGroup> groups;
-    int rc = group_jobs(jobs, job_count, (int)B->cfg.tile_size, &groups);
-    if (rc) return rc;
-    const size_t G = groups.size();
+    const size_t G = specs.size();
+    if (G == 0) return fail(CPV_ERROR_INVALID, "a batch needs at least one launch");
     B->plans.resize(G);
     Staging st;
     for (size_t g = 0; g < G; g++) {
-        rc = plan_launch(B->cfg, groups[g].prog, groups[g].targets, groups[g].lds, groups[g].passes, groups[g].dt, &B->plans[g]);
+        int rc = plan_launch(B->cfg, specs[g].prog, specs[g].targets, specs[g].lds, specs[g].passes, specs[g].dt, &B->plans[g]);
         if (rc) return rc;
         stage_plan(B->plans[g], st);
     }
     if (B->arena.ensure(std::max<size_t>(st.bytes.size(), 256))) return CPV_ERROR_OUT_OF_MEMORY;
     uint8_t* arena = B->arena.as<uint8_t>();
     for (LaunchPlan& P : B->plans) patch_plan(P, st, arena);
-    rc = upload_staging(c, B->arena, st);
+    int rc = upload_staging(c, B->arena, st);
     if (rc) return rc;
     // sizing run: the front half of every launch, to learn its pair count
     B->caps.resize(G);
@@ -1183,50 +1452,82 @@ static int batch_build(cpv_context* c, cpv_batch* B, const cpv_job_desc* jobs, u
                         (unsigned long long)pairs);
         // room for vertex data that moves a little between submits
         B->caps[g] = (uint32_t)std::min<uint64_t>(pairs + pairs / 8 + 4096, UINT32_MAX);
-        const LaunchPlan& P = B->plans[g];
-        for (cpv_batch::Set& S : B->set) {
-            Exec y = context_exec(c, &S.b[g]);
-            y.cfg = B->cfg;
-            rc = ensure_front(y, P);
-            if (rc == CPV_OK) rc = ensure_back(y, P, B->caps[g]);
-            if (rc) return rc;
-            if (S.b[g].stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
-            fill_u32(c->stream, S.b[g].stats.p, 0u, kStatAlloc);
+        if (resident_workgroups(x, B->plans[g].prog.raster[B->plans[g].raster_variant], 64 * (int)B->cfg.waves_per_tile) <= 0)
+            return CPV_ERROR_DEVICE;
+    }
+    return batch_record(c, B);
+}
+
+// Enqueues one replay of set si (front graph, read-back of its overflow word
+// for the next call's check, raster graph).
+static int batch_launch(cpv_context* c, cpv_batch* B, uint32_t si, bool chained)
+{
+    cpv_batch::Set& S = B->set[si];
+    if (!chained) {
+        // other work was enqueued since this batch's last submit: the front
+        // half waits for all of it
+        HIP_TRY(hipEventRecord(B->pre, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->front_stream, B->pre, 0));
+    }
+    if (S.used) HIP_TRY(hipStreamWaitEvent(c->front_stream, S.raster_done, 0));   // the set's buffers are free again
+    HIP_TRY(hipEventRecord(B->e0, c->front_stream));
+    HIP_TRY(hipGraphLaunch(S.front, c->front_stream));
+    HIP_TRY(hipMemcpyAsync(c->readback + 36, S.overflow(), 4, hipMemcpyDeviceToHost, c->front_stream));
+    HIP_TRY(hipEventRecord(c->ev_check, c->front_stream));
+    HIP_TRY(hipEventRecord(S.front_done, c->front_stream));
+    HIP_TRY(hipStreamWaitEvent(c->stream, S.front_done, 0));
+    HIP_TRY(hipGraphLaunch(S.raster, c->stream));
+    HIP_TRY(hipEventRecord(S.raster_done, c->stream));
+    HIP_TRY(hipEventRecord(B->e1, c->stream));
+    S.used = true;
+    c->pending_batch = B;
+    c->pending_set = si;
+    c->launches += B->plans.size();
+    return CPV_OK;
+}
+
+// A submit of set si raised its overflow word and drew nothing: every
+// launch of the batch is given the capacity its pairs needed (x 1.125 +
+// 4096), both sets are recorded again, and the submit is replayed.
+int cpvrt::batch_redo(cpv_context* c, cpv_batch* B, uint32_t si, bool may_allocate)
+{
+    HIP_TRY(sync_context(c));
+    cpv_batch::Set& S = B->set[si];
+    for (size_t g = 0; g < B->plans.size(); g++) {
+        uint64_t total = 0;
+        HIP_TRY(hipMemcpy(&total, S.b[g].stats.as<uint32_t>() + kStatPairTotal, 8, hipMemcpyDeviceToHost));
+        if (total > (uint64_t)UINT32_MAX)
+            return fail(CPV_ERROR_OVERFLOW,
+                        "a submit of a recorded batch needs %llu (tile, primitive) pairs in launch %zu, more than 2^32 - 1 "
+                        "(nothing was drawn)",
+                        (unsigned long long)total, g);
+        if (total > B->caps[g]) {
+            if (!may_allocate)
+                return fail(CPV_ERROR_OVERFLOW, "a submit of a recorded batch outgrew its pair arrays (nothing was drawn); "
+                                                "call cpv_synchronize before allocation-free calls");
+            B->caps[g] = (uint32_t)std::min<uint64_t>(total + total / 8 + 4096, UINT32_MAX);
         }
-        if (resident_workgroups(x, P.prog.raster[P.raster_variant], 64 * (int)B->cfg.waves_per_tile) <= 0) return CPV_ERROR_DEVICE;
     }
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    // record: no allocation, no host sync, no memset node
-    for (cpv_batch::Set& S : B->set) {
-        std::vector<Binned> bins(G);
-        rc = capture(c->front_stream, &S.front, [&]() {
-            for (size_t g = 0; g < G; g++) {
-                Exec x = context_exec(c, &S.b[g]);
-                x.cfg = B->cfg;
-                x.s = c->front_stream;
-                x.frozen = true;
-                // the fragment counter restarts; the overflow word is cleared
-                // only when cpv_batch_wait has reported it
-                fill_u32(x.s, S.b[g].stats.p, 0u, 2);
-                int r = enqueue_front(x, B->plans[g], arena);
-                if (r == CPV_OK) r = enqueue_bin(x, B->plans[g], arena, B->caps[g], &bins[g]);
-                if (r) return r;
-            }
-            return (int)CPV_OK;
-        });
-        if (rc) return rc;
-        rc = capture(c->stream, &S.raster, [&]() {
-            for (size_t g = 0; g < G; g++) {
-                Exec x = context_exec(c, &S.b[g]);
-                x.cfg = B->cfg;
-                x.frozen = true;
-                const int r = enqueue_raster(x, B->plans[g], arena, bins[g]);
-                if (r) return r;
-            }
-            return (int)CPV_OK;
-        });
-        if (rc) return rc;
+    release_graphs(B);
+    int rc = batch_record(c, B);
+    if (rc) return rc;
+    B->set[0].used = B->set[1].used = false;
+    return batch_launch(c, B, si, false);
+}
+
+static int create_batch(cpv_context* c, const cpv_config& cfg, std::vector<LaunchSpec>& specs, cpv_batch** out)
+{
+    cpv_batch* B = new cpv_batch();
+    B->c = c;
+    B->cfg = cfg;
+    int rc = batch_build(c, B, specs);
+    if (rc) {
+        const std::string msg = cpv_last_error();
+        free_batch(B);
+        return fail(rc, "%s", msg.c_str());
     }
+    note_enqueue(c);
+    *out = B;
     return CPV_OK;
 }
 
@@ -1237,18 +1538,26 @@ extern "C" int cpv_batch_create(cpv_context* c, const cpv_job_desc* jobs, uint32
     std::lock_guard<std::mutex> guard(c->lock);
     HIP_TRY(hipSetDevice(c->device));
     if (int rc = resolve_pending(c)) return rc;
-    cpv_batch* B = new cpv_batch();
-    B->c = c;
-    B->cfg = call_config(c->cfg, jobs_have_points_or_lines(jobs, job_count));
-    int rc = batch_build(c, B, jobs, job_count);
-    if (rc) {
-        const std::string msg = cpv_last_error();
-        cpv_batch_destroy(B);
-        return fail(rc, "%s", msg.c_str());
-    }
-    note_enqueue(c);
-    *out = B;
-    return CPV_OK;
+    const cpv_config cfg = call_config(c->cfg, jobs_have_points_or_lines(jobs, job_count));
+    std::vector<LaunchSpec> specs;
+    if (int rc = group_jobs(jobs, job_count, (int)cfg.tile_size, &specs)) return rc;
+    return create_batch(c, cfg, specs, out);
+}
+
+extern "C" int cpv_batch_create_passes(cpv_context* c, const cpv_pass_desc* passes, uint32_t pass_count, const cpv_pipeline_desc* pipelines,
+                                       const cpv_draw_desc* draws, const uint32_t* draw_pipeline, const uint32_t* first_draw,
+                                       cpv_batch** out)
+{
+    if (!out) return fail(CPV_ERROR_INVALID, "null argument");
+    if (int rc = check_passes_args(c, passes, pass_count, pipelines, draws, draw_pipeline, first_draw)) return rc;
+    if (pass_count == 0) return fail(CPV_ERROR_INVALID, "a batch needs at least one render pass");
+    std::lock_guard<std::mutex> guard(c->lock);
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = resolve_pending(c)) return rc;
+    const cpv_config cfg = call_config(c->cfg, passes_have_points_or_lines(pipelines, draw_pipeline, first_draw[pass_count]));
+    std::vector<LaunchSpec> specs;
+    if (int rc = plan_passes(passes, pass_count, pipelines, draws, draw_pipeline, first_draw, (int)cfg.tile_size, &specs)) return rc;
+    return create_batch(c, cfg, specs, out);
 }
 
 extern "C" int cpv_batch_submit(cpv_batch* B)
@@ -1258,23 +1567,9 @@ extern "C" int cpv_batch_submit(cpv_batch* B)
     std::lock_guard<std::mutex> guard(c->lock);
     HIP_TRY(hipSetDevice(c->device));
     if (int rc = resolve_pending(c)) return rc;
-    cpv_batch::Set& S = B->set[B->submits & 1u];
     const bool chained = B->submits > 0 && B->last_seq == c->enqueue_seq;
-    if (!chained) {
-        // other work was enqueued since this batch's last submit: the front
-        // half waits for all of it
-        HIP_TRY(hipEventRecord(B->pre, c->stream));
-        HIP_TRY(hipStreamWaitEvent(c->front_stream, B->pre, 0));
-    }
-    if (S.used) HIP_TRY(hipStreamWaitEvent(c->front_stream, S.raster_done, 0));   // the set's buffers are free again
-    HIP_TRY(hipEventRecord(B->e0, c->front_stream));
-    HIP_TRY(hipGraphLaunch(S.front, c->front_stream));
-    HIP_TRY(hipEventRecord(S.front_done, c->front_stream));
-    HIP_TRY(hipStreamWaitEvent(c->stream, S.front_done, 0));
-    HIP_TRY(hipGraphLaunch(S.raster, c->stream));
-    HIP_TRY(hipEventRecord(S.raster_done, c->stream));
-    HIP_TRY(hipEventRecord(B->e1, c->stream));
-    S.used = true;
+    int rc = batch_launch(c, B, (uint32_t)(B->submits & 1u), chained);
+    if (rc) return rc;
     B->submits++;
     B->submitted = true;
     note_enqueue(c);
@@ -1286,30 +1581,18 @@ extern "C" int cpv_batch_wait(cpv_batch* B, uint64_t* fragments)
 {
     if (!B) return fail(CPV_ERROR_INVALID, "null batch");
     if (!B->submitted) return fail(CPV_ERROR_INVALID, "batch not submitted");
-    HIP_TRY(hipSetDevice(B->c->device));
+    cpv_context* c = B->c;
+    std::lock_guard<std::mutex> guard(c->lock);
+    HIP_TRY(hipSetDevice(c->device));
+    if (int rc = resolve_pending(c)) return rc;
     HIP_TRY(hipEventSynchronize(B->e1));
-    bool overflow = false;
     uint64_t frags = 0;
     const cpv_batch::Set& last = B->set[(B->submits - 1) & 1u];
-    for (cpv_batch::Set& S : B->set) {
-        if (!S.used) continue;
-        for (cpvrt::Buffers& b : S.b) {
-            uint32_t words[kStatWords];
-            HIP_TRY(hipMemcpy(words, b.stats.p, sizeof(words), hipMemcpyDeviceToHost));
-            if (words[kStatOverflow]) {
-                overflow = true;
-                const uint32_t zero = 0;
-                HIP_TRY(hipMemcpy(b.stats.as<uint32_t>() + kStatOverflow, &zero, 4, hipMemcpyHostToDevice));
-            }
-            if (&S == &last) {
-                uint64_t f;
-                std::memcpy(&f, words + kStatFragments, 8);
-                frags += f;
-            }
-        }
+    for (const cpvrt::Buffers& b : last.b) {
+        uint64_t f = 0;
+        HIP_TRY(hipMemcpy(&f, b.stats.as<uint32_t>() + kStatFragments, 8, hipMemcpyDeviceToHost));
+        frags += f;
     }
-    if (overflow)
-        return fail(CPV_ERROR_OVERFLOW, "the batch's (tile, primitive) pairs exceeded the capacity recorded for it; record it again");
     if (fragments) {
         if (!B->cfg.count_fragments) return fail(CPV_ERROR_INVALID, "the batch was recorded with count_fragments off");
         *fragments = frags;
@@ -1321,9 +1604,135 @@ extern "C" int cpv_batch_timing(cpv_batch* B, float* ms)
 {
     if (!B || !ms) return fail(CPV_ERROR_INVALID, "null argument");
     if (!B->submitted) return fail(CPV_ERROR_INVALID, "batch not submitted");
+    std::lock_guard<std::mutex> guard(B->c->lock);
     HIP_TRY(hipSetDevice(B->c->device));
+    if (int rc = resolve_pending(B->c)) return rc;
     HIP_TRY(hipEventSynchronize(B->e1));
     HIP_TRY(hipEventElapsedTime(ms, B->e0, B->e1));
+    return CPV_OK;
+}
+
+// ------------------------------------------------------------------ fences
+
+// A fence marks the end of the work enqueued on a context up to its signal
+// (Queue::Submit signalling its VkFence after the command buffers,
+// CPVulkan/Queue.cpp:11-77; Fence::Wait / getStatus, CPVulkan/Fence.cpp:
+// 47-56, 157-160).  It is a HIP event recorded on the context's stream,
+// which every launch and transfer ends on.  A fence signalled after a
+// launch whose pair count is still unchecked is recorded again after the
+// redraw if that launch has to be redrawn.
+struct cpv_fence {
+    cpv_context* c = nullptr;
+    hipEvent_t ev = nullptr;
+    bool armed = false;      // an event was recorded by cpv_fence_signal
+    bool signaled = false;   // created signalled, with no signal since
+};
+
+static void unlist_fence(cpv_context* c, cpv_fence* f)
+{
+    auto& v = c->fences_after_pending;
+    v.erase(std::remove(v.begin(), v.end(), f), v.end());
+}
+
+void cpvrt::rerecord_fences(cpv_context* c)
+{
+    for (cpv_fence* f : c->fences_after_pending) (void)hipEventRecord(f->ev, c->stream);
+}
+
+extern "C" int cpv_fence_create(cpv_context* c, int signaled, cpv_fence** out)
+{
+    if (!c || !out) return fail(CPV_ERROR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    cpv_fence* f = new cpv_fence();
+    f->c = c;
+    f->signaled = signaled != 0;
+    if (hipEventCreateWithFlags(&f->ev, hipEventDisableTiming) != hipSuccess) {
+        delete f;
+        return fail(CPV_ERROR_DEVICE, "hipEventCreate failed");
+    }
+    *out = f;
+    return CPV_OK;
+}
+
+extern "C" int cpv_fence_signal(cpv_fence* f)
+{
+    if (!f) return fail(CPV_ERROR_INVALID, "null fence");
+    cpv_context* c = f->c;
+    std::lock_guard<std::mutex> guard(c->lock);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventRecord(f->ev, c->stream));
+    f->armed = true;
+    f->signaled = false;
+    unlist_fence(c, f);
+    if (c->pending || c->pending_batch) c->fences_after_pending.push_back(f);
+    return CPV_OK;
+}
+
+extern "C" int cpv_fence_reset(cpv_fence* f)
+{
+    if (!f) return fail(CPV_ERROR_INVALID, "null fence");
+    std::lock_guard<std::mutex> guard(f->c->lock);
+    unlist_fence(f->c, f);
+    f->armed = false;
+    f->signaled = false;
+    return CPV_OK;
+}
+
+extern "C" int cpv_fence_status(cpv_fence* f)
+{
+    if (!f) return fail(CPV_ERROR_INVALID, "null fence");
+    cpv_context* c = f->c;
+    std::lock_guard<std::mutex> guard(c->lock);
+    if (!f->armed) return f->signaled ? CPV_OK : CPV_NOT_READY;
+    HIP_TRY(hipSetDevice(c->device));
+    for (int pass = 0; pass < 2; pass++) {
+        const hipError_t e = hipEventQuery(f->ev);
+        if (e == hipErrorNotReady) return CPV_NOT_READY;
+        if (e != hipSuccess) return fail(CPV_ERROR_DEVICE, "hipEventQuery: %s", hipGetErrorString(e));
+        const auto& v = c->fences_after_pending;
+        if (std::find(v.begin(), v.end(), f) == v.end()) return CPV_OK;
+        // the work before the fence is done, and with it the pair-count
+        // check it depends on: resolve it (a redraw records the fence again)
+        if (int rc = resolve_pending(c)) return rc;
+    }
+    return CPV_NOT_READY;
+}
+
+extern "C" int cpv_fence_wait(cpv_fence* f, uint64_t timeout_ns)
+{
+    if (!f) return fail(CPV_ERROR_INVALID, "null fence");
+    const auto start = std::chrono::steady_clock::now();
+    for (;;) {
+        const int st = cpv_fence_status(f);
+        if (st != CPV_NOT_READY) return st;
+        const uint64_t waited =
+            (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - start).count();
+        if (waited >= timeout_ns) return CPV_TIMEOUT;
+        if (f->armed && timeout_ns == UINT64_MAX) {
+            (void)hipEventSynchronize(f->ev);
+        } else {
+            const uint64_t left = timeout_ns - waited;
+            std::this_thread::sleep_for(std::chrono::nanoseconds(std::min<uint64_t>(left, 20000)));
+        }
+    }
+}
+
+extern "C" void cpv_fence_destroy(cpv_fence* f)
+{
+    if (!f) return;
+    {
+        std::lock_guard<std::mutex> guard(f->c->lock);
+        unlist_fence(f->c, f);
+    }
+    (void)hipEventDestroy(f->ev);
+    delete f;
+}
+
+extern "C" int cpv_launch_count(cpv_context* c, uint64_t* launches)
+{
+    if (!c || !launches) return fail(CPV_ERROR_INVALID, "null argument");
+    std::lock_guard<std::mutex> guard(c->lock);
+    *launches = c->launches;
     return CPV_OK;
 }
 

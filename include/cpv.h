@@ -80,7 +80,10 @@ enum {
     CPV_ERROR_DEVICE = -3,       /* HIP runtime failure (VK_ERROR_DEVICE_LOST) */
     CPV_ERROR_OUT_OF_MEMORY = -4,
     CPV_ERROR_COMPILE = -5,      /* SPIR-V translation or hiprtc failure */
-    CPV_ERROR_OVERFLOW = -6      /* a recorded batch outgrew the capacity sized when it was recorded */
+    CPV_ERROR_OVERFLOW = -6,     /* more (tile, primitive) pairs than a launch can hold (2^32 - 1, or the
+                                    allocation-free entry point's array) */
+    CPV_NOT_READY = 1,           /* cpv_fence_status: the work before the fence is still running */
+    CPV_TIMEOUT = 2              /* cpv_fence_wait: the timeout passed first */
 };
 
 /* Built-in shader programs.  Each mirrors a GLSL shader of the benchmark
@@ -345,6 +348,24 @@ int cpv_render_pass_noalloc(cpv_context* ctx, const cpv_pass_desc* pass,
                             const cpv_pipeline_desc* pipelines, const cpv_draw_desc* draws,
                             const uint32_t* draw_pipeline, uint32_t draw_count);
 
+/* Several render passes in one call, each with its own draws: draws
+ * first_draw[p] .. first_draw[p + 1] - 1 belong to passes[p] (first_draw has
+ * pass_count + 1 entries, first_draw[0] = 0; draw_pipeline indexes
+ * pipelines as in cpv_render_pass).  The result is that of one
+ * cpv_render_pass per pass, in array order (a vkQueueSubmit replaying its
+ * command buffers' render passes, CPVulkan/Queue.cpp:11-77).  Consecutive
+ * passes that neither write one another's attachments nor sample them are
+ * merged: the k-th run of draws sharing a shader program of every such pass
+ * goes into one launch with one render target per pass, so N independent
+ * single-program passes take one launch. */
+int cpv_render_passes(cpv_context* ctx, const cpv_pass_desc* passes, uint32_t pass_count,
+                      const cpv_pipeline_desc* pipelines, const cpv_draw_desc* draws,
+                      const uint32_t* draw_pipeline, const uint32_t* first_draw);
+
+/* Raster launches enqueued on the context so far (render calls and batch
+ * submits; the launches of a batch submit count once per submit). */
+int cpv_launch_count(cpv_context* ctx, uint64_t* launches);
+
 /* Number of (primitive, pixel) pairs that passed coverage and culling in the
  * last cpv_render_pass / cpv_render_jobs call (synchronises). */
 int cpv_last_fragment_count(cpv_context* ctx, uint64_t* fragments);
@@ -374,11 +395,19 @@ int cpv_get_tile_traces(cpv_context* ctx, cpv_tile_trace* traces, uint32_t capac
  * does, on two alternating sets of working buffers; the results are those
  * of running the submits one after the other.
  * Vertex and index data may change between submits (not while a submit is in
- * flight); if a replay needs more pairs than were recorded, the overflow is
- * detected and cpv_batch_wait reports CPV_ERROR_OVERFLOW (record the batch
- * again). */
+ * flight).  A submit that needs more (tile, primitive) pairs than were
+ * recorded draws nothing; the next call on the context finds it, records
+ * the batch again with the pair counts it needed and replays that submit
+ * before anything else is enqueued, so the results are those of the
+ * submits run in order.  CPV_ERROR_OVERFLOW is returned only for more than
+ * 2^32 - 1 pairs in one launch. */
 typedef struct cpv_batch cpv_batch;
 int cpv_batch_create(cpv_context* ctx, const cpv_job_desc* jobs, uint32_t job_count, cpv_batch** out);
+/* A batch of render passes laid out as for cpv_render_passes (a command
+ * buffer's render passes, recorded once). */
+int cpv_batch_create_passes(cpv_context* ctx, const cpv_pass_desc* passes, uint32_t pass_count,
+                            const cpv_pipeline_desc* pipelines, const cpv_draw_desc* draws,
+                            const uint32_t* draw_pipeline, const uint32_t* first_draw, cpv_batch** out);
 int cpv_batch_submit(cpv_batch* batch);
 /* Waits for the last submit; *fragments (optional) receives its covered
  * fragment count when the batch was recorded with count_fragments on. */
@@ -386,6 +415,22 @@ int cpv_batch_wait(cpv_batch* batch, uint64_t* fragments);
 /* Device time of the last submit in milliseconds (HIP events around the graph). */
 int cpv_batch_timing(cpv_batch* batch, float* ms);
 void cpv_batch_destroy(cpv_batch* batch);
+
+/* ---- fences ----
+ * A fence marks the end of the work enqueued on its context up to
+ * cpv_fence_signal (a VkFence signalled by vkQueueSubmit,
+ * CPVulkan/Queue.cpp:68-71).  cpv_fence_status returns CPV_OK once that work
+ * has finished (or when the fence was created signalled and not signalled
+ * or reset since) and CPV_NOT_READY before; cpv_fence_wait returns CPV_OK or
+ * CPV_TIMEOUT after timeout_ns nanoseconds (UINT64_MAX: no limit), as
+ * Fence::Wait and getStatus do (CPVulkan/Fence.cpp:47-56, 157-160). */
+typedef struct cpv_fence cpv_fence;
+int cpv_fence_create(cpv_context* ctx, int signaled, cpv_fence** out);
+int cpv_fence_signal(cpv_fence* fence);
+int cpv_fence_reset(cpv_fence* fence);
+int cpv_fence_status(cpv_fence* fence);
+int cpv_fence_wait(cpv_fence* fence, uint64_t timeout_ns);
+void cpv_fence_destroy(cpv_fence* fence);
 
 /* ---- autotuner ----
  * Records the jobs as a batch under every tile configuration (tile size 8,

@@ -116,9 +116,11 @@ def test_render_jobs_immediate(ctx):
         mem.release()
 
 
-def test_batch_overflow_detected(ctx):
-    """Vertex data that grows after recording must be reported, not drawn
-    out of bounds: the emit kernels drop pairs past the recorded capacity."""
+def test_batch_outgrowing_its_capacity_is_redrawn(ctx):
+    """Vertex data that grows after recording: the emit kernels drop pairs
+    past the recorded capacity (never writing out of bounds), the submit
+    draws nothing, and the next call records the batch again and redraws
+    it."""
     scenes = _jobs(4, seed=8)
     mem, ms, jobs = materialize_jobs(ctx, scenes)
     try:
@@ -126,14 +128,21 @@ def test_batch_overflow_detected(ctx):
             b.submit()
             b.wait()
             # grow every triangle 50x about its centroid: many more (tile, primitive) pairs
+            grown = []
             for s, m in zip(scenes, ms):
-                v = s.draws[0].vertices[0].view(np.float32).reshape(-1, 3, 8).copy()
+                g = copy.deepcopy(s)
+                v = g.draws[0].vertices[0].view(np.float32).reshape(-1, 3, 8).copy()
                 c = v[:, :, 0:2].mean(axis=1, keepdims=True)
                 v[:, :, 0:2] = c + (v[:, :, 0:2] - c) * 50.0
+                g.draws[0].vertices[0] = v.reshape(-1).view(np.uint8)
                 mem.upload(m.draws[0].vertex_address[0], v)
+                grown.append(g)
+            refs = oracle_jobs(grown)
+            for m in ms:
+                m.reset_attachments()
             b.submit()
-            with pytest.raises(cpvulkan_amd.CpvError, match="exceeded the capacity"):
-                b.wait()
+            assert b.wait(count=True) == sum(r[2] for r in refs)
+            compare_jobs(grown, ms, refs, "redrawn after regrowth")
     finally:
         mem.release()
 
