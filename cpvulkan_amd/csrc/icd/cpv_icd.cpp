@@ -7,8 +7,12 @@
 // include/cpv.h.  The object model is the minimum the draw path needs; it
 // mirrors the reference's (CPVulkan/*.cpp): commands are recorded into the
 // command buffer and executed at submit (CommandBuffer.h:161,
-// CommandBuffer.cpp:964-968); a render pass's draws are collected and, at
-// vkCmdEndRenderPass, issued as one cpv_render_pass.
+// CommandBuffer.cpp:964-968).  A submit gathers the render passes of its
+// command buffers and hands them to libcpv in one cpv_render_passes call
+// (independent passes share launches); a command buffer recorded without
+// ONE_TIME_SUBMIT whose commands are only render passes is recorded once as
+// a libcpv batch (hipGraphs) and replayed by each submit; a VkFence is a HIP
+// event recorded after the submit's work.
 //
 // Memory: memory type 0 is device-local HBM (cpv_alloc); memory type 1 is
 // host-visible, host-coherent pinned memory (cpv_alloc_host) that kernels
@@ -19,13 +23,16 @@
 // Errors never abort: unsupported state returns VK_ERROR_FEATURE_NOT_PRESENT
 // (where the reference calls TODO_ERROR), HIP failures VK_ERROR_DEVICE_LOST.
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../../include/cpv.h"
@@ -86,6 +93,7 @@ struct Device : Dispatchable {
     cpv_context* ctx = nullptr;
     Queue queue;
     std::mutex lock;
+    uint64_t submits = 0;
 };
 
 struct Memory {
@@ -173,7 +181,7 @@ struct CommandPool {};
 
 enum class Op {
     BeginPass, EndPass, BindPipeline, BindVertex, BindIndex, BindSets, SetViewport, Draw, DrawIndexed, DrawIndirect,
-    DrawIndexedIndirect, CopyBuffer, CopyImage, CopyBufferToImage, CopyImageToBuffer, FillBuffer, UpdateBuffer, ClearColorImage,
+    DrawIndexedIndirect, DrawIndirectCount, DrawIndexedIndirectCount, CopyBuffer, CopyImage, CopyBufferToImage, CopyImageToBuffer, FillBuffer, UpdateBuffer, ClearColorImage,
     ClearDepthStencilImage, ClearAttachments, BlitImage, SetBlendConstants, SetDepthBounds, SetStencilCompareMask,
     SetStencilWriteMask, SetStencilReference
 };
@@ -194,6 +202,8 @@ struct Cmd {
     int32_t vertex_offset = 0;
     // transfers, clears and indirect draws
     Buffer* src_buffer = nullptr;
+    Buffer* count_buffer = nullptr;     // DrawIndirectCount: the draw count's buffer, at count_offset
+    VkDeviceSize count_offset = 0;
     Buffer* dst_buffer = nullptr;
     Image* src_image = nullptr;
     Image* dst_image = nullptr;
@@ -215,11 +225,22 @@ struct Cmd {
 struct CommandBuffer : Dispatchable {
     Device* device = nullptr;
     std::vector<Cmd> cmds;
+    bool one_time = false;          // recorded with VK_COMMAND_BUFFER_USAGE_ONE_TIME_SUBMIT_BIT
+    cpv_batch* batch = nullptr;     // its render passes recorded once, replayed by every submit
+    bool batch_tried = false;       // a batch was attempted (and failed or was not possible)
+    void forget()
+    {
+        if (batch) cpv_batch_destroy(batch);
+        batch = nullptr;
+        batch_tried = false;
+        cmds.clear();
+    }
+    ~CommandBuffer() { forget(); }
 };
 
+// A VkFence: a libcpv fence (a HIP event recorded after the submit's work).
 struct Fence {
-    bool signaled = false;
-    Device* device = nullptr;
+    cpv_fence* f = nullptr;
 };
 
 template <class H, class T> H handle(T* p) { return reinterpret_cast<H>(p); }
@@ -355,10 +376,20 @@ VkResult EnumerateExtensionProperties(const char*, uint32_t* count, VkExtensionP
     return VK_SUCCESS;
 }
 
-VkResult EnumerateDeviceExtensionProperties(VkPhysicalDevice, const char*, uint32_t* count, VkExtensionProperties*)
+// Device extensions: the indirect-count draws (CPVulkan/Extensions.cpp:765-775, 1030-1040).
+const VkExtensionProperties kDeviceExtensions[] = {{"VK_KHR_draw_indirect_count", 1}, {"VK_AMD_draw_indirect_count", 2}};
+
+VkResult EnumerateDeviceExtensionProperties(VkPhysicalDevice, const char* layer, uint32_t* count, VkExtensionProperties* out)
 {
-    *count = 0;
-    return VK_SUCCESS;
+    const uint32_t n = layer ? 0u : (uint32_t)(sizeof(kDeviceExtensions) / sizeof(kDeviceExtensions[0]));
+    if (!out) {
+        *count = n;
+        return VK_SUCCESS;
+    }
+    const uint32_t k = std::min(*count, n);
+    for (uint32_t i = 0; i < k; i++) out[i] = kDeviceExtensions[i];
+    *count = k;
+    return k < n ? VK_INCOMPLETE : VK_SUCCESS;
 }
 
 VkResult EnumerateLayerProperties(uint32_t* count, VkLayerProperties*)
@@ -369,7 +400,11 @@ VkResult EnumerateLayerProperties(uint32_t* count, VkLayerProperties*)
 
 VkResult CreateDevice(VkPhysicalDevice, const VkDeviceCreateInfo* info, const VkAllocationCallbacks*, VkDevice* out)
 {
-    if (info && info->enabledExtensionCount) return VK_ERROR_EXTENSION_NOT_PRESENT;
+    for (uint32_t i = 0; info && i < info->enabledExtensionCount; i++) {
+        bool known = false;
+        for (const VkExtensionProperties& e : kDeviceExtensions) known = known || std::strcmp(e.extensionName, info->ppEnabledExtensionNames[i]) == 0;
+        if (!known) return VK_ERROR_EXTENSION_NOT_PRESENT;
+    }
     Device* d = new (std::nothrow) Device();
     if (!d) return VK_ERROR_OUT_OF_HOST_MEMORY;
     const int rc = cpv_context_create(0, &d->ctx);
@@ -838,9 +873,13 @@ void FreeCommandBuffers(VkDevice, VkCommandPool, uint32_t n, const VkCommandBuff
     for (uint32_t i = 0; i < n; i++) delete reinterpret_cast<CommandBuffer*>(cbs[i]);
 }
 
-VkResult BeginCommandBuffer(VkCommandBuffer cb, const VkCommandBufferBeginInfo*)
+// Beginning a command buffer resets it (CommandBuffer.cpp:964-968): a batch
+// recorded from its previous contents is dropped.
+VkResult BeginCommandBuffer(VkCommandBuffer cb, const VkCommandBufferBeginInfo* info)
 {
-    reinterpret_cast<CommandBuffer*>(cb)->cmds.clear();
+    CommandBuffer* c = reinterpret_cast<CommandBuffer*>(cb);
+    c->forget();
+    c->one_time = info && (info->flags & VK_COMMAND_BUFFER_USAGE_ONE_TIME_SUBMIT_BIT);
     return VK_SUCCESS;
 }
 
@@ -848,7 +887,7 @@ VkResult EndCommandBuffer(VkCommandBuffer) { return VK_SUCCESS; }
 
 VkResult ResetCommandBuffer(VkCommandBuffer cb, VkFlags)
 {
-    reinterpret_cast<CommandBuffer*>(cb)->cmds.clear();
+    reinterpret_cast<CommandBuffer*>(cb)->forget();
     return VK_SUCCESS;
 }
 
@@ -984,6 +1023,35 @@ void CmdDrawIndexedIndirect(VkCommandBuffer cb, VkBuffer buffer, VkDeviceSize of
     CB(cb)->cmds.push_back(c);
 }
 
+// vkCmdDraw(Indexed)IndirectCount{KHR,AMD}: at most maxDrawCount records,
+// as many as the count buffer holds when the command runs
+// (DrawIndirectCountCommand / DrawIndexedIndirectCountCommand,
+// CommandBuffer.Draw.cpp:2226-2348).
+void draw_indirect_count(VkCommandBuffer cb, Op op, VkBuffer buffer, VkDeviceSize offset, VkBuffer countBuffer,
+                         VkDeviceSize countBufferOffset, uint32_t maxDrawCount, uint32_t stride)
+{
+    Cmd c{op};
+    c.src_buffer = reinterpret_cast<Buffer*>(buffer);
+    c.offset = offset;
+    c.count_buffer = reinterpret_cast<Buffer*>(countBuffer);
+    c.count_offset = countBufferOffset;
+    c.count = maxDrawCount;
+    c.stride = stride;
+    CB(cb)->cmds.push_back(c);
+}
+
+void CmdDrawIndirectCount(VkCommandBuffer cb, VkBuffer buffer, VkDeviceSize offset, VkBuffer countBuffer, VkDeviceSize countBufferOffset,
+                          uint32_t maxDrawCount, uint32_t stride)
+{
+    draw_indirect_count(cb, Op::DrawIndirectCount, buffer, offset, countBuffer, countBufferOffset, maxDrawCount, stride);
+}
+
+void CmdDrawIndexedIndirectCount(VkCommandBuffer cb, VkBuffer buffer, VkDeviceSize offset, VkBuffer countBuffer,
+                                 VkDeviceSize countBufferOffset, uint32_t maxDrawCount, uint32_t stride)
+{
+    draw_indirect_count(cb, Op::DrawIndexedIndirectCount, buffer, offset, countBuffer, countBufferOffset, maxDrawCount, stride);
+}
+
 void CmdCopyBuffer(VkCommandBuffer cb, VkBuffer src, VkBuffer dst, uint32_t n, const VkBufferCopy* regions)
 {
     Cmd c{Op::CopyBuffer};
@@ -1103,6 +1171,45 @@ struct ReplayState {
     std::vector<uint32_t> draw_pipeline;
 };
 
+// A render pass ready for libcpv, with its draws: the passes of a submit
+// are gathered and handed over together (cpv_render_passes), so that
+// independent passes share launches.
+struct QueuedPass {
+    cpv_pass_desc pass;
+    std::vector<cpv_pipeline_desc> pipelines;
+    std::vector<cpv_draw_desc> draws;
+    std::vector<uint32_t> draw_pipeline;
+};
+
+// The flat arrays of cpv_render_passes / cpv_batch_create_passes.
+struct PassArrays {
+    std::vector<cpv_pass_desc> passes;
+    std::vector<cpv_pipeline_desc> pipelines;
+    std::vector<cpv_draw_desc> draws;
+    std::vector<uint32_t> draw_pipeline, first_draw;
+    explicit PassArrays(const std::vector<QueuedPass>& q)
+    {
+        first_draw.push_back(0);
+        for (const QueuedPass& p : q) {
+            const uint32_t base = (uint32_t)pipelines.size();
+            passes.push_back(p.pass);
+            pipelines.insert(pipelines.end(), p.pipelines.begin(), p.pipelines.end());
+            draws.insert(draws.end(), p.draws.begin(), p.draws.end());
+            for (uint32_t k : p.draw_pipeline) draw_pipeline.push_back(base + k);
+            first_draw.push_back((uint32_t)draws.size());
+        }
+    }
+};
+
+VkResult flush_passes(Device* dev, std::vector<QueuedPass>& q)
+{
+    if (q.empty()) return VK_SUCCESS;
+    const PassArrays a(q);
+    q.clear();
+    return device_error(cpv_render_passes(dev->ctx, a.passes.data(), (uint32_t)a.passes.size(), a.pipelines.data(), a.draws.data(),
+                                          a.draw_pipeline.data(), a.first_draw.data()));
+}
+
 cpv_draw_desc make_draw(const ReplayState& s, const Cmd& c)
 {
     cpv_draw_desc d;
@@ -1148,7 +1255,7 @@ cpv_draw_desc make_draw(const ReplayState& s, const Cmd& c)
     return d;
 }
 
-VkResult end_pass(Device* dev, ReplayState& s)
+VkResult end_pass(ReplayState& s, std::vector<QueuedPass>* queue)
 {
     const bool first_segment = !s.pass_split;
     cpv_pass_desc p;
@@ -1172,12 +1279,14 @@ VkResult end_pass(Device* dev, ReplayState& s)
         p.has_depth_stencil = 1;
         attach(s.pass->depth, &p.depth_stencil);
     }
-    const int rc = cpv_render_pass(dev->ctx, &p, s.pipelines.data(), s.draws.data(), s.draw_pipeline.data(), (uint32_t)s.draws.size());
-    s.pipelines.clear();
+    QueuedPass qp;
+    qp.pass = p;
+    qp.pipelines.swap(s.pipelines);
+    qp.draws.swap(s.draws);
+    qp.draw_pipeline.swap(s.draw_pipeline);
     s.pipeline_objs.clear();
-    s.draws.clear();
-    s.draw_pipeline.clear();
-    return device_error(rc);
+    queue->push_back(std::move(qp));
+    return VK_SUCCESS;
 }
 
 // ------------------------------------------------------ transfer replay
@@ -1339,10 +1448,11 @@ VkResult run_transfer(Device* dev, const Cmd& c)
 // render pass: the draws recorded so far are issued first, then each
 // rectangle of the attachment's first view level is set to the clear value
 // (colour, or depth and stencil together, as the reference writes both).
-VkResult clear_attachments(Device* dev, ReplayState& s, const Cmd& c)
+VkResult clear_attachments(Device* dev, ReplayState& s, const Cmd& c, std::vector<QueuedPass>* queue)
 {
     if (!s.pass || !s.fb) return VK_ERROR_DEVICE_LOST;
-    if (VkResult e = end_pass(dev, s)) return e;
+    if (VkResult e = end_pass(s, queue)) return e;
+    if (VkResult e = flush_passes(dev, *queue)) return e;
     s.pass_split = true;
     for (const VkClearAttachment& a : c.attachments) {
         const int32_t idx = (a.aspectMask & VK_IMAGE_ASPECT_COLOR_BIT) ? s.pass->color : s.pass->depth;
@@ -1365,22 +1475,36 @@ VkResult clear_attachments(Device* dev, ReplayState& s, const Cmd& c)
 // read their VkDraw(Indexed)IndirectCommand records when the command runs;
 // here they are read when the command buffer is replayed, after the stream
 // has finished the work recorded before them, and become direct draws.
+// Reads bytes of a buffer once the work enqueued before has finished.
+VkResult read_buffer(Device* dev, const Buffer* b, VkDeviceSize offset, void* out, uint64_t bytes)
+{
+    if (offset + bytes > b->size) return VK_ERROR_DEVICE_LOST;
+    if (b->memory && b->memory->host) {
+        if (VkResult e = device_error(cpv_synchronize(dev->ctx))) return e;
+        std::memcpy(out, (const void*)(uintptr_t)(b->address() + offset), bytes);
+        return VK_SUCCESS;
+    }
+    return device_error(cpv_download(dev->ctx, out, b->address() + offset, bytes));
+}
+
 VkResult read_indirect(Device* dev, const Cmd& c, std::vector<Cmd>* out)
 {
-    const bool indexed = c.op == Op::DrawIndexedIndirect;
+    const bool indexed = c.op == Op::DrawIndexedIndirect || c.op == Op::DrawIndexedIndirectCount;
+    const bool counted = c.op == Op::DrawIndirectCount || c.op == Op::DrawIndexedIndirectCount;
     const uint32_t rec = indexed ? (uint32_t)sizeof(VkDrawIndexedIndirectCommand) : (uint32_t)sizeof(VkDrawIndirectCommand);
-    const uint32_t stride = c.count > 1 ? c.stride : rec;
-    if (!c.count) return VK_SUCCESS;
-    const uint64_t span = (uint64_t)stride * (c.count - 1) + rec;
-    if (c.offset + span > c.src_buffer->size) return VK_ERROR_DEVICE_LOST;
-    std::vector<uint8_t> host(span);
-    if (c.src_buffer->memory && c.src_buffer->memory->host) {
-        if (VkResult e = device_error(cpv_synchronize(dev->ctx))) return e;
-        std::memcpy(host.data(), (const void*)(uintptr_t)(c.src_buffer->address() + c.offset), span);
-    } else {
-        if (VkResult e = device_error(cpv_download(dev->ctx, host.data(), c.src_buffer->address() + c.offset, span))) return e;
+    uint32_t count = c.count;
+    if (counted) {
+        // min(maxDrawCount, the count buffer's value) records (CommandBuffer.Draw.cpp:2247, 2305)
+        uint32_t n = 0;
+        if (VkResult e = read_buffer(dev, c.count_buffer, c.count_offset, &n, 4)) return e;
+        count = std::min(count, n);
     }
-    for (uint32_t j = 0; j < c.count; j++) {
+    const uint32_t stride = (counted || count > 1) && c.stride ? c.stride : rec;
+    if (!count) return VK_SUCCESS;
+    const uint64_t span = (uint64_t)stride * (count - 1) + rec;
+    std::vector<uint8_t> host(span);
+    if (VkResult e = read_buffer(dev, c.src_buffer, c.offset, host.data(), span)) return e;
+    for (uint32_t j = 0; j < count; j++) {
         Cmd d{indexed ? Op::DrawIndexed : Op::Draw};
         if (indexed) {
             VkDrawIndexedIndirectCommand r;
@@ -1433,7 +1557,10 @@ VkResult record_draw(ReplayState& s, const Cmd& c)
     return VK_SUCCESS;
 }
 
-VkResult replay(Device* dev, const CommandBuffer* cb)
+// Replays a command buffer: its render passes are queued for the submit's
+// cpv_render_passes call; transfers, clears inside a pass and indirect
+// reads first hand over the passes queued before them.
+VkResult replay(Device* dev, const CommandBuffer* cb, std::vector<QueuedPass>* queue)
 {
     ReplayState s;
     for (const Cmd& c : cb->cmds) {
@@ -1445,7 +1572,7 @@ VkResult replay(Device* dev, const CommandBuffer* cb)
             s.pass_split = false;
             break;
         case Op::EndPass: {
-            const VkResult r = end_pass(dev, s);
+            const VkResult r = end_pass(s, queue);
             if (r != VK_SUCCESS) return r;
             s.pass = nullptr;
             break;
@@ -1478,7 +1605,7 @@ VkResult replay(Device* dev, const CommandBuffer* cb)
             if (!s.pass || !s.pipeline) return VK_ERROR_DEVICE_LOST;
             if (VkResult e = record_draw(s, c)) return e;
             break;
-        case Op::DrawIndirect: case Op::DrawIndexedIndirect: {
+        case Op::DrawIndirect: case Op::DrawIndexedIndirect: case Op::DrawIndirectCount: case Op::DrawIndexedIndirectCount: {
             std::vector<Cmd> direct;
             if (VkResult e = read_indirect(dev, c, &direct)) return e;
             for (const Cmd& d : direct)
@@ -1486,10 +1613,11 @@ VkResult replay(Device* dev, const CommandBuffer* cb)
             break;
         }
         case Op::ClearAttachments:
-            if (VkResult e = clear_attachments(dev, s, c)) return e;
+            if (VkResult e = clear_attachments(dev, s, c, queue)) return e;
             break;
         default:
             if (s.pass) return VK_ERROR_DEVICE_LOST;   // transfers are recorded outside render passes
+            if (VkResult e = flush_passes(dev, *queue)) return e;
             if (VkResult e = run_transfer(dev, c)) return e;
             break;
         }
@@ -1497,59 +1625,141 @@ VkResult replay(Device* dev, const CommandBuffer* cb)
     return VK_SUCCESS;
 }
 
+// A command buffer holding only render passes with direct draws and their
+// state can be recorded once as a libcpv batch.
+bool batchable(const CommandBuffer* cb)
+{
+    bool any_pass = false;
+    for (const Cmd& c : cb->cmds) {
+        switch (c.op) {
+        case Op::BeginPass: any_pass = true; break;
+        case Op::EndPass: case Op::BindPipeline: case Op::BindVertex: case Op::BindIndex: case Op::BindSets: case Op::SetViewport:
+        case Op::Draw: case Op::DrawIndexed: case Op::SetBlendConstants: case Op::SetDepthBounds: case Op::SetStencilCompareMask:
+        case Op::SetStencilWriteMask: case Op::SetStencilReference:
+            break;
+        default:
+            return false;
+        }
+    }
+    return any_pass;
+}
+
+// The batch of a command buffer recorded without ONE_TIME_SUBMIT: its
+// render passes, recorded into hipGraphs on its first submit (with the
+// context's tile configuration, scheduler and predictor at that time) and
+// replayed by every later submit with no host work.
+cpv_batch* command_buffer_batch(Device* dev, CommandBuffer* cb)
+{
+    if (cb->batch || cb->batch_tried || cb->one_time || !batchable(cb)) return cb->batch;
+    cb->batch_tried = true;
+    std::vector<QueuedPass> q;
+    if (replay(dev, cb, &q) != VK_SUCCESS || q.empty()) return nullptr;
+    const PassArrays a(q);
+    cpv_batch* b = nullptr;
+    if (cpv_batch_create_passes(dev->ctx, a.passes.data(), (uint32_t)a.passes.size(), a.pipelines.data(), a.draws.data(),
+                                a.draw_pipeline.data(), a.first_draw.data(), &b) != CPV_OK)
+        return nullptr;   // replayed pass by pass instead
+    cb->batch = b;
+    return b;
+}
+
+// Queue::Submit (CPVulkan/Queue.cpp:11-77) runs every command buffer and then
+// signals the fence.  Here the render passes of all command buffers of the
+// call are gathered and handed to libcpv together, a reusable command
+// buffer replays its recorded batch, and the fence is a HIP event after the
+// submit's work: nothing waits on the host.
 VkResult QueueSubmit(VkQueue q, uint32_t n, const VkSubmitInfo* submits, VkFence fence)
 {
     Device* dev = reinterpret_cast<Queue*>(q)->device;
     std::lock_guard<std::mutex> guard(dev->lock);
+    uint64_t before = 0;
+    const bool trace = std::getenv("CPVULKAN_AMD_TRACE_SUBMITS") != nullptr;
+    if (trace) (void)cpv_launch_count(dev->ctx, &before);
+    std::vector<QueuedPass> queue;
     for (uint32_t i = 0; i < n; i++) {
         for (uint32_t k = 0; k < submits[i].commandBufferCount; k++) {
-            const VkResult r = replay(dev, reinterpret_cast<CommandBuffer*>(submits[i].pCommandBuffers[k]));
-            if (r != VK_SUCCESS) return r;
+            CommandBuffer* cb = reinterpret_cast<CommandBuffer*>(submits[i].pCommandBuffers[k]);
+            if (cpv_batch* b = command_buffer_batch(dev, cb)) {
+                if (VkResult e = flush_passes(dev, queue)) return e;
+                if (VkResult e = device_error(cpv_batch_submit(b))) return e;
+                continue;
+            }
+            if (VkResult e = replay(dev, cb, &queue)) return e;
         }
     }
+    if (VkResult e = flush_passes(dev, queue)) return e;
     if (fence) {
-        Fence* f = reinterpret_cast<Fence*>(fence);
-        f->device = dev;
-        f->signaled = false;   // signalled once the stream has drained (vkWaitForFences / vkGetFenceStatus)
+        if (VkResult e = device_error(cpv_fence_signal(reinterpret_cast<Fence*>(fence)->f))) return e;
+    }
+    if (trace) {
+        uint64_t after = 0;
+        (void)cpv_launch_count(dev->ctx, &after);
+        dev->submits++;
+        std::fprintf(stderr, "cpvulkan_amd: submit %llu: %llu launches\n", (unsigned long long)dev->submits,
+                     (unsigned long long)(after - before));
     }
     return VK_SUCCESS;
 }
 
 VkResult QueueWaitIdle(VkQueue q) { return device_error(cpv_synchronize(reinterpret_cast<Queue*>(q)->device->ctx)); }
 
-VkResult CreateFence(VkDevice, const VkFenceCreateInfo* info, const VkAllocationCallbacks*, VkFence* out)
+VkResult CreateFence(VkDevice dev, const VkFenceCreateInfo* info, const VkAllocationCallbacks*, VkFence* out)
 {
-    Fence* f = new Fence();
-    f->signaled = (info->flags & VK_FENCE_CREATE_SIGNALED_BIT) != 0;
+    Fence* f = new (std::nothrow) Fence();
+    if (!f) return VK_ERROR_OUT_OF_HOST_MEMORY;
+    const int signaled = (info->flags & VK_FENCE_CREATE_SIGNALED_BIT) != 0;
+    if (cpv_fence_create(reinterpret_cast<Device*>(dev)->ctx, signaled, &f->f) != CPV_OK) {
+        delete f;
+        return VK_ERROR_OUT_OF_DEVICE_MEMORY;
+    }
     *out = handle<VkFence>(f);
     return VK_SUCCESS;
 }
 
-void DestroyFence(VkDevice, VkFence f, const VkAllocationCallbacks*) { delete reinterpret_cast<Fence*>(f); }
-
-VkResult WaitForFences(VkDevice, uint32_t n, const VkFence* fences, VkBool32, uint64_t)
-{
-    for (uint32_t i = 0; i < n; i++) {
-        Fence* f = reinterpret_cast<Fence*>(fences[i]);
-        if (!f->signaled && f->device) {
-            const VkResult r = device_error(cpv_synchronize(f->device->ctx));
-            if (r != VK_SUCCESS) return r;
-        }
-        f->signaled = true;
-    }
-    return VK_SUCCESS;
-}
-
-VkResult GetFenceStatus(VkDevice dev, VkFence fence)
+void DestroyFence(VkDevice, VkFence fence, const VkAllocationCallbacks*)
 {
     Fence* f = reinterpret_cast<Fence*>(fence);
-    if (!f->signaled) return WaitForFences(dev, 1, &fence, VK_TRUE, ~0ull);
-    return VK_SUCCESS;
+    if (!f) return;
+    cpv_fence_destroy(f->f);
+    delete f;
+}
+
+VkResult fence_result(int rc)
+{
+    if (rc == CPV_NOT_READY) return VK_NOT_READY;
+    if (rc == CPV_TIMEOUT) return VK_TIMEOUT;
+    return device_error(rc);
+}
+
+VkResult GetFenceStatus(VkDevice, VkFence fence) { return fence_result(cpv_fence_status(reinterpret_cast<Fence*>(fence)->f)); }
+
+// vkWaitForFences: all of the fences (waitAll) or any one, within timeout
+// nanoseconds (VK_TIMEOUT after it; Device::WaitForFences,
+// CPVulkan/Fence.cpp:192-200).
+VkResult WaitForFences(VkDevice, uint32_t n, const VkFence* fences, VkBool32 waitAll, uint64_t timeout)
+{
+    if (n == 1) return fence_result(cpv_fence_wait(reinterpret_cast<Fence*>(fences[0])->f, timeout));
+    const auto start = std::chrono::steady_clock::now();
+    for (;;) {
+        uint32_t done = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const int rc = cpv_fence_status(reinterpret_cast<Fence*>(fences[i])->f);
+            if (rc < 0) return device_error(rc);
+            if (rc == CPV_OK) done++;
+        }
+        if (waitAll ? done == n : done > 0) return VK_SUCCESS;
+        const uint64_t waited =
+            (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - start).count();
+        if (waited >= timeout) return VK_TIMEOUT;
+        std::this_thread::sleep_for(std::chrono::nanoseconds(std::min<uint64_t>(timeout - waited, 20000)));
+    }
 }
 
 VkResult ResetFences(VkDevice, uint32_t n, const VkFence* fences)
 {
-    for (uint32_t i = 0; i < n; i++) reinterpret_cast<Fence*>(fences[i])->signaled = false;
+    for (uint32_t i = 0; i < n; i++) {
+        if (VkResult e = device_error(cpv_fence_reset(reinterpret_cast<Fence*>(fences[i])->f))) return e;
+    }
     return VK_SUCCESS;
 }
 
@@ -1584,7 +1794,12 @@ const Entry kDevice[] = {
     E(CmdBindIndexBuffer), E(CmdBindDescriptorSets), E(CmdSetViewport), E(CmdSetScissor), E(CmdSetDepthBounds),
     E(CmdSetLineWidth), E(CmdSetBlendConstants), E(CmdSetStencilCompareMask), E(CmdSetStencilWriteMask),
     E(CmdSetStencilReference), E(CmdDraw), E(CmdDrawIndexed),
-    E(CmdDrawIndirect), E(CmdDrawIndexedIndirect), E(CmdCopyBuffer), E(CmdCopyImage), E(CmdCopyBufferToImage),
+    E(CmdDrawIndirect), E(CmdDrawIndexedIndirect),
+    {"vkCmdDrawIndirectCountKHR", reinterpret_cast<PFN_vkVoidFunction>(&CmdDrawIndirectCount)},
+    {"vkCmdDrawIndexedIndirectCountKHR", reinterpret_cast<PFN_vkVoidFunction>(&CmdDrawIndexedIndirectCount)},
+    {"vkCmdDrawIndirectCountAMD", reinterpret_cast<PFN_vkVoidFunction>(&CmdDrawIndirectCount)},
+    {"vkCmdDrawIndexedIndirectCountAMD", reinterpret_cast<PFN_vkVoidFunction>(&CmdDrawIndexedIndirectCount)},
+    E(CmdCopyBuffer), E(CmdCopyImage), E(CmdCopyBufferToImage),
     E(CmdCopyImageToBuffer), E(CmdFillBuffer), E(CmdUpdateBuffer), E(CmdClearColorImage), E(CmdClearDepthStencilImage),
     E(CmdClearAttachments), E(CmdBlitImage),
     E(QueueSubmit), E(QueueWaitIdle), E(CreateFence), E(DestroyFence), E(WaitForFences), E(GetFenceStatus), E(ResetFences),

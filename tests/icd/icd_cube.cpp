@@ -2,7 +2,7 @@
 // frame through the driver's own entry points (as the loader would reach
 // them) and writes the colour and depth images.
 //
-//   icd_cube <icd.so> <dir> <width> <height> [draw|indirect|midclear|lines|dynamic]
+//   icd_cube <icd.so> <dir> <width> <height> [mode]
 //
 // draw: vkCmdDraw(36, 1, 0, 0); indirect: the same vertices as two
 // VkDrawIndirectCommand records read by vkCmdDrawIndirect; midclear:
@@ -11,11 +11,27 @@
 // then vkCmdDraw again, all in one render pass; lines: the vertices as a
 // LINE_LIST of width 3; dynamic: constant-colour blending and a depth-bounds
 // test with dynamic constants and range, drawn twice: constants (0.25, 0.5,
-// 0.75, 1) and bounds [0, 1], then (0.8, 0.1, 0.3, 0.5) and [0, 0.995].
+// 0.75, 1) and bounds [0, 1], then (0.8, 0.1, 0.3, 0.5) and [0, 0.995];
+// indexed_indirect: a uint16 index buffer 0..35 and two
+// VkDrawIndexedIndirectCommand records read by vkCmdDrawIndexedIndirect;
+// indirect_count / indexed_indirect_count: vkCmdDrawIndirectCountKHR /
+// vkCmdDrawIndexedIndirectCountAMD with three records, maxDrawCount 3 and a
+// count buffer holding 2 (the third record, which would draw the whole cube
+// again shifted, must be ignored); stencil: a D24_UNORM_S8_UINT attachment,
+// stencil test EQUAL / REPLACE with compare mask, write mask and reference
+// all dynamic, drawn twice: masks 0x00 / 0xF0 with reference 0xA5, then
+// masks 0xF0 / 0x0F with references 0xAF (front) and 0xA3 (back);
+// multipass: three framebuffers (own colour and depth images, clear colours
+// 0.2, 0.3, 0.4) drawn by three render passes of one command buffer, one
+// submit; resubmit: the same command buffer recorded without
+// ONE_TIME_SUBMIT and submitted three times, the uniform rewritten from
+// mvp0.bin, mvp1.bin, mvp2.bin before each submit, with the fence checks
+// printed as "fence <what>: <VkResult>".
 //
 // <dir> holds vs.spv, fs.spv (SPIR-V), vertices.bin (36 x {vec4 pos, vec4
 // colour}) and mvp.bin (mat4); the program writes color.bin (B8G8R8A8) and
-// depth.bin (D16) there.  Pipeline state as the sample: LESS_OR_EQUAL depth,
+// depth.bin (D16, or D24S8 in stencil mode) there; multipass and resubmit
+// write color_<submit>_<pass>.bin and depth_<submit>_<pass>.bin.  Pipeline state as the sample: LESS_OR_EQUAL depth,
 // FRONT_FACE_CLOCKWISE, cull NONE, clear colour 0.2, depth 1.0, dynamic
 // viewport (Samples/15-draw_cube/15-draw_cube.cpp:38-149,
 // Samples/utils/util_init.cpp:1541-1693).
@@ -70,7 +86,7 @@ template <class F> F load(const char* name)
 int main(int argc, char** argv)
 {
     if (argc != 5 && argc != 6) {
-        std::fprintf(stderr, "usage: icd_cube <icd.so> <dir> <width> <height> [draw|indirect|midclear]\n");
+        std::fprintf(stderr, "usage: icd_cube <icd.so> <dir> <width> <height> [mode]\n");
         return 2;
     }
     const std::string mode = argc == 6 ? argv[5] : "draw";
@@ -134,6 +150,14 @@ int main(int argc, char** argv)
     VFN(CmdSetDepthBounds, VkCommandBuffer, float, float);
     VFN(CmdDraw, VkCommandBuffer, uint32_t, uint32_t, uint32_t, uint32_t);
     VFN(CmdDrawIndirect, VkCommandBuffer, VkBuffer, VkDeviceSize, uint32_t, uint32_t);
+    VFN(CmdDrawIndexedIndirect, VkCommandBuffer, VkBuffer, VkDeviceSize, uint32_t, uint32_t);
+    VFN(CmdBindIndexBuffer, VkCommandBuffer, VkBuffer, VkDeviceSize, int32_t);
+    VFN(CmdSetStencilCompareMask, VkCommandBuffer, VkFlags, uint32_t);
+    VFN(CmdSetStencilWriteMask, VkCommandBuffer, VkFlags, uint32_t);
+    VFN(CmdSetStencilReference, VkCommandBuffer, VkFlags, uint32_t);
+    FN(GetFenceStatus, VkDevice, VkFence);
+    FN(ResetFences, VkDevice, uint32_t, const VkFence*);
+    FN(EnumerateDeviceExtensionProperties, VkPhysicalDevice, const char*, uint32_t*, VkExtensionProperties*);
     VFN(CmdClearAttachments, VkCommandBuffer, uint32_t, const VkClearAttachment*, uint32_t, const VkClearRect*);
     FN(QueueSubmit, VkQueue, uint32_t, const VkSubmitInfo*, VkFence);
     FN(CreateFence, VkDevice, const VkFenceCreateInfo*, const VkAllocationCallbacks*, VkFence*);
@@ -154,9 +178,17 @@ int main(int argc, char** argv)
         if (mp.memoryTypes[i].propertyFlags & VK_MEMORY_PROPERTY_HOST_VISIBLE_BIT) { host_type = i; break; }
     if (host_type == ~0u) return 4;
 
+    // the indirect-count draws are device extensions
+    uint32_t n_ext = 0;
+    CHECK(vkEnumerateDeviceExtensionProperties(phys, nullptr, &n_ext, nullptr));
+    std::vector<VkExtensionProperties> exts(n_ext);
+    CHECK(vkEnumerateDeviceExtensionProperties(phys, nullptr, &n_ext, exts.data()));
+    std::vector<const char*> ext_names;
+    for (const VkExtensionProperties& e : exts) ext_names.push_back(e.extensionName);
     const float prio = 1.0f;
     VkDeviceQueueCreateInfo qci{VK_STRUCTURE_TYPE_DEVICE_QUEUE_CREATE_INFO, nullptr, 0, 0, 1, &prio};
-    VkDeviceCreateInfo dci{VK_STRUCTURE_TYPE_DEVICE_CREATE_INFO, nullptr, 0, 1, &qci, 0, nullptr, 0, nullptr, nullptr};
+    VkDeviceCreateInfo dci{VK_STRUCTURE_TYPE_DEVICE_CREATE_INFO, nullptr, 0, 1, &qci, 0, nullptr, (uint32_t)ext_names.size(),
+                           ext_names.data(), nullptr};
     VkDevice dev;
     CHECK(vkCreateDevice(phys, &dci, nullptr, &dev));
     VkQueue queue;
@@ -166,8 +198,8 @@ int main(int argc, char** argv)
         VkMemoryAllocateInfo ai{VK_STRUCTURE_TYPE_MEMORY_ALLOCATE_INFO, nullptr, req.size, host_type};
         CHECK(vkAllocateMemory(dev, &ai, nullptr, mem));
     };
-    auto buffer = [&](const std::vector<char>& data, VkBuffer* buf) {
-        VkBufferCreateInfo bi{VK_STRUCTURE_TYPE_BUFFER_CREATE_INFO, nullptr, 0, data.size(), 0x90, 0, 0, nullptr};
+    auto buffer = [&](const std::vector<char>& data, VkBuffer* buf) -> void* {
+        VkBufferCreateInfo bi{VK_STRUCTURE_TYPE_BUFFER_CREATE_INFO, nullptr, 0, data.size(), 0x1D0, 0, 0, nullptr};
         CHECK(vkCreateBuffer(dev, &bi, nullptr, buf));
         VkMemoryRequirements req;
         vkGetBufferMemoryRequirements(dev, *buf, &req);
@@ -177,6 +209,7 @@ int main(int argc, char** argv)
         CHECK(vkMapMemory(dev, mem, 0, req.size, 0, &p));
         std::memcpy(p, data.data(), data.size());
         CHECK(vkBindBufferMemory(dev, *buf, mem, 0));
+        return p;
     };
     auto image = [&](int32_t format, VkImage* img, VkImageView* view, void** mapped) {
         VkImageCreateInfo ii{VK_STRUCTURE_TYPE_IMAGE_CREATE_INFO, nullptr, 0, VK_IMAGE_TYPE_2D, format, {W, H, 1}, 1, 1, 1,
@@ -192,14 +225,21 @@ int main(int argc, char** argv)
         CHECK(vkCreateImageView(dev, &vi, nullptr, view));
     };
 
+    const bool stencil = mode == "stencil";
+    const bool many = mode == "multipass" || mode == "resubmit";
+    const uint32_t passes = many ? 3 : 1, submits = mode == "resubmit" ? 3 : 1;
+    const int32_t ds_format = stencil ? CPV_FORMAT_D24_UNORM_S8_UINT : CPV_FORMAT_D16_UNORM;
+    const uint32_t ds_bytes = stencil ? 4 : 2;
     VkBuffer vbuf, ubuf;
     buffer(read_file(dir + "/vertices.bin"), &vbuf);
-    buffer(read_file(dir + "/mvp.bin"), &ubuf);
-    VkImage color, depth;
-    VkImageView color_view, depth_view;
-    void *color_map, *depth_map;
-    image(CPV_FORMAT_B8G8R8A8_UNORM, &color, &color_view, &color_map);
-    image(CPV_FORMAT_D16_UNORM, &depth, &depth_view, &depth_map);
+    void* mvp_map = buffer(read_file(dir + (mode == "resubmit" ? "/mvp0.bin" : "/mvp.bin")), &ubuf);
+    std::vector<VkImage> color(passes), depth(passes);
+    std::vector<VkImageView> color_view(passes), depth_view(passes);
+    std::vector<void*> color_map(passes), depth_map(passes);
+    for (uint32_t k = 0; k < passes; k++) {
+        image(CPV_FORMAT_B8G8R8A8_UNORM, &color[k], &color_view[k], &color_map[k]);
+        image(ds_format, &depth[k], &depth_view[k], &depth_map[k]);
+    }
 
     VkDescriptorSetLayoutBinding lb{0, VK_DESCRIPTOR_TYPE_UNIFORM_BUFFER, 1, VK_SHADER_STAGE_VERTEX_BIT, nullptr};
     VkDescriptorSetLayoutCreateInfo dli{VK_STRUCTURE_TYPE_DESCRIPTOR_SET_LAYOUT_CREATE_INFO, nullptr, 0, 1, &lb};
@@ -222,17 +262,19 @@ int main(int argc, char** argv)
 
     VkAttachmentDescription att[2] = {
         {0, CPV_FORMAT_B8G8R8A8_UNORM, 1, VK_ATTACHMENT_LOAD_OP_CLEAR, 0, 2, 1, 0, 2},
-        {0, CPV_FORMAT_D16_UNORM, 1, VK_ATTACHMENT_LOAD_OP_CLEAR, 1, 2, 1, 0, 3},
+        {0, ds_format, 1, VK_ATTACHMENT_LOAD_OP_CLEAR, 1, VK_ATTACHMENT_LOAD_OP_CLEAR, 1, 0, 3},
     };
     VkAttachmentReference cref{0, 2}, dref{1, 3};
     VkSubpassDescription sp{0, VK_PIPELINE_BIND_POINT_GRAPHICS, 0, nullptr, 1, &cref, nullptr, &dref, 0, nullptr};
     VkRenderPassCreateInfo rpi{VK_STRUCTURE_TYPE_RENDER_PASS_CREATE_INFO, nullptr, 0, 2, att, 1, &sp, 0, nullptr};
     VkRenderPass rp;
     CHECK(vkCreateRenderPass(dev, &rpi, nullptr, &rp));
-    VkImageView views[2] = {color_view, depth_view};
-    VkFramebufferCreateInfo fbi{VK_STRUCTURE_TYPE_FRAMEBUFFER_CREATE_INFO, nullptr, 0, rp, 2, views, W, H, 1};
-    VkFramebuffer fb;
-    CHECK(vkCreateFramebuffer(dev, &fbi, nullptr, &fb));
+    std::vector<VkFramebuffer> fbs(passes);
+    for (uint32_t k = 0; k < passes; k++) {
+        VkImageView views[2] = {color_view[k], depth_view[k]};
+        VkFramebufferCreateInfo fbi{VK_STRUCTURE_TYPE_FRAMEBUFFER_CREATE_INFO, nullptr, 0, rp, 2, views, W, H, 1};
+        CHECK(vkCreateFramebuffer(dev, &fbi, nullptr, &fbs[k]));
+    }
 
     std::vector<char> vs_code = read_file(dir + "/vs.spv"), fs_code = read_file(dir + "/fs.spv");
     VkShaderModuleCreateInfo smi{VK_STRUCTURE_TYPE_SHADER_MODULE_CREATE_INFO, nullptr, 0, vs_code.size(),
@@ -266,9 +308,20 @@ int main(int argc, char** argv)
         ias.topology = CPV_TOPOLOGY_LINE_LIST;
         rs.lineWidth = 3.0f;
     }
-    int32_t dyn[4] = {VK_DYNAMIC_STATE_VIEWPORT, VK_DYNAMIC_STATE_SCISSOR, VK_DYNAMIC_STATE_BLEND_CONSTANTS,
-                      VK_DYNAMIC_STATE_DEPTH_BOUNDS};
+    int32_t dyn[5] = {VK_DYNAMIC_STATE_VIEWPORT, VK_DYNAMIC_STATE_SCISSOR, VK_DYNAMIC_STATE_BLEND_CONSTANTS,
+                      VK_DYNAMIC_STATE_DEPTH_BOUNDS, 0};
     VkPipelineDynamicStateCreateInfo dys{VK_STRUCTURE_TYPE_PIPELINE_DYNAMIC_STATE_CREATE_INFO, nullptr, 0, 2, dyn};
+    int32_t dyn_stencil[5] = {VK_DYNAMIC_STATE_VIEWPORT, VK_DYNAMIC_STATE_SCISSOR, VK_DYNAMIC_STATE_STENCIL_COMPARE_MASK,
+                              VK_DYNAMIC_STATE_STENCIL_WRITE_MASK, VK_DYNAMIC_STATE_STENCIL_REFERENCE};
+    if (stencil) {
+        // EQUAL / REPLACE on both faces; the pipeline's masks and reference
+        // (all zero) are replaced by vkCmdSetStencil*
+        const VkStencilOpState st{CPV_STENCIL_KEEP, CPV_STENCIL_REPLACE, CPV_STENCIL_KEEP, CPV_COMPARE_EQUAL, 0, 0, 0};
+        dss.stencilTestEnable = 1;
+        dss.front = dss.back = st;
+        dys.pDynamicStates = dyn_stencil;
+        dys.dynamicStateCount = 5;
+    }
     if (mode == "dynamic") {
         // constant-colour blending and a depth-bounds test whose constants
         // and range come from vkCmdSetBlendConstants / vkCmdSetDepthBounds
@@ -289,12 +342,15 @@ int main(int argc, char** argv)
     VkCommandBufferAllocateInfo cai{VK_STRUCTURE_TYPE_COMMAND_BUFFER_ALLOCATE_INFO, nullptr, cpool, VK_COMMAND_BUFFER_LEVEL_PRIMARY, 1};
     VkCommandBuffer cmd;
     CHECK(vkAllocateCommandBuffers(dev, &cai, &cmd));
-    VkCommandBufferBeginInfo cbi{VK_STRUCTURE_TYPE_COMMAND_BUFFER_BEGIN_INFO, nullptr, 0, nullptr};
+    // a command buffer submitted more than once is recorded without ONE_TIME_SUBMIT
+    VkCommandBufferBeginInfo cbi{VK_STRUCTURE_TYPE_COMMAND_BUFFER_BEGIN_INFO, nullptr,
+                                 mode == "resubmit" ? 0u : (VkFlags)VK_COMMAND_BUFFER_USAGE_ONE_TIME_SUBMIT_BIT, nullptr};
     CHECK(vkBeginCommandBuffer(cmd, &cbi));
+    for (uint32_t pass = 0; pass < passes; pass++) {
     VkClearValue clears[2];
-    for (int i = 0; i < 4; i++) clears[0].color.float32[i] = 0.2f;
+    for (int i = 0; i < 4; i++) clears[0].color.float32[i] = 0.2f + 0.1f * (float)pass;
     clears[1].depthStencil = VkClearDepthStencilValue{1.0f, 0};
-    VkRenderPassBeginInfo rbi{VK_STRUCTURE_TYPE_RENDER_PASS_BEGIN_INFO, nullptr, rp, fb, {{0, 0}, {W, H}}, 2, clears};
+    VkRenderPassBeginInfo rbi{VK_STRUCTURE_TYPE_RENDER_PASS_BEGIN_INFO, nullptr, rp, fbs[pass], {{0, 0}, {W, H}}, 2, clears};
     vkCmdBeginRenderPass(cmd, &rbi, VK_SUBPASS_CONTENTS_INLINE);
     vkCmdBindPipeline(cmd, VK_PIPELINE_BIND_POINT_GRAPHICS, pipe);
     vkCmdBindDescriptorSets(cmd, VK_PIPELINE_BIND_POINT_GRAPHICS, layout, 0, 1, &set, 0, nullptr);
@@ -305,11 +361,54 @@ int main(int argc, char** argv)
     const VkDeviceSize off = 0;
     vkCmdBindVertexBuffers(cmd, 0, 1, &vbuf, &off);
     VkBuffer ibuf = nullptr;
+    auto bytes_of = [](const void* p, size_t n) { return std::vector<char>((const char*)p, (const char*)p + n); };
     if (mode == "indirect") {
         const VkDrawIndirectCommand recs[2] = {{18, 1, 0, 0}, {18, 1, 18, 0}};
-        std::vector<char> bytes((const char*)recs, (const char*)recs + sizeof(recs));
-        buffer(bytes, &ibuf);
+        buffer(bytes_of(recs, sizeof(recs)), &ibuf);
         vkCmdDrawIndirect(cmd, ibuf, 0, 2, sizeof(VkDrawIndirectCommand));
+    } else if (mode == "indexed_indirect" || mode == "indexed_indirect_count") {
+        uint16_t idx[36];
+        for (int i = 0; i < 36; i++) idx[i] = (uint16_t)i;
+        VkBuffer index_buf;
+        buffer(bytes_of(idx, sizeof(idx)), &index_buf);
+        vkCmdBindIndexBuffer(cmd, index_buf, 0, VK_INDEX_TYPE_UINT16);
+        // the third record (all 36 indices, vertexOffset 1) is past the count
+        const VkDrawIndexedIndirectCommand recs[3] = {{18, 1, 0, 0, 0}, {18, 1, 18, 0, 0}, {36, 1, 0, 1, 0}};
+        buffer(bytes_of(recs, sizeof(recs)), &ibuf);
+        if (mode == "indexed_indirect") {
+            vkCmdDrawIndexedIndirect(cmd, ibuf, 0, 2, sizeof(VkDrawIndexedIndirectCommand));
+        } else {
+            const uint32_t count = 2;
+            VkBuffer cbuf;
+            buffer(bytes_of(&count, 4), &cbuf);
+            auto draw = load<void (*)(VkCommandBuffer, VkBuffer, VkDeviceSize, VkBuffer, VkDeviceSize, uint32_t, uint32_t)>(
+                "vkCmdDrawIndexedIndirectCountAMD");
+            draw(cmd, ibuf, 0, cbuf, 0, 3, sizeof(VkDrawIndexedIndirectCommand));
+        }
+    } else if (mode == "indirect_count") {
+        // records 24 bytes apart, with a third one past the count
+        struct Rec {
+            VkDrawIndirectCommand c;
+            uint32_t pad[2];
+        };
+        const Rec recs[3] = {{{18, 1, 0, 0}, {7, 7}}, {{18, 1, 18, 0}, {7, 7}}, {{36, 2, 0, 0}, {7, 7}}};
+        const uint32_t count[2] = {0xFFFFFFFFu, 2};   // read at offset 4
+        buffer(bytes_of(recs, sizeof(recs)), &ibuf);
+        VkBuffer cbuf;
+        buffer(bytes_of(count, sizeof(count)), &cbuf);
+        auto draw = load<void (*)(VkCommandBuffer, VkBuffer, VkDeviceSize, VkBuffer, VkDeviceSize, uint32_t, uint32_t)>(
+            "vkCmdDrawIndirectCountKHR");
+        draw(cmd, ibuf, 0, cbuf, 4, 3, sizeof(Rec));
+    } else if (stencil) {
+        vkCmdSetStencilCompareMask(cmd, VK_STENCIL_FACE_FRONT_AND_BACK, 0x00);
+        vkCmdSetStencilWriteMask(cmd, VK_STENCIL_FACE_FRONT_AND_BACK, 0xF0);
+        vkCmdSetStencilReference(cmd, VK_STENCIL_FACE_FRONT_AND_BACK, 0xA5);
+        vkCmdDraw(cmd, 36, 1, 0, 0);
+        vkCmdSetStencilCompareMask(cmd, VK_STENCIL_FACE_FRONT_AND_BACK, 0xF0);
+        vkCmdSetStencilWriteMask(cmd, VK_STENCIL_FACE_FRONT_AND_BACK, 0x0F);
+        vkCmdSetStencilReference(cmd, VK_STENCIL_FACE_FRONT_BIT, 0xAF);
+        vkCmdSetStencilReference(cmd, VK_STENCIL_FACE_BACK_BIT, 0xA3);
+        vkCmdDraw(cmd, 36, 1, 0, 0);
     } else if (mode == "dynamic") {
         const float k1[4] = {0.25f, 0.5f, 0.75f, 1.0f}, k2[4] = {0.8f, 0.1f, 0.3f, 0.5f};
         vkCmdSetBlendConstants(cmd, k1);
@@ -337,17 +436,43 @@ int main(int argc, char** argv)
         vkCmdDraw(cmd, 36, 1, 0, 0);
     }
     vkCmdEndRenderPass(cmd);
+    }
     CHECK(vkEndCommandBuffer(cmd));
 
     VkFenceCreateInfo fci{VK_STRUCTURE_TYPE_FENCE_CREATE_INFO, nullptr, 0};
     VkFence fence;
     CHECK(vkCreateFence(dev, &fci, nullptr, &fence));
     VkSubmitInfo si{VK_STRUCTURE_TYPE_SUBMIT_INFO, nullptr, 0, nullptr, nullptr, 1, &cmd, 0, nullptr};
-    CHECK(vkQueueSubmit(queue, 1, &si, fence));
-    CHECK(vkWaitForFences(dev, 1, &fence, VK_TRUE, ~0ull));
-
-    std::ofstream(dir + "/color.bin", std::ios::binary).write((const char*)color_map, (std::streamsize)W * H * 4);
-    std::ofstream(dir + "/depth.bin", std::ios::binary).write((const char*)depth_map, (std::streamsize)W * H * 2);
+    if (mode == "resubmit") {
+        // a fence never submitted: not ready, and a timed wait times out;
+        // with a signalled fence beside it, waiting for any one succeeds
+        VkFenceCreateInfo on{VK_STRUCTURE_TYPE_FENCE_CREATE_INFO, nullptr, VK_FENCE_CREATE_SIGNALED_BIT};
+        VkFence lit;
+        CHECK(vkCreateFence(dev, &on, nullptr, &lit));
+        const VkFence both[2] = {fence, lit};
+        std::printf("fence unsubmitted status: %d\n", vkGetFenceStatus(dev, fence));
+        std::printf("fence unsubmitted wait 1ms: %d\n", vkWaitForFences(dev, 1, &fence, VK_TRUE, 1000000ull));
+        std::printf("fence any of two: %d\n", vkWaitForFences(dev, 2, both, VK_FALSE, 0));
+        std::printf("fence all of two: %d\n", vkWaitForFences(dev, 2, both, VK_TRUE, 0));
+    }
+    for (uint32_t sub = 0; sub < submits; sub++) {
+        if (sub > 0) {
+            // the previous submit has finished (its fence waited on): rewrite the uniform
+            std::vector<char> m = read_file(dir + "/mvp" + std::to_string(sub) + ".bin");
+            std::memcpy(mvp_map, m.data(), m.size());
+            CHECK(vkResetFences(dev, 1, &fence));
+            std::printf("fence after reset: %d\n", vkGetFenceStatus(dev, fence));
+        }
+        CHECK(vkQueueSubmit(queue, 1, &si, fence));
+        CHECK(vkWaitForFences(dev, 1, &fence, VK_TRUE, ~0ull));
+        if (mode == "resubmit") std::printf("fence after wait: %d\n", vkGetFenceStatus(dev, fence));
+        for (uint32_t k = 0; k < passes; k++) {
+            const std::string tag = many ? "_" + std::to_string(sub) + "_" + std::to_string(k) : "";
+            std::ofstream(dir + "/color" + tag + ".bin", std::ios::binary).write((const char*)color_map[k], (std::streamsize)W * H * 4);
+            std::ofstream(dir + "/depth" + tag + ".bin", std::ios::binary)
+                .write((const char*)depth_map[k], (std::streamsize)W * H * ds_bytes);
+        }
+    }
     vkDestroyDevice(dev, nullptr);
     vkDestroyInstance(g_inst, nullptr);
     std::printf("rendered %ux%u\n", W, H);

@@ -5,6 +5,8 @@ uniform-buffer descriptor set, vkCmdDraw recorded into a command buffer and
 replayed by vkQueueSubmit -- and its colour and depth images must equal the
 CPU oracle's bytes."""
 import copy
+import os
+import re
 import subprocess
 from pathlib import Path
 
@@ -113,3 +115,117 @@ def test_clear_attachments_inside_a_render_pass(tmp_path):
     c2, d2, _ = oracle_lib.render_scene(again)
     assert np.array_equal(color, c2), f"colour differs in {np.count_nonzero(color != c2)} bytes"
     assert np.array_equal(depth, d2), f"depth differs in {np.count_nonzero(depth != d2)} bytes"
+
+
+@pytest.mark.parametrize("mode", ["indexed_indirect", "indirect_count", "indexed_indirect_count"])
+def test_indexed_and_counted_indirect_through_the_icd(tmp_path, mode):
+    """vkCmdDrawIndexedIndirect, vkCmdDrawIndirectCountKHR and
+    vkCmdDrawIndexedIndirectCountAMD split the cube's 36 vertices into two
+    records; the counted forms read a count of 2 with maxDrawCount 3, so
+    their third record (which would draw more) is ignored: the frame is
+    vkCmdDraw(36)'s."""
+    color, depth = _run_cube(tmp_path, 200, mode)
+    ref_color, ref_ds, _ = oracle_lib.render_scene(W.cube_scene(width=200, height=200))
+    assert np.array_equal(color, ref_color), f"colour differs in {np.count_nonzero(color != ref_color)} bytes"
+    assert np.array_equal(depth, ref_ds), f"depth differs in {np.count_nonzero(depth != ref_ds)} bytes"
+
+
+def test_dynamic_stencil_masks_and_references(tmp_path):
+    """vkCmdSetStencilCompareMask / WriteMask / Reference between two draws of
+    a pipeline with EQUAL / REPLACE stencil on a D24_UNORM_S8_UINT
+    attachment (separate front and back references in the second draw): the
+    oracle draws the same with two pipelines holding those values
+    statically (CommandBuffer.h:40-42)."""
+    size = 180
+    color, depth = _run_cube(tmp_path, size, "stencil")
+    scene = W.cube_scene(width=size, height=size)
+    scene.ds_format = abi.FORMAT["D24_UNORM_S8_UINT"]
+    S_ = abi.STENCIL
+    pipes = []
+    for (cmask, wmask, ref_front, ref_back) in ((0x00, 0xF0, 0xA5, 0xA5), (0xF0, 0x0F, 0xAF, 0xA3)):
+        p = dict(scene.pipeline, stencil_test=1,
+                 front=(S_["KEEP"], S_["REPLACE"], S_["KEEP"], abi.COMPARE["EQUAL"], cmask, wmask, ref_front),
+                 back=(S_["KEEP"], S_["REPLACE"], S_["KEEP"], abi.COMPARE["EQUAL"], cmask, wmask, ref_back))
+        pipes.append(p)
+    scene.pipelines = pipes
+    scene.draws = [scene.draws[0], copy.deepcopy(scene.draws[0])]
+    scene.draw_pipeline = [0, 1]
+    ref_color, ref_ds, _ = oracle_lib.render_scene(scene)
+    st = ref_ds.reshape(-1, 4)[:, 3]
+    assert 0xAF in st and 0 in st, "the second draw's front faces replace the stencil inside the cube"
+    assert np.array_equal(color, ref_color), f"colour differs in {np.count_nonzero(color != ref_color)} bytes"
+    assert np.array_equal(depth, ref_ds), f"depth/stencil differs in {np.count_nonzero(depth != ref_ds)} bytes"
+
+
+def _mvps(size):
+    """The sample's MVP, then the same turned about z by 0.3 and 0.7 radians."""
+    base = W.cube_mvp(size, size).reshape(4, 4)
+    out = [base]
+    for a in (0.3, 0.7):
+        c, s = np.float32(np.cos(a)), np.float32(np.sin(a))
+        rot = np.eye(4, dtype=np.float32)
+        rot[0, 0], rot[0, 1], rot[1, 0], rot[1, 1] = c, -s, s, c
+        out.append((base @ rot).astype(np.float32))
+    return out
+
+
+def _cube_pass(size, clear, mvp):
+    s = W.cube_scene(width=size, height=size)
+    s.clear_color = (clear,) * 4
+    s.draws[0].uniforms = {0: np.ascontiguousarray(mvp, np.float32).reshape(-1).view(np.uint8)}
+    return s
+
+
+def _run_traced(tmp_path, size, mode, mvps):
+    np.asarray(S.vs_mvp_color(), np.uint32).tofile(tmp_path / "vs.spv")
+    np.asarray(S.fs_color(), np.uint32).tofile(tmp_path / "fs.spv")
+    W.cube_vertices().tofile(tmp_path / "vertices.bin")
+    W.cube_mvp(size, size).tofile(tmp_path / "mvp.bin")
+    for j, m in enumerate(mvps):
+        np.ascontiguousarray(m, np.float32).tofile(tmp_path / f"mvp{j}.bin")
+    env = dict(os.environ, CPVULKAN_AMD_TRACE_SUBMITS="1")
+    print(f"[icd] running {APP.name} {size}x{size} {mode}", flush=True)
+    res = subprocess.run([str(APP), str(ICD), str(tmp_path), str(size), str(size), mode], capture_output=True, text=True,
+                         timeout=240, env=env)
+    print(res.stdout + res.stderr, flush=True)
+    assert res.returncode == 0
+    return res
+
+
+def _check_passes(tmp_path, size, sub, mvp):
+    for k in range(3):
+        clear = float(np.float32(0.2) + np.float32(0.1) * np.float32(k))
+        ref_color, ref_ds, _ = oracle_lib.render_scene(_cube_pass(size, clear, mvp))
+        color = np.fromfile(tmp_path / f"color_{sub}_{k}.bin", np.uint8)
+        depth = np.fromfile(tmp_path / f"depth_{sub}_{k}.bin", np.uint8)
+        assert np.array_equal(color, ref_color), f"submit {sub} pass {k}: colour differs in {np.count_nonzero(color != ref_color)} bytes"
+        assert np.array_equal(depth, ref_ds), f"submit {sub} pass {k}: depth differs"
+
+
+def test_three_render_passes_one_launch(tmp_path):
+    """Three render passes of one command buffer, each into its own
+    framebuffer with its own clear colour, go to the GPU as one launch."""
+    size = 160
+    res = _run_traced(tmp_path, size, "multipass", [])
+    assert re.findall(r"submit (\d+): (\d+) launches", res.stderr) == [("1", "1")]
+    _check_passes(tmp_path, size, 0, W.cube_mvp(size, size).reshape(4, 4))
+
+
+def test_resubmitted_command_buffer_replays_its_batch(tmp_path):
+    """A command buffer recorded without ONE_TIME_SUBMIT, submitted three
+    times with the uniform rewritten in between: every submit is one launch
+    of the recorded batch and draws the frame of its own MVP; the fence
+    reports VK_NOT_READY before any submit and after a reset, VK_TIMEOUT for
+    a timed wait that nothing will signal, and waits for any or all of two
+    fences (Fence.cpp:47-56, 157-200)."""
+    size = 144
+    mvps = _mvps(size)
+    res = _run_traced(tmp_path, size, "resubmit", mvps)
+    assert re.findall(r"submit (\d+): (\d+) launches", res.stderr) == [("1", "1"), ("2", "1"), ("3", "1")]
+    fence = dict(re.findall(r"fence ([a-z0-9 ]+): (-?\d+)", res.stdout))
+    assert fence["unsubmitted status"] == "1"        # VK_NOT_READY
+    assert fence["unsubmitted wait 1ms"] == "2"      # VK_TIMEOUT
+    assert fence["any of two"] == "0" and fence["all of two"] == "2"
+    assert fence["after wait"] == "0" and fence["after reset"] == "1"
+    for j, m in enumerate(mvps):
+        _check_passes(tmp_path, size, j, m)
