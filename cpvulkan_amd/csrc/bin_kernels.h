@@ -691,14 +691,20 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint32_t* keys, 
 // ---------------------------------------------------------- radix sort
 
 // A pass sorts by an 8-bit digit.  The pair array is cut into kSortSegments
-// contiguous segments, one per wavefront; the histogram kernel counts each
+// contiguous segments, one per workgroup; the histogram kernel counts each
 // segment's digits, a scan over (digit, segment) turns the counts into
-// output bases, and the scatter kernel walks each segment in order, ranking
-// equal digits inside every 64-item round with ballots so the pass is
-// stable.
-constexpr int kSortSegments = 4096;
-constexpr int kSortWaves = 4;
-constexpr int kSortBatch = 8;   // 64-item rounds whose loads are issued together
+// output bases, and the scatter kernel walks its segment in order in tiles
+// of kSortTile items.  Each wave ranks its quarter of a tile (64-item
+// rounds, equal digits ranked with ballots), the workgroup orders the tile
+// by digit in LDS -- stable: by wave, then round, then lane, which is the
+// items' order -- and the sorted tile is written out with consecutive
+// threads on consecutive positions of each digit's run, so the stores of a
+// run coalesce instead of scattering one item per digit bucket.
+constexpr int kSortSegments = 1024;
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortRounds = 8;                              // 64-item rounds per wave and tile
+constexpr int kSortTile = kSortThreads * kSortRounds;       // 2048 items
 
 __device__ __forceinline__ void segment_range(uint32_t n, uint32_t seg, uint32_t& b, uint32_t& e)
 {
@@ -707,63 +713,69 @@ __device__ __forceinline__ void segment_range(uint32_t n, uint32_t seg, uint32_t
     e = min(n, b + per);
 }
 
-__global__ __launch_bounds__(64 * kSortWaves) void radix_hist_kernel(const uint32_t* keys, const uint32_t* n_ptr, int shift,
-                                                                     uint32_t* hist)
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* keys, const uint32_t* n_ptr, int shift,
+                                                                 uint32_t* hist)
 {
     __shared__ uint32_t s_cnt[kSortWaves][256];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int i = lane; i < 256; i += 64) s_cnt[wave][i] = 0;
-    __syncthreads();
-    const uint32_t seg = blockIdx.x * kSortWaves + wave;
-    uint32_t b, e;
-    segment_range(*n_ptr, seg, b, e);
-    // kSortBatch rounds of keys loaded before they are counted, so the loads overlap
-    for (uint32_t i0 = b; i0 < e; i0 += 64u * kSortBatch) {
-        uint32_t ks[kSortBatch];
+    const int wave = threadIdx.x >> 6;
 #pragma unroll
-        for (int r = 0; r < kSortBatch; r++) {
-            const uint32_t i = i0 + (uint32_t)r * 64u + (uint32_t)lane;
+    for (int w = 0; w < kSortWaves; w++) s_cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t b, e;
+    segment_range(*n_ptr, blockIdx.x, b, e);
+    // kSortRounds keys per thread loaded before they are counted, so the loads overlap
+    for (uint32_t i0 = b; i0 < e; i0 += (uint32_t)kSortTile) {
+        uint32_t ks[kSortRounds];
+#pragma unroll
+        for (int r = 0; r < kSortRounds; r++) {
+            const uint32_t i = i0 + (uint32_t)r * kSortThreads + threadIdx.x;
             ks[r] = i < e ? keys[i] : 0u;
         }
 #pragma unroll
-        for (int r = 0; r < kSortBatch; r++)
-            if (i0 + (uint32_t)r * 64u + (uint32_t)lane < e) atomicAdd(&s_cnt[wave][(ks[r] >> shift) & 255u], 1u);
+        for (int r = 0; r < kSortRounds; r++)
+            if (i0 + (uint32_t)r * kSortThreads + threadIdx.x < e) atomicAdd(&s_cnt[wave][(ks[r] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    for (int i = lane; i < 256; i += 64) hist[(uint32_t)i * kSortSegments + seg] = s_cnt[wave][i];
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; w++) c += s_cnt[w][threadIdx.x];
+    hist[threadIdx.x * kSortSegments + blockIdx.x] = c;
 }
 
-__global__ __launch_bounds__(64 * kSortWaves) void radix_scatter_kernel(const uint32_t* keys_in, const uint32_t* vals_in,
-                                                                        uint32_t* keys_out, uint32_t* vals_out,
-                                                                        const uint32_t* n_ptr, int shift,
-                                                                        const uint32_t* base)
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint32_t* keys_in, const uint32_t* vals_in,
+                                                                    uint32_t* keys_out, uint32_t* vals_out,
+                                                                    const uint32_t* n_ptr, int shift, const uint32_t* base)
 {
-    __shared__ uint32_t s_run[kSortWaves][256];
+    __shared__ uint32_t s_cnt[kSortWaves][256];   // a wave's items per digit, then the wave's offset in the digit's run
+    __shared__ uint32_t s_start[256];             // where a digit's run starts in the sorted tile
+    __shared__ uint32_t s_run[256];               // the segment's next output position per digit
+    __shared__ uint32_t s_keys[kSortTile], s_vals[kSortTile];
+    __shared__ uint64_t s_tmp[kSortWaves];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t seg = blockIdx.x * kSortWaves + wave;
-    for (int i = lane; i < 256; i += 64) s_run[wave][i] = base[(uint32_t)i * kSortSegments + seg];
-    __syncthreads();
+    const uint32_t d_me = threadIdx.x;            // the digit this thread keeps the books of
+    s_run[d_me] = base[d_me * kSortSegments + blockIdx.x];
     uint32_t b, e;
-    const uint32_t e_total = *n_ptr;
-    segment_range(e_total, seg, b, e);
+    const uint32_t n_total = *n_ptr;
+    segment_range(n_total, blockIdx.x, b, e);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint32_t i0 = b; i0 < e; i0 += 64u * kSortBatch) {
-        // kSortBatch rounds loaded before they are ranked, so the loads overlap
-        uint32_t ks[kSortBatch], vs[kSortBatch];
+    for (uint32_t t0 = b; t0 < e; t0 += (uint32_t)kSortTile) {
 #pragma unroll
-        for (int r = 0; r < kSortBatch; r++) {
-            const uint32_t i = i0 + (uint32_t)r * 64u + (uint32_t)lane;
+        for (int w = 0; w < kSortWaves; w++) s_cnt[w][d_me] = 0;
+        __syncthreads();
+        // this wave's quarter of the tile: kSortRounds rounds of 64 consecutive items
+        const uint32_t w0 = t0 + (uint32_t)wave * (64u * kSortRounds);
+        uint32_t ks[kSortRounds], vs[kSortRounds], rk[kSortRounds];
+#pragma unroll
+        for (int r = 0; r < kSortRounds; r++) {
+            const uint32_t i = w0 + (uint32_t)r * 64u + (uint32_t)lane;
             ks[r] = i < e ? keys_in[i] : 0u;
             vs[r] = i < e ? vals_in[i] : 0u;
         }
 #pragma unroll
-        for (int r = 0; r < kSortBatch; r++) {
-            const uint32_t i = i0 + (uint32_t)r * 64u + (uint32_t)lane;
-            const bool valid = i < e;
-            const uint32_t key = ks[r];
-            const uint32_t val = vs[r];
-            const uint32_t dg = (key >> shift) & 255u;
-            // lanes holding the same digit
+        for (int r = 0; r < kSortRounds; r++) {
+            const bool valid = w0 + (uint32_t)r * 64u + (uint32_t)lane < e;
+            const uint32_t dg = (ks[r] >> shift) & 255u;
+            // lanes of this round holding the same digit
             uint64_t match = __ballot(valid);
 #pragma unroll
             for (int bit = 0; bit < 8; bit++) {
@@ -771,18 +783,54 @@ __global__ __launch_bounds__(64 * kSortWaves) void radix_scatter_kernel(const ui
                 const uint64_t bal = __ballot(valid && set);
                 match &= set ? bal : ~bal;
             }
-            const uint32_t rank = (uint32_t)__popcll(match & lt);
             const int leader = __ffsll((long long)match) - 1;
             uint32_t run = 0;
-            if (valid) run = s_run[wave][dg];
+            if (valid) run = s_cnt[wave][dg];
             __builtin_amdgcn_wave_barrier();
-            if (valid && lane == leader) s_run[wave][dg] = run + (uint32_t)__popcll(match);
+            if (valid && lane == leader) s_cnt[wave][dg] = run + (uint32_t)__popcll(match);
             __builtin_amdgcn_wave_barrier();
-            if (valid && run + rank < e_total) {
-                keys_out[run + rank] = key;
-                vals_out[run + rank] = val;
+            rk[r] = run + (uint32_t)__popcll(match & lt);
+        }
+        __syncthreads();
+        // per digit: the waves' offsets within the digit's run, and its total
+        uint32_t total = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; w++) {
+            const uint32_t c = s_cnt[w][d_me];
+            s_cnt[w][d_me] = total;
+            total += c;
+        }
+        uint64_t tile_n;
+        const uint32_t start = (uint32_t)block_exclusive_scan64(total, s_tmp, tile_n);
+        s_start[d_me] = start;
+        __syncthreads();
+        // the tile in digit order, in LDS
+#pragma unroll
+        for (int r = 0; r < kSortRounds; r++) {
+            if (w0 + (uint32_t)r * 64u + (uint32_t)lane < e) {
+                const uint32_t dg = (ks[r] >> shift) & 255u;
+                const uint32_t pos = s_start[dg] + s_cnt[wave][dg] + rk[r];
+                s_keys[pos] = ks[r];
+                s_vals[pos] = vs[r];
             }
         }
+        __syncthreads();
+        // consecutive threads write consecutive positions of a digit's run
+#pragma unroll
+        for (int r = 0; r < kSortRounds; r++) {
+            const uint32_t p = (uint32_t)r * kSortThreads + threadIdx.x;
+            if (p < (uint32_t)tile_n) {
+                const uint32_t key = s_keys[p];
+                const uint32_t dg = (key >> shift) & 255u;
+                const uint32_t dst = s_run[dg] + (p - s_start[dg]);
+                if (dst < n_total) {
+                    keys_out[dst] = key;
+                    vals_out[dst] = s_vals[p];
+                }
+            }
+        }
+        __syncthreads();
+        s_run[d_me] += total;
     }
 }
 

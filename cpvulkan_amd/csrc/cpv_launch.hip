@@ -273,11 +273,11 @@ int radix_sort(Exec& x, DevBuf* keys, DevBuf* vals, const uint32_t* d_n, int bit
     Buffers& b = *x.b;
     int src = 0;
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(kSortSegments / kSortWaves), dim3(64 * kSortWaves), 0, x.s,
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(kSortSegments), dim3(kSortThreads), 0, x.s,
                            (const uint32_t*)keys[src].p, d_n, shift, b.hist.as<uint32_t>());
         int rc = scan_u32(x, b.hist.as<uint32_t>(), 256u * kSortSegments, b.hist_base.as<uint32_t>(), false);
         if (rc) return rc;
-        hipLaunchKernelGGL(radix_scatter_kernel, dim3(kSortSegments / kSortWaves), dim3(64 * kSortWaves), 0, x.s,
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(kSortSegments), dim3(kSortThreads), 0, x.s,
                            (const uint32_t*)keys[src].p, (const uint32_t*)vals[src].p, keys[1 - src].as<uint32_t>(),
                            vals[1 - src].as<uint32_t>(), d_n, shift, (const uint32_t*)b.hist_base.p);
         HIP_TRY(hipGetLastError());
