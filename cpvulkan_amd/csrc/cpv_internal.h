@@ -75,6 +75,12 @@ struct MlpDevice;   // the tile-cost predictor's device state (cpv_mlp.hip)
 // earlier launches, and the next call on the context redoes it with a larger
 // one if its pairs did not fit (its raster kernel then drew nothing).
 struct PendingLaunch;
+
+// A byte range of device memory [lo, hi).
+struct Span {
+    uint64_t lo = 0, hi = 0;
+    bool meets(const Span& o) const { return lo < o.hi && o.lo < hi; }
+};
 void destroy_pending(PendingLaunch* p);
 int resolve_pending(cpv_context* c, bool may_allocate = true);
 // Records a recorded batch's submit again after it outgrew its pair arrays
@@ -96,11 +102,22 @@ struct cpv_context {
     int device = 0;
     int cu_count = 0;
     hipStream_t stream = nullptr;
-    hipStream_t front_stream = nullptr;   // front halves of recorded batches (created with the first batch)
+    hipStream_t front_stream = nullptr;   // front halves (vertex, setup, binning) of launches and batch submits
     uint64_t enqueue_seq = 0;             // calls that enqueued work on the context (cpvrt::note_enqueue)
     hipEvent_t ev_begin = nullptr, ev_raster0 = nullptr, ev_raster1 = nullptr, ev_end = nullptr;
     cpv_config cfg{};
-    cpvrt::Buffers buf;
+    cpvrt::Buffers buf;                   // the render calls' control words (fragment counter)
+    // Working buffers of immediate launches, used alternately: a launch's
+    // front half on front_stream waits only for the raster kernel that last
+    // used its set, so it runs under the previous launch's raster kernel.
+    cpvrt::Buffers lbuf[2];
+    hipEvent_t set_front_done[2] = {}, set_raster_done[2] = {};
+    bool set_used[2] = {false, false};
+    uint32_t next_set = 0, last_set = 0;
+    uint64_t chain_seq = ~0ull;           // enqueue_seq at the last launch: equal means nothing came in between
+    hipEvent_t ev_pre = nullptr;          // joins front_stream to the context stream's earlier work
+    bool call_first_launch = false;       // the render call has not launched yet (its begin event is pending)
+    std::vector<cpvrt::Span> last_written;   // attachment bytes the last immediate launch draws into
     // Pinned staging of host-to-device uploads (launch descriptors,
     // cpv_update_buffer data): a ring of slots, each written again only once
     // the copy issued from it has completed, so an upload does not wait for
@@ -158,7 +175,7 @@ void fill_u32(hipStream_t s, void* p, uint32_t v, uint64_t n);
 // stream through the next pinned staging slot; src is free on return.
 // Without may_grow a slot too small for the bytes is an error, not an
 // allocation.
-int stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes, bool may_grow = true);
+int stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes, bool may_grow = true, hipStream_t s = nullptr);
 // Counts a call that enqueued work on the context's stream: a recorded
 // batch overlaps a submit with its previous one only when nothing else was
 // enqueued in between.

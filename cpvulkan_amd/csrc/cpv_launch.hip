@@ -9,12 +9,15 @@
 // initialisation, so a frame touches each framebuffer byte once on the way
 // in (LOAD only) and once on the way out.
 //
-// The sequence is split at the only point where the host needs a device
-// value -- the number of (tile, primitive) pairs, which sizes the sort -- so
-// that a recorded batch (cpv_batch_*) can capture the whole frame into one
-// hipGraph: its pair arrays are sized once at record time, and the emit
-// kernels drop pairs past that capacity and raise an overflow word instead
-// of writing out of bounds.
+// The one device value the host needs -- the number of (tile, primitive)
+// pairs, which sizes the sort -- is never waited for before the raster
+// kernel: the pair arrays have a capacity taken from earlier launches (or,
+// in a recorded batch, from the recording), the emit kernels drop pairs past
+// it and raise an overflow word instead of writing out of bounds, the raster
+// kernels that see the word draw nothing, and the next call on the context
+// checks the count and redraws what did not fit.  Everything before the
+// raster kernel (the front half) runs on the context's second stream, so a
+// launch's front half overlaps the previous launch's raster kernel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -244,6 +247,7 @@ struct Exec {
     bool frozen = false;          // recording a graph: no allocation
     hipEvent_t r0 = nullptr, r1 = nullptr;
     uint32_t* overflow = nullptr; // overflow word shared by several launches (nullptr: the launch's own)
+    unsigned long long* fragments = nullptr;   // covered-fragment counter (nullptr: the launch's own)
 };
 
 // Exclusive scan of n counts into out (out[n] = the total when write_total);
@@ -478,7 +482,7 @@ int enqueue_raster(Exec& x, const LaunchPlan& P, uint8_t* arena, const Binned& b
     ra.line_ext = P.lines ? b.line_ext.as<LineExt>() : nullptr;
     ra.order = order;
     ra.queue = stats + kStatQueues;
-    ra.fragments = (unsigned long long*)(stats + kStatFragments);
+    ra.fragments = x.fragments ? x.fragments : (unsigned long long*)(stats + kStatFragments);
     ra.overflow = x.overflow ? x.overflow : stats + kStatOverflow;
     ra.trace_cycles = x.cfg.record_costs ? b.traces.as<float>() : nullptr;
     ra.vout_stride = P.stride;
@@ -536,6 +540,7 @@ struct cpvrt::PendingLaunch {
     LaunchPlan P;
     Staging st;
     cpv_config cfg{};
+    uint32_t set = 0;            // the context's buffer set it runs in
     bool in_scratch = false;     // ran in the scratch array (cpv_render_pass_noalloc)
     uint32_t scratch_cap = 0;    // its pair capacity there
 };
@@ -544,31 +549,64 @@ void cpvrt::destroy_pending(PendingLaunch* p) { delete p; }
 
 namespace {
 
-// Uploads a launch's descriptors and enqueues all of it with the context's
-// pair capacity (at least 16 pairs per primitive and 1M), then the
-// read-back of its 64-bit pair total and overflow word, which the next call
-// on the context checks (resolve_pending): no host round trip sits between
-// the launch's binning and its raster kernel.
+// Enqueues an immediate launch in its buffer set with the context's pair
+// capacity (at least 16 pairs per primitive and 1M): the descriptors'
+// upload, the front half (vertex, setup, binning) and the read-back of the
+// 64-bit pair total and overflow word on front_stream, then the raster
+// kernel on the context's stream after it.  The next call on the context
+// checks the pair count (resolve_pending): no host round trip sits between
+// a launch's binning and its raster kernel.  The front half waits for the
+// raster kernel that last used the set and, unless the previous context
+// call was a launch too, for all work enqueued on the context's stream; so
+// back-to-back launches run one front half under the previous raster.
 int enqueue_checked(cpv_context* c, PendingLaunch& L)
 {
-    int rc = upload_staging(c, c->buf.arena, L.st);
+    Buffers& B = c->lbuf[L.set];
+    hipStream_t fs = c->front_stream;
+    const bool chained = c->chain_seq == c->enqueue_seq;
+    if (!chained) {
+        HIP_TRY(hipEventRecord(c->ev_pre, c->stream));
+        HIP_TRY(hipStreamWaitEvent(fs, c->ev_pre, 0));
+    }
+    if (c->set_used[L.set]) HIP_TRY(hipStreamWaitEvent(fs, c->set_raster_done[L.set], 0));
+    if (c->call_first_launch) {
+        HIP_TRY(hipEventRecord(c->ev_begin, fs));
+        c->call_first_launch = false;
+    }
+    if (B.arena.ensure(L.st.bytes.size())) return CPV_ERROR_OUT_OF_MEMORY;
+    uint8_t* arena = B.arena.as<uint8_t>();
+    patch_plan(L.P, L.st, arena);
+    int rc = stage_upload(c, arena, L.st.bytes.data(), L.st.bytes.size(), true, fs);
     if (rc) return rc;
-    uint8_t* arena = c->buf.arena.as<uint8_t>();
-    Exec x = context_exec(c, &c->buf);
+    const bool fresh = B.stats.p == nullptr;
+    if (B.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+    if (fresh) fill_u32(fs, B.stats.p, 0u, kStatAlloc);   // the overflow word starts clear
+    Exec x = context_exec(c, &B);
     x.cfg = L.cfg;
+    x.s = fs;
+    x.fragments = (unsigned long long*)(c->buf.stats.as<uint32_t>() + kStatFragments);
     rc = enqueue_front(x, L.P, arena);
     if (rc) return rc;
     const uint64_t want = std::max<uint64_t>(c->pair_cap, 16ull * L.P.prim_total + (1u << 20));
     const uint32_t cap = (uint32_t)std::min<uint64_t>(want, UINT32_MAX);
-    x.r0 = c->ev_raster0;
-    x.r1 = c->ev_raster1;
-    rc = enqueue_back(x, L.P, arena, cap);
+    Binned bn;
+    rc = enqueue_bin(x, L.P, arena, cap, &bn);
     if (rc) return rc;
     c->pair_cap = std::max<uint64_t>(c->pair_cap, cap);
-    uint32_t* stats = c->buf.stats.as<uint32_t>();
-    HIP_TRY(hipMemcpyAsync(c->readback + 32, stats + kStatPairTotal, 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->readback + 34, stats + kStatOverflow, 4, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipEventRecord(c->ev_check, c->stream));
+    uint32_t* stats = B.stats.as<uint32_t>();
+    HIP_TRY(hipMemcpyAsync(c->readback + 32, stats + kStatPairTotal, 8, hipMemcpyDeviceToHost, fs));
+    HIP_TRY(hipMemcpyAsync(c->readback + 34, stats + kStatOverflow, 4, hipMemcpyDeviceToHost, fs));
+    HIP_TRY(hipEventRecord(c->ev_check, fs));
+    HIP_TRY(hipEventRecord(c->set_front_done[L.set], fs));
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->set_front_done[L.set], 0));
+    x.s = c->stream;
+    x.r0 = c->ev_raster0;
+    x.r1 = c->ev_raster1;
+    rc = enqueue_raster(x, L.P, arena, bn);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(c->set_raster_done[L.set], c->stream));
+    c->set_used[L.set] = true;
+    c->chain_seq = c->enqueue_seq;
     return CPV_OK;
 }
 
@@ -738,7 +776,8 @@ int cpvrt::resolve_pending(cpv_context* c, bool may_allocate)
                   (unsigned long long)total, L->scratch_cap);
     else if (may_allocate) {
         c->pair_cap = std::min<uint64_t>(total + total / 8 + 4096, UINT32_MAX);
-        fill_u32(c->stream, c->buf.stats.as<uint32_t>() + kStatOverflow, 0u, 1);
+        fill_u32(c->stream, c->lbuf[L->set].stats.as<uint32_t>() + kStatOverflow, 0u, 1);
+        c->chain_seq = ~0ull;   // the redraw's front half waits for the context stream (the fill, the skipped raster)
         rc = enqueue_checked(c, *L);
     } else {
         Buffers probe;
@@ -768,6 +807,37 @@ int cpvrt::resolve_pending(cpv_context* c, bool may_allocate)
 
 namespace {
 
+// Byte ranges of device memory, for the hazards between render passes.
+
+void attachment_spans(const cpv_pass_desc& p, std::vector<Span>* out)
+{
+    auto add = [&](const cpv_attachment_desc& a) {
+        FormatDesc f;
+        if (!a.address || !format_desc(a.format, &f)) return;
+        out->push_back({a.address, a.address + (uint64_t)p.width * p.height * (uint64_t)f.total});
+    };
+    for (uint32_t k = 0; k < p.color_count && k < CPV_MAX_COLOR_ATTACHMENTS; k++) add(p.color[k]);
+    if (p.has_depth_stencil) add(p.depth_stencil);
+}
+
+void texture_spans(const cpv_draw_desc& d, std::vector<Span>* out)
+{
+    for (const cpv_texture_desc& t : d.textures) {
+        FormatDesc f;
+        if (!t.address || !format_desc(t.format, &f)) continue;
+        // every level of a mip chain fits in twice the base level
+        out->push_back({t.address, t.address + 2ull * t.width * t.height * (uint64_t)f.total});
+    }
+}
+
+bool spans_meet(const std::vector<Span>& a, const std::vector<Span>& b)
+{
+    for (const Span& x : a)
+        for (const Span& y : b)
+            if (x.meets(y)) return true;
+    return false;
+}
+
 // One immediate launch on the context: plan, then upload, front half and
 // back half with no host synchronisation (enqueue_checked).
 int run_launch(cpv_context* c, const cpv_config& cfg, const Program& prog, const std::vector<TargetParams>& targets,
@@ -781,10 +851,28 @@ int run_launch(cpv_context* c, const cpv_config& cfg, const Program& prog, const
     rc = plan_launch(cfg, prog, targets, lds, passes, draw_target, &L->P);
     if (rc) return rc;
     stage_plan(L->P, L->st);
-    if (c->buf.arena.ensure(L->st.bytes.size())) return CPV_ERROR_OUT_OF_MEMORY;
-    patch_plan(L->P, L->st, c->buf.arena.as<uint8_t>());
+    L->set = c->next_set;
+    c->next_set ^= 1u;
+    // The front half may run before the previous launch's raster kernel has
+    // finished: not when this launch's vertex stage reads memory that
+    // launch draws into (an attachment used as a vertex, index or uniform
+    // buffer).
+    std::vector<Span> reads;
+    for (const LaunchDraw& d : lds) {
+        const cpv_draw_desc& dd = *d.draw;
+        for (int b = 0; b < CPV_MAX_VERTEX_BINDINGS; b++)
+            if (dd.vertex_address[b]) reads.push_back({dd.vertex_address[b], dd.vertex_address[b] + dd.vertex_bytes[b]});
+        if (dd.indexed && dd.index_address)
+            reads.push_back({dd.index_address, dd.index_address + ((uint64_t)dd.first + dd.count) * dd.index_size});
+        for (int u = 0; u < CPV_MAX_UNIFORMS; u++)
+            if (dd.uniform_address[u]) reads.push_back({dd.uniform_address[u], dd.uniform_address[u] + (64u << 10)});
+    }
+    if (spans_meet(reads, c->last_written)) c->chain_seq = ~0ull;
+    c->last_written.clear();
+    for (const cpv_pass_desc& p : passes) attachment_spans(p, &c->last_written);
     rc = enqueue_checked(c, *L);
     if (rc) return rc;
+    c->last_set = L->set;
     const LaunchPlan& P = L->P;
     c->last_tiles = P.tiles;
     c->last_waves = cfg.waves_per_tile;
@@ -872,6 +960,40 @@ int group_jobs(const cpv_job_desc* jobs, uint32_t job_count, int tile, std::vect
 
 namespace {
 
+// Opens an immediate render call: checks the previous launch, keeps the
+// launch chain when the previous context call was a render call too, clears
+// the call's fragment counter (on the context stream, before this call's
+// raster kernels) and arms the begin event for its first launch.
+int begin_render_call(cpv_context* c)
+{
+    const bool chained = c->chain_seq == c->enqueue_seq;
+    note_enqueue(c);
+    if (int rc = resolve_pending(c)) return rc;
+    if (chained) c->chain_seq = c->enqueue_seq;
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+    fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
+    c->call_first_launch = true;
+    return CPV_OK;
+}
+
+int end_render_call(cpv_context* c)
+{
+    if (c->call_first_launch) {   // nothing was launched
+        HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+        c->call_first_launch = false;
+    }
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    c->timing_valid = true;
+    c->last_count_exact = c->cfg.count_fragments != 0;
+    c->last_stats = c->buf.stats.as<uint32_t>();
+    return CPV_OK;
+}
+
+}  // namespace
+
+namespace {
+
 // The launches of one render pass: one per run of consecutive draws that
 // share a shader program (the first applies the pass's load ops, the rest
 // load what it stored); a pass with no draw is one empty launch, which still
@@ -929,41 +1051,6 @@ bool pass_has_points_or_lines(const cpv_pipeline_desc* pipelines, const uint32_t
     for (uint32_t k = 0; k < draw_count; k++)
         any = any || pipelines[draw_pipeline ? draw_pipeline[k] : 0].topology < CPV_TOPOLOGY_TRIANGLE_LIST;
     return any;
-}
-
-// Byte ranges of device memory, for the hazards between render passes.
-struct Span {
-    uint64_t lo = 0, hi = 0;
-    bool meets(const Span& o) const { return lo < o.hi && o.lo < hi; }
-};
-
-void attachment_spans(const cpv_pass_desc& p, std::vector<Span>* out)
-{
-    auto add = [&](const cpv_attachment_desc& a) {
-        FormatDesc f;
-        if (!a.address || !format_desc(a.format, &f)) return;
-        out->push_back({a.address, a.address + (uint64_t)p.width * p.height * (uint64_t)f.total});
-    };
-    for (uint32_t k = 0; k < p.color_count && k < CPV_MAX_COLOR_ATTACHMENTS; k++) add(p.color[k]);
-    if (p.has_depth_stencil) add(p.depth_stencil);
-}
-
-void texture_spans(const cpv_draw_desc& d, std::vector<Span>* out)
-{
-    for (const cpv_texture_desc& t : d.textures) {
-        FormatDesc f;
-        if (!t.address || !format_desc(t.format, &f)) continue;
-        // every level of a mip chain fits in twice the base level
-        out->push_back({t.address, t.address + 2ull * t.width * t.height * (uint64_t)f.total});
-    }
-}
-
-bool spans_meet(const std::vector<Span>& a, const std::vector<Span>& b)
-{
-    for (const Span& x : a)
-        for (const Span& y : b)
-            if (x.meets(y)) return true;
-    return false;
 }
 
 // The launches of several render passes, each with its own draws
@@ -1060,12 +1147,7 @@ extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const 
 {
     if (int rc = check_pass_args(c, pass, pipelines, draws, draw_count)) return rc;
     std::lock_guard<std::mutex> guard(c->lock);
-    note_enqueue(c);
-    if (int rc = resolve_pending(c)) return rc;
-    HIP_TRY(hipSetDevice(c->device));
-    if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
-    fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
-    HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+    if (int rc = begin_render_call(c)) return rc;
     const cpv_config cfg = call_config(c->cfg, pass_has_points_or_lines(pipelines, draw_pipeline, draw_count));
     const std::vector<cpv_pass_desc> passes(1, *pass);
     std::vector<PassLaunch> launches;
@@ -1078,11 +1160,7 @@ extern "C" int cpv_render_pass(cpv_context* c, const cpv_pass_desc* pass, const 
         rc = run_launch(c, cfg, L.prog, targets, L.lds, passes, L.dt);
         if (rc) return rc;
     }
-    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
-    c->timing_valid = true;
-    c->last_count_exact = c->cfg.count_fragments != 0;
-    c->last_stats = c->buf.stats.as<uint32_t>();
-    return CPV_OK;
+    return end_render_call(c);
 }
 
 // cpv_render_pass with every working buffer in this GPU's scratch array:
@@ -1153,12 +1231,7 @@ extern "C" int cpv_render_jobs(cpv_context* c, const cpv_job_desc* jobs, uint32_
 {
     if (!c || (job_count && !jobs)) return fail(CPV_ERROR_INVALID, "null argument");
     std::lock_guard<std::mutex> guard(c->lock);
-    note_enqueue(c);
-    if (int rc = resolve_pending(c)) return rc;
-    HIP_TRY(hipSetDevice(c->device));
-    if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
-    fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
-    HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+    if (int rc = begin_render_call(c)) return rc;
     const cpv_config cfg = call_config(c->cfg, jobs_have_points_or_lines(jobs, job_count));
     std::vector<LaunchSpec> groups;
     int rc = group_jobs(jobs, job_count, (int)cfg.tile_size, &groups);
@@ -1167,11 +1240,7 @@ extern "C" int cpv_render_jobs(cpv_context* c, const cpv_job_desc* jobs, uint32_
         rc = run_launch(c, cfg, G.prog, G.targets, G.lds, G.passes, G.dt);
         if (rc) return rc;
     }
-    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
-    c->timing_valid = true;
-    c->last_count_exact = c->cfg.count_fragments != 0;
-    c->last_stats = c->buf.stats.as<uint32_t>();
-    return CPV_OK;
+    return end_render_call(c);
 }
 
 namespace {
@@ -1188,12 +1257,7 @@ extern "C" int cpv_render_passes(cpv_context* c, const cpv_pass_desc* passes, ui
 {
     if (int rc = check_passes_args(c, passes, pass_count, pipelines, draws, draw_pipeline, first_draw)) return rc;
     std::lock_guard<std::mutex> guard(c->lock);
-    note_enqueue(c);
-    if (int rc = resolve_pending(c)) return rc;
-    HIP_TRY(hipSetDevice(c->device));
-    if (c->buf.stats.ensure(kStatAlloc * 4)) return CPV_ERROR_OUT_OF_MEMORY;
-    fill_u32(c->stream, c->buf.stats.p, 0u, kStatWords);
-    HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+    if (int rc = begin_render_call(c)) return rc;
     const uint32_t n = pass_count ? first_draw[pass_count] : 0;
     const cpv_config cfg = call_config(c->cfg, passes_have_points_or_lines(pipelines, draw_pipeline, n));
     std::vector<LaunchSpec> specs;
@@ -1203,11 +1267,7 @@ extern "C" int cpv_render_passes(cpv_context* c, const cpv_pass_desc* passes, ui
         rc = run_launch(c, cfg, G.prog, G.targets, G.lds, G.passes, G.dt);
         if (rc) return rc;
     }
-    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
-    c->timing_valid = true;
-    c->last_count_exact = c->cfg.count_fragments != 0;
-    c->last_stats = c->buf.stats.as<uint32_t>();
-    return CPV_OK;
+    return end_render_call(c);
 }
 
 extern "C" int cpv_last_fragment_count(cpv_context* c, uint64_t* fragments)
@@ -1247,8 +1307,9 @@ extern "C" int cpv_get_tile_traces(cpv_context* c, cpv_tile_trace* traces, uint3
     if (n == 0) return CPV_OK;
     std::vector<float> feat((size_t)n * kFeatureStride), cyc((size_t)n * waves);
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(feat.data(), c->buf.features.p, feat.size() * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(cyc.data(), c->buf.traces.p, cyc.size() * 4, hipMemcpyDeviceToHost));
+    const Buffers& B = c->lbuf[c->last_set];
+    HIP_TRY(hipMemcpy(feat.data(), B.features.p, feat.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cyc.data(), B.traces.p, cyc.size() * 4, hipMemcpyDeviceToHost));
     for (uint32_t t = 0; t < n; t++) {
         cpv_tile_trace& o = traces[t];
         std::memset(&o, 0, sizeof(o));

@@ -163,7 +163,7 @@ void Buffers::release_all()
     for (DevBuf* b : bufs) b->release();
 }
 
-int cpvrt::stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes, bool may_grow)
+int cpvrt::stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes, bool may_grow, hipStream_t s)
 {
     if (!bytes) return CPV_OK;
     cpv_context::Staged& slot = c->staged[c->staged_next++ % 4u];
@@ -180,8 +180,9 @@ int cpvrt::stage_upload(cpv_context* c, void* dst, const void* src, size_t bytes
         slot.cap = want;
     }
     std::memcpy(slot.p, src, bytes);
-    HIP_TRY(hipMemcpyAsync(dst, slot.p, bytes, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipEventRecord(slot.done, c->stream));
+    if (!s) s = c->stream;
+    HIP_TRY(hipMemcpyAsync(dst, slot.p, bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(slot.done, s));
     slot.used = true;
     return CPV_OK;
 }
@@ -204,6 +205,12 @@ extern "C" int cpv_context_create(int device_ordinal, cpv_context** out)
         hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
         hipEventCreate(&c->ev_raster0) != hipSuccess || hipEventCreate(&c->ev_raster1) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_check, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->set_front_done[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->set_front_done[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->set_raster_done[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->set_raster_done[1], hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->front_stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void**)&c->readback, 256, hipHostMallocDefault) != hipSuccess) {
         delete c;
         return fail(CPV_ERROR_DEVICE, "stream/event/pinned allocation failed");
@@ -230,10 +237,13 @@ extern "C" void cpv_context_destroy(cpv_context* c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    (void)sync_context(c);
     destroy_pending(c->pending);
     c->pending = nullptr;
     c->buf.release_all();
+    for (cpvrt::Buffers& b : c->lbuf) b.release_all();
+    for (hipEvent_t e : {c->ev_pre, c->set_front_done[0], c->set_front_done[1], c->set_raster_done[0], c->set_raster_done[1]})
+        if (e) (void)hipEventDestroy(e);
     for (cpv_context::Staged& slot : c->staged) {
         if (slot.p) (void)hipHostFree(slot.p);
         if (slot.done) (void)hipEventDestroy(slot.done);
