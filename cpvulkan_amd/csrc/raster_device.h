@@ -410,31 +410,21 @@ CPV_DEV void apply_blend(const FragmentState& st, const float s[4], const float 
 // ---------------------------------------------------------- fragment ops
 
 // CompileFCompareTest / CompileICompareTest (LLVMRuntime/PipelineCompiler.cpp:1492-1526).
+// A VkCompareOp's three low bits select less, equal and greater (NEVER 0,
+// LESS 1, EQUAL 2, LESS_OR_EQUAL 3, GREATER 4, NOT_EQUAL 5, GREATER_OR_EQUAL
+// 6), so the ordered predicates are unions of the three ordered relations;
+// ALWAYS (7) also passes unordered (NaN) operands, through bit 3 of both
+// sides.  Computed in vector registers from the op's bits, so no branch and
+// no per-op scalar mask depends on it.
 CPV_DEV bool fcompare(float a, float b, int32_t op)
 {
-    switch (op) {
-    case 0: return false;
-    case 1: return a < b;
-    case 2: return a == b;
-    case 3: return a <= b;
-    case 4: return a > b;
-    case 5: return a < b || a > b;
-    case 6: return a >= b;
-    default: return true;
-    }
+    const uint32_t rel = (a < b ? 1u : 0u) | (a == b ? 2u : 0u) | (a > b ? 4u : 0u) | (op == 7 ? 8u : 0u);
+    return (rel & ((uint32_t)op | 8u)) != 0u;
 }
 CPV_DEV bool icompare(uint32_t a, uint32_t b, int32_t op)
 {
-    switch (op) {
-    case 0: return false;
-    case 1: return a < b;
-    case 2: return a == b;
-    case 3: return a <= b;
-    case 4: return a > b;
-    case 5: return a != b;
-    case 6: return a >= b;
-    default: return true;
-    }
+    const uint32_t rel = (a < b ? 1u : 0u) | (a == b ? 2u : 0u) | (a > b ? 4u : 0u) | (op == 7 ? 8u : 0u);
+    return (rel & ((uint32_t)op | 8u)) != 0u;
 }
 
 // CompileGetStencilResult (PipelineCompiler.cpp:1382-1413); the clamped
@@ -505,6 +495,19 @@ CPV_DEV bool depth_stencil_ops(const FragmentState& st, PixelState& px, float de
     return depth_ok;
 }
 
+// True when every depth in [lo, hi] fails the compare op against every
+// stored depth in [bmin, bmax] (NaN operands make the relations false: only
+// NEVER then rejects).  Bit `op` of a word of the per-op answers: NEVER
+// always, LESS lo >= bmax, EQUAL lo > bmax or hi < bmin, LESS_OR_EQUAL
+// lo > bmax, GREATER hi <= bmin, GREATER_OR_EQUAL hi < bmin; NOT_EQUAL and
+// ALWAYS never.  No branch depends on the op.
+CPV_DEV bool interval_fails(int32_t op, float lo, float hi, float bmin, float bmax)
+{
+    const uint32_t ge = lo >= bmax ? 1u : 0u, gt = lo > bmax ? 1u : 0u, le = hi <= bmin ? 1u : 0u, lt = hi < bmin ? 1u : 0u;
+    const uint32_t bits = 1u | (ge << 1) | ((gt | lt) << 2) | (gt << 3) | (le << 4) | (lt << 6);
+    return ((bits >> (uint32_t)(op & 7)) & 1u) != 0u;
+}
+
 // Conservative early depth rejection.  The reference computes
 //   w_i = e_i / area (correctly rounded), depth = (z0*w0 + z1*w1) + z2*w2,
 //   dt = (maxD - minD) * depth + minD
@@ -531,15 +534,7 @@ CPV_DEV bool depth_surely_fails(int32_t op, float cur, float e0, float e1, float
     const float tl = sc * lo + vp_min, th = sc * hi + vp_min;
     if (!(__builtin_isfinite(tl) && __builtin_isfinite(th))) return false;
     const float tmin = fminf(tl, th), tmax = fmaxf(tl, th);
-    switch (op) {
-    case 0: return true;                           // NEVER
-    case 1: return tmin >= cur;                    // LESS fails when dt >= cur
-    case 2: return tmin > cur || tmax < cur;       // EQUAL
-    case 3: return tmin > cur;                     // LESS_OR_EQUAL
-    case 4: return tmax <= cur;                    // GREATER
-    case 6: return tmax < cur;                     // GREATER_OR_EQUAL
-    default: return false;                         // NOT_EQUAL, ALWAYS
-    }
+    return interval_fails(op, tmin, tmax, cur, cur);
 }
 
 // depth_surely_fails for the viewport depth range [0, 1], where
@@ -553,15 +548,7 @@ CPV_DEV bool depth_surely_fails_unit(int32_t op, float cur, float e0, float e1, 
     const float B = S * 0x1p-19f + 0x1p-120f;
     const float lo = da - B, hi = da + B;
     if (!(__builtin_isfinite(lo) && __builtin_isfinite(hi))) return false;
-    switch (op) {
-    case 0: return true;
-    case 1: return lo >= cur;
-    case 2: return lo > cur || hi < cur;
-    case 3: return lo > cur;
-    case 4: return hi <= cur;
-    case 6: return hi < cur;
-    default: return false;
-    }
+    return interval_fails(op, lo, hi, cur, cur);
 }
 
 // Colour write of CompileWriteFragmentAttachment/Blend (PipelineCompiler.cpp:1528-1727)

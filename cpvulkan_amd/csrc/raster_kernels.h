@@ -85,15 +85,7 @@ struct RasterArgs {
 // [bmin, bmax] (NaN stored depths fail every ordered compare anyway).
 __device__ __forceinline__ bool hiz_rejects(int32_t op, float lo, float hi, float bmin, float bmax)
 {
-    switch (op) {
-    case 0: return true;                        // NEVER
-    case 1: return lo >= bmax;                  // LESS
-    case 2: return lo > bmax || hi < bmin;      // EQUAL
-    case 3: return lo > bmax;                   // LESS_OR_EQUAL
-    case 4: return hi <= bmin;                  // GREATER
-    case 6: return hi < bmin;                   // GREATER_OR_EQUAL
-    default: return false;
-    }
+    return interval_fails(op, lo, hi, bmin, bmax);
 }
 
 __device__ __forceinline__ float stored_depth(int32_t kind, uint32_t d0)

@@ -287,7 +287,7 @@ raster_kernel(RasterArgs a)
 #pragma unroll
                 for (int k = 0; k < BPW; k++) {
                     if (inside[k]) {
-                        const float cd = stored_depth(a.hiz_depth_kind, px[k].ds0);
+                        const float cd = stored_depth(FIXED ? 32 : a.hiz_depth_kind, px[k].ds0);
                         bmin = fminf(bmin, cd);
                         bmax = fmaxf(bmax, cd);
                     }
@@ -429,7 +429,7 @@ raster_kernel(RasterArgs a)
                     const int x = pxx[k], y = pxy[k];
                     if (x < r.x0 || x >= r.x1 || y < r.y0 || y >= r.y1) continue;
                     if (a.hiz_op >= 0) {
-                        const float cd = stored_depth(a.hiz_depth_kind, px[k].ds0);
+                        const float cd = stored_depth(FIXED ? 32 : a.hiz_depth_kind, px[k].ds0);
                         if (hiz_rejects(a.hiz_op, r.dt_lo, r.dt_hi, cd, cd)) continue;
                     }
                     float w0, w1, w2, depth;
