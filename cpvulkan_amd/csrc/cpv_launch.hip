@@ -70,6 +70,7 @@ struct LaunchPlan {
     int raster_variant = 3;          // index into Program::raster
     bool lines = false;              // some draw rasterises lines (LineExt records)
     int32_t hiz_op = -1, hiz_kind = 32;
+    int32_t uniform_fstate = -1;     // every draw's fragment state is fstates[0] (0), or not (-1)
     size_t o_env = 0, o_dd = 0, o_fs = 0, o_tp = 0, o_vb = 0, o_sb = 0;   // offsets in the arena
 };
 
@@ -212,6 +213,9 @@ int plan_launch(const cpv_config& cfg, const Program& prog, const std::vector<Ta
             P->hiz_kind = fs[0].ds.depth_kind;
         }
     }
+    P->uniform_fstate = P->fstates.empty() ? -1 : 0;
+    for (const FragmentState& f : P->fstates)
+        if (std::memcmp(&f, &P->fstates[0], sizeof(f)) != 0) P->uniform_fstate = -1;
     // the block tables are never empty
     if (P->vblock.empty()) P->vblock.push_back(0);
     if (P->sblock.empty()) P->sblock.push_back(0);
@@ -498,6 +502,7 @@ int enqueue_raster(Exec& x, const LaunchPlan& P, uint8_t* arena, const Binned& b
     ra.persistent = sched == CPV_SCHED_COUNTER ? 1u : 0u;
     ra.hiz_op = P.hiz_op;
     ra.hiz_depth_kind = P.hiz_kind;
+    ra.uniform_fstate = P.uniform_fstate;
     uint32_t grid = P.tiles;
     if (ra.persistent) {
         const int resident = resident_workgroups(x, rk, block);

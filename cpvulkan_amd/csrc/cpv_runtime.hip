@@ -495,6 +495,11 @@ int build_fragment_state(const cpv_pipeline_desc& p, const cpv_pass_desc& pass, 
                                b.src_alpha_factor > CPV_BLEND_SRC_ALPHA_SATURATE || b.dst_alpha_factor > CPV_BLEND_SRC_ALPHA_SATURATE ||
                                b.color_op > CPV_BLEND_OP_MAX || b.alpha_op > CPV_BLEND_OP_MAX))
             return fail(CPV_ERROR_UNSUPPORTED, "blend factor/op");
+        fs->blend_kind = b.blend_enable && b.src_color_factor == CPV_BLEND_SRC_ALPHA && b.dst_color_factor == CPV_BLEND_ONE_MINUS_SRC_ALPHA &&
+                                 b.color_op == CPV_BLEND_OP_ADD && b.src_alpha_factor == CPV_BLEND_SRC_ALPHA &&
+                                 b.dst_alpha_factor == CPV_BLEND_ONE_MINUS_SRC_ALPHA && b.alpha_op == CPV_BLEND_OP_ADD
+                             ? 1
+                             : 0;
     }
     for (int i = 0; i < 4; i++) fs->blend_constants[i] = p.blend_constants[i];
     const cpv_stencil_state* st[2] = {&p.front, &p.back};

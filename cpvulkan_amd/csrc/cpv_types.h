@@ -135,7 +135,10 @@ struct FragmentState {
     int32_t st_front[7];
     int32_t st_back[7];
     int32_t origin_upper_left;
-    int32_t pad;
+    // 1 when blending is SRC_ALPHA / ONE_MINUS_SRC_ALPHA / ADD for both colour
+    // and alpha (set by the host; the kernel then skips the factor and op
+    // selection), else 0
+    int32_t blend_kind;
 };
 
 // Per-render-target parameters (one per job in a batch).

@@ -570,7 +570,14 @@ CPV_DEV void color_write(const FragmentState& st, PixelState& px, const float co
     if (st.blend_enable) {
         float dst[4], out[4];
         unpack_color(f, px.color, dst);
-        apply_blend(st, colour, dst, out);
+        if (st.blend_kind == 1) {
+            // apply_blend with sf = (s.a, s.a, s.a, s.a), df = (1 - s.a, ...), ADD
+            const float sa = colour[3], da = 1.0f - colour[3];
+#pragma unroll
+            for (int i = 0; i < 4; i++) out[i] = colour[i] * sa + dst[i] * da;
+        } else {
+            apply_blend(st, colour, dst, out);
+        }
         pack_color_f32(f, out, px.color);
     } else {
         pack_color_f32(f, colour, px.color);

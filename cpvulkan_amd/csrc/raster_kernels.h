@@ -79,6 +79,7 @@ struct RasterArgs {
     uint32_t persistent;            // 0: one workgroup per tile; 1: work queue
     int32_t hiz_op;                 // depth compare op shared by every draw when depth culling is exact, else -1
     int32_t hiz_depth_kind;         // 16, 24 or 32
+    int32_t uniform_fstate;         // the fragment state every draw of the launch has, -1 when they differ
 };
 
 // True when a depth in [lo, hi] fails `op` against every stored depth in
@@ -210,14 +211,17 @@ __device__ __forceinline__ void fragment_inputs(const Rec& g, const float* vout,
     }
 }
 
+// The shader and the colour write of one queued fragment (lanes hold
+// fragments of different primitives).  When every draw of the launch has
+// the same fragment state (uniform_fstate >= 0) it is read once for the
+// wave, so the blend and format state are scalar and its branches uniform.
 template <class FS, bool ANYPRIM = false, bool FIXED = false>
 __device__ __forceinline__ PixelState shade_fragment(const TriSetup* recs, const DrawDesc* draws, const FragmentState* fstates,
-                                                  const float* vout, uint32_t stride, uint32_t gprim, float w0, float w1,
-                                                  float w2, int x, int y, PixelState ps)
+                                                  int32_t uniform_fstate, const float* vout, uint32_t stride, uint32_t gprim,
+                                                  float w0, float w1, float w2, int x, int y, PixelState ps)
 {
     const TriSetup& g = recs[gprim];
     const DrawDesc& d = draws[g.draw];
-    const FragmentState& fs = fstates[d.fstate];
     const DrawEnv& de = *(const DrawEnv*)d.env;
     float in[FS::kInputs > 0 ? FS::kInputs : 1];
     fragment_inputs<FS, ANYPRIM>(g, vout, stride, w0, w1, w2, in);
@@ -226,10 +230,18 @@ __device__ __forceinline__ PixelState shade_fragment(const TriSetup* recs, const
     const float depth = (ANYPRIM && (g.flags & kPrimPoint)) ? g.z[0]
                         : (ANYPRIM && (g.flags & kPrimLine)) ? g.z[0] * w1 + g.z[1] * (1.0f - w1)
                                                                : g.z[0] * w0 + g.z[1] * w1 + g.z[2] * w2;
-    const float fc[4] = {fs.origin_upper_left ? (float)x : d.vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
     float colour[4];
-    FS::run(de, in, fc, (g.flags & 1u) != 0, colour);
-    color_write_state<FIXED>(fs, ps, colour);
+    if (uniform_fstate >= 0) {
+        const FragmentState& fs = fstates[uniform_fstate];
+        const float fc[4] = {fs.origin_upper_left ? (float)x : d.vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
+        FS::run(de, in, fc, (g.flags & 1u) != 0, colour);
+        color_write_state<FIXED>(fs, ps, colour);
+    } else {
+        const FragmentState& fs = fstates[d.fstate];
+        const float fc[4] = {fs.origin_upper_left ? (float)x : d.vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
+        FS::run(de, in, fc, (g.flags & 1u) != 0, colour);
+        color_write_state<FIXED>(fs, ps, colour);
+    }
     return ps;
 }
 

@@ -253,7 +253,8 @@ raster_kernel(RasterArgs a)
 #pragma unroll
             for (int kk = 1; kk < BPW; kk++)
                 if (kk == k) { ps = px[kk]; sx = pxx[kk]; sy = pxy[kk]; }
-            ps = shade_fragment<FS, ANYPRIM, FIXED>(a.recs, a.draws, a.fstates, a.vout, a.vout_stride, gprim, w0, w1, w2, sx, sy, ps);
+            ps = shade_fragment<FS, ANYPRIM, FIXED>(a.recs, a.draws, a.fstates, a.uniform_fstate, a.vout, a.vout_stride, gprim, w0, w1,
+                                                    w2, sx, sy, ps);
 #pragma unroll
             for (int kk = 0; kk < BPW; kk++)
                 if (kk == k) px[kk] = ps;
