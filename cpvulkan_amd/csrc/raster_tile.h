@@ -384,6 +384,10 @@ raster_kernel(RasterArgs a)
                     }
                 } else {
                     load_record_uniform(a.recs, gprim, r);
+                    // the loads of all visit fields issued before the first of
+                    // them is used: one memory latency per survivor, not one
+                    // for the draw index and another for the rest
+                    __asm__ volatile("" ::"s"(r.ax[0]), "s"(r.y1), "s"(r.dt_lo), "s"(r.draw), "s"(r.z[0]), "s"(r.flags));
                 }
                 if (r.draw != cur_draw) {
                     cur_draw = r.draw;
