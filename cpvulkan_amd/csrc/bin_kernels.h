@@ -501,60 +501,77 @@ __device__ __forceinline__ float pixel_centre(int p, float extent, float half)
     return ((float)p / extent + half) * 2.0f - 1.0f;
 }
 
-// Whether a pixel of the rectangle with corner centres xs x ys can pass all
-// three edge tests of triangle record r, evaluated exactly as the raster
-// kernel evaluates them: (x - a.x) * dy - (y - a.y) * dx, fp32 without
-// contraction, at the centres pixel_centre gives.  Each rounded step is
-// monotone in x alone or in y alone, so the largest rounded edge value over
-// the rectangle is taken at one of its four corners: when it is negative
-// (and no corner gives NaN) no pixel of the rectangle is covered.
-__device__ __forceinline__ bool rect_may_cover(const TriSetup& r, const float xs[2], const float ys[2])
-{
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        bool neg = true;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const float e = (xs[c & 1] - r.ax[i]) * r.dy[i] - (ys[c >> 1] - r.ay[i]) * r.dx[i];
-            neg = neg && e < 0.0f;   // NaN is not < 0: no cull
-        }
-        if (neg) return false;
-    }
-    return true;
-}
-
-// pair_blocks less the quarters where triangle r covers no pixel (points
-// and lines keep their box's quarters).  box is relative to the primitive's
-// first tile, whose pixel origin is (r.x0 - box.x, r.y0 - box.y).  The
-// corner centres of the tile's two quarter columns and rows are computed
-// once.
-__device__ __forceinline__ uint32_t pair_blocks_covered(int tile_shift, int4 box, uint32_t col, uint32_t row, const TriSetup& r,
-                                                        const DrawDesc& d)
+// pair_blocks less, for a triangle, the quarters where it covers no pixel
+// (points and lines keep their box's quarters).  box is relative to the
+// primitive's first tile, whose pixel origin is (ox, oy); e holds the
+// record's edges (bytes 0-47: ax[3], ay[3], dy[3], dx[3]) and vp the draw's
+// (vp_w, vp_h, half_x, half_y).
+//
+// The raster kernel's edge value at a pixel centre (x, y) is
+// fl(A(x) - B(y)), A(x) = fl(fl(x - a.x) * dy), B(y) = fl(fl(y - a.y) * dx),
+// fp32 without contraction, at the centres pixel_centre gives.  Rounding is
+// monotone, so with finite edge terms and centres A is non-decreasing in x
+// when dy > 0, non-increasing when dy < 0 and zero when dy = 0, B likewise
+// in y with dx, and fl(A - B) is non-decreasing in A and non-increasing in
+// B.  Over a quarter's centres, x in [xlo, xhi] and y in [ylo, yhi], no
+// pixel's value exceeds A(dy > 0 ? xhi : xlo) - B(dx < 0 ? yhi : ylo); when
+// that is negative every pixel's is (a pixel's NaN needs A = B = +-inf,
+// which makes the bound +inf or NaN) and the quarter is dropped.  NaN is
+// not < 0, and non-finite terms keep every quarter.  Each edge's bound is
+// computed once per quarter column and row.
+__device__ __forceinline__ uint32_t pair_blocks_covered(int tile_shift, int4 box, uint32_t col, uint32_t row, bool triangle,
+                                                        const float4 e[3], float4 vp, int ox, int oy)
 {
     const uint32_t m = pair_blocks(tile_shift, box, col, row);
-    if (m == 0u || (r.flags & (kPrimPoint | kPrimLine))) return m;
-    const int ox = r.x0 - box.x, oy = r.y0 - box.y;
+    if (m == 0u || !triangle) return m;
     const int half = 1 << (tile_shift - 1);
     const int X = (int)col << tile_shift, Y = (int)row << tile_shift;
     const uint32_t bits = m >> kPairBlockShift;
-    float xs[2][2], ys[2][2];
+    const float ax[3] = {e[0].x, e[0].y, e[0].z}, ay[3] = {e[0].w, e[1].x, e[1].y};
+    const float dy[3] = {e[1].z, e[1].w, e[2].x}, dx[3] = {e[2].y, e[2].z, e[2].w};
+    bool finite = true;   // & rather than &&: no branch per term
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        finite = finite & __builtin_isfinite(ax[i]) & __builtin_isfinite(ay[i]) & __builtin_isfinite(dx[i]) &
+                 __builtin_isfinite(dy[i]);
+    float xlo[2], xhi[2], ylo[2], yhi[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
+        xlo[h] = xhi[h] = ylo[h] = yhi[h] = 0.0f;
         // quarter column h is in the box when a quarter of it is (bits h, h + 2)
         if (bits & (5u << h)) {
-            xs[h][0] = pixel_centre(ox + max(X + half * h, box.x), d.vp_w, d.half_x);
-            xs[h][1] = pixel_centre(ox + min(X + half * h + half, box.z) - 1, d.vp_w, d.half_x);
+            const float u = pixel_centre(ox + max(X + half * h, box.x), vp.x, vp.z);
+            const float v = pixel_centre(ox + min(X + half * h + half, box.z) - 1, vp.x, vp.z);
+            finite = finite & __builtin_isfinite(u) & __builtin_isfinite(v);
+            xlo[h] = fminf(u, v);
+            xhi[h] = fmaxf(u, v);
         }
         // quarter row h (bits 2h, 2h + 1)
         if (bits & (3u << (2 * h))) {
-            ys[h][0] = pixel_centre(oy + max(Y + half * h, box.y), d.vp_h, d.half_y);
-            ys[h][1] = pixel_centre(oy + min(Y + half * h + half, box.w) - 1, d.vp_h, d.half_y);
+            const float u = pixel_centre(oy + max(Y + half * h, box.y), vp.y, vp.w);
+            const float v = pixel_centre(oy + min(Y + half * h + half, box.w) - 1, vp.y, vp.w);
+            finite = finite & __builtin_isfinite(u) & __builtin_isfinite(v);
+            ylo[h] = fminf(u, v);
+            yhi[h] = fmaxf(u, v);
         }
     }
+    if (!finite) return m;
+    float amax[3][2], bmin[3][2];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            amax[i][h] = ((dy[i] > 0.0f ? xhi[h] : xlo[h]) - ax[i]) * dy[i];
+            bmin[i][h] = ((dx[i] < 0.0f ? yhi[h] : ylo[h]) - ay[i]) * dx[i];
+        }
     uint32_t keep = 0u;
 #pragma unroll
-    for (int b = 0; b < 4; b++)
-        if (((bits >> b) & 1u) && rect_may_cover(r, xs[b & 1], ys[b >> 1])) keep |= 1u << (kPairBlockShift + b);
+    for (int b = 0; b < 4; b++) {
+        bool cover = ((bits >> b) & 1u) != 0u;
+#pragma unroll
+        for (int i = 0; i < 3; i++) cover = cover & !(amax[i][b & 1] - bmin[i][b >> 1] < 0.0f);
+        if (cover) keep |= 1u << (kPairBlockShift + b);
+    }
     return keep;
 }
 
@@ -564,17 +581,23 @@ __device__ __forceinline__ uint32_t pair_blocks_covered(int tile_shift, int4 box
 // (less the runs of primitives with more than kEmitBig pairs, queued for
 // emit_big_kernel): its threads stride over the run, so consecutive lanes
 // write consecutive pairs, and each finds its primitive by binary search of
-// the workgroup's scanned counts in LDS.  Tiles of a primitive go in
-// row-major order of its tile rectangle.
+// the workgroup's scanned counts in LDS.  What a pair needs of its
+// primitive (tile rectangle, edges, viewport) is staged in LDS by the
+// primitive's thread, so the pair loop reads no record.  Tiles of a
+// primitive go in row-major order of its tile rectangle.
 __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
 {
+    constexpr uint32_t kNoEdgeTest = 0x80000000u;   // s_w flag: a point or line keeps its box's quarters
     __shared__ uint32_t s_tmp[4];
     __shared__ uint32_t s_off[257];   // exclusive scan of the kept pair counts
     __shared__ uint32_t s_dst[256];   // output offset of each primitive's first pair
     __shared__ uint32_t s_key[256];   // global tile index of each primitive's first tile
-    __shared__ uint32_t s_w[256];     // tile columns of each primitive's rectangle
+    __shared__ uint32_t s_w[256];     // tile columns of each primitive's rectangle, | kNoEdgeTest
     __shared__ uint32_t s_tx[256];    // tile columns of each primitive's target
     __shared__ int4 s_box[256];       // pixel box of each primitive, relative to its first tile
+    __shared__ int2 s_org[256];       // pixel origin of each primitive's first tile
+    __shared__ float4 s_edge[256][3]; // each primitive's edges (record bytes 0-47)
+    __shared__ float4 s_vp[256];      // its draw's vp_w, vp_h, half_x, half_y
     const uint32_t nblk = (a.prims + 255u) / 256u;
     for (uint32_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
         const uint32_t gp = blk * 256u + threadIdx.x;
@@ -585,38 +608,48 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
         }
         if (c) {
             const TriSetup& r = a.recs[gp];
-            const TargetParams& t = a.targets[a.draws[r.draw].target];
+            const float4* e4 = (const float4*)&r;
+            const float4 e0 = e4[0], e1 = e4[1], e2 = e4[2];
+            const DrawDesc& dd = a.draws[r.draw];
+            const TargetParams& t = a.targets[dd.target];
             const int sh = a.tile_shift;
             const uint32_t tx0 = (uint32_t)r.x0 >> sh, tx1 = (uint32_t)(r.x1 - 1) >> sh;
             const uint32_t ty0 = (uint32_t)r.y0 >> sh;
             s_dst[threadIdx.x] = a.offsets[gp];
             s_key[threadIdx.x] = t.tile_base + ty0 * t.tiles_x + tx0;
-            s_w[threadIdx.x] = tx1 - tx0 + 1;
+            s_w[threadIdx.x] = (tx1 - tx0 + 1) | ((r.flags & (kPrimPoint | kPrimLine)) ? kNoEdgeTest : 0u);
             s_tx[threadIdx.x] = t.tiles_x;
             const int ox = (int)(tx0 << sh), oy = (int)(ty0 << sh);
             s_box[threadIdx.x] = make_int4(r.x0 - ox, r.y0 - oy, r.x1 - ox, r.y1 - oy);
+            s_org[threadIdx.x] = make_int2(ox, oy);
+            s_edge[threadIdx.x][0] = e0;
+            s_edge[threadIdx.x][1] = e1;
+            s_edge[threadIdx.x][2] = e2;
+            s_vp[threadIdx.x] = make_float4(dd.vp_w, dd.vp_h, dd.half_x, dd.half_y);
         }
         uint32_t total;
         s_off[threadIdx.x] = block_exclusive_scan(c, s_tmp, total);
         if (threadIdx.x == 0) s_off[256] = total;
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < total; j += 256u) {
-            // q: the last primitive whose run starts at or before j
-            uint32_t lo = 0, hi = 256;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_off[mid] <= j) lo = mid; else hi = mid;
-            }
+            // lo: the last primitive whose run starts at or before j (s_off is
+            // non-decreasing and s_off[0] = 0), in eight unrolled halvings
+            uint32_t lo = 0;
+#pragma unroll
+            for (uint32_t step = 128; step >= 1; step >>= 1)
+                if (s_off[lo + step] <= j) lo += step;
             const uint32_t k = j - s_off[lo];
-            const uint32_t w = s_w[lo];
+            const uint32_t wl = s_w[lo];
+            const uint32_t w = wl & ~kNoEdgeTest;
             const uint32_t row = k / w;
             const uint32_t out = s_dst[lo] + k;
             if (out < a.cap) {
                 const uint32_t col = k - row * w;
                 a.keys[out] = s_key[lo] + row * s_tx[lo] + col;
-                const uint32_t gp = blk * 256u + lo;
-                const TriSetup& r = a.recs[gp];
-                a.vals[out] = gp | pair_blocks_covered(a.tile_shift, s_box[lo], col, row, r, a.draws[r.draw]);
+                const int2 org = s_org[lo];
+                const float4 e[3] = {s_edge[lo][0], s_edge[lo][1], s_edge[lo][2]};
+                a.vals[out] = (blk * 256u + lo) |
+                              pair_blocks_covered(a.tile_shift, s_box[lo], col, row, (wl & kNoEdgeTest) == 0u, e, s_vp[lo], org.x, org.y);
             }
         }
         __syncthreads();
@@ -640,10 +673,15 @@ __global__ __launch_bounds__(256) void emit_big_kernel(EmitArgs a)
         const uint32_t c = a.counts[gp], off = a.offsets[gp];
         const int4 box = make_int4(r.x0 - (int)(tx0 << sh), r.y0 - (int)(ty0 << sh), r.x1 - (int)(tx0 << sh),
                                    r.y1 - (int)(ty0 << sh));
+        const DrawDesc& dd = a.draws[r.draw];
+        const float4* e4 = (const float4*)&r;
+        const float4 e[3] = {e4[0], e4[1], e4[2]};
+        const float4 vp = make_float4(dd.vp_w, dd.vp_h, dd.half_x, dd.half_y);
+        const bool triangle = (r.flags & (kPrimPoint | kPrimLine)) == 0u;
         for (uint32_t k = threadIdx.x; k < c && off + k < a.cap; k += 256u) {
             const uint32_t row = k / w, col = k % w;
             a.keys[off + k] = t.tile_base + (ty0 + row) * t.tiles_x + tx0 + col;
-            a.vals[off + k] = gp | pair_blocks_covered(sh, box, col, row, r, a.draws[r.draw]);
+            a.vals[off + k] = gp | pair_blocks_covered(sh, box, col, row, triangle, e, vp, (int)(tx0 << sh), (int)(ty0 << sh));
         }
     }
 }
