@@ -251,8 +251,10 @@ struct DepthStencilResult {
 };
 
 // depth_stencil_ops with the pixel state passed and returned by value (one
-// call site per raster kernel instantiation, inlined).
-template <bool FIXED = false>
+// call site per raster kernel instantiation, inlined).  OP >= 0: the draw
+// has the depth test with compare op OP and no stencil or depth-bounds
+// test, folded into the state as constants.
+template <bool FIXED = false, int OP = -1>
 __device__ __forceinline__ DepthStencilResult depth_stencil_call(const FragmentState* st, PixelState px, float depth, uint32_t front)
 {
     DepthStencilResult r;
@@ -260,6 +262,12 @@ __device__ __forceinline__ DepthStencilResult depth_stencil_call(const FragmentS
     if constexpr (FIXED) {
         FragmentState s = *st;
         fold_fixed_formats(s);
+        if constexpr (OP >= 0) {
+            s.depth_test = 1;
+            s.depth_compare = OP;
+            s.stencil_test = 0;
+            s.depth_bounds_test = 0;
+        }
         r.write = depth_stencil_ops(s, r.px, depth, front != 0) ? 1u : 0u;
     } else {
         r.write = depth_stencil_ops(*st, r.px, depth, front != 0) ? 1u : 0u;

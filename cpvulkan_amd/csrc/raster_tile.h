@@ -103,13 +103,23 @@ __device__ __forceinline__ void wave_lds_fence()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ANYPRIM: the launch may hold point and line records besides triangles
-// (one 16x16 / four-wave instantiation per fragment shader carries it).
-// FIXED: every pipeline and target of the launch has the fixed formats
-// (fold_fixed_formats).
-template <class FS, int TILE, int WAVES, bool ANYPRIM = false, bool FIXED = false>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(raster_waves_per_eu<FS, TILE, WAVES>()))) void
-raster_kernel(RasterArgs a)
+// Staging area of a workgroup: the visit fields of a chunk's candidates,
+// one region per wave, used with several blocks per lane (see GATHER).
+template <class FS, int TILE, int WAVES>
+struct RasterStage {
+    static constexpr int kBlocksPerLane = (TILE / 8) * (TILE / 8) / WAVES;
+    static constexpr bool kOn = !FS::kMayDiscard && kBlocksPerLane > 1;
+    uint4 v[kOn ? WAVES : 1][kOn ? kVisitParts : 1][64];
+};
+
+// The kernel body (raster_kernel below).  OP >= 0 is the launch's depth
+// compare op as a compile-time constant, for launches whose every draw has
+// the depth test with that op and no stencil or depth-bounds test (the
+// launches with hierarchical culling): the depth interval tests and the
+// depth compare then fold to single comparisons.  OP = -1 reads the ops at
+// run time.
+template <class FS, int TILE, int WAVES, bool ANYPRIM, bool FIXED, int OP>
+__device__ __forceinline__ void raster_body(const RasterArgs& a, RasterStage<FS, TILE, WAVES>& stage_area)
 {
     constexpr int BW = TILE / 8;               // 8x8 blocks per tile row
     constexpr int BLOCKS = BW * BW;
@@ -137,9 +147,11 @@ raster_kernel(RasterArgs a)
     // reads the whole record per fragment and keeps the scalar loads too.
     // Every configuration loads the next chunk's pairs while it works on the
     // current one.
-    constexpr bool GATHER = !FS::kMayDiscard && BPW > 1;
+    constexpr bool GATHER = RasterStage<FS, TILE, WAVES>::kOn;
     constexpr bool STAGE = GATHER;
-    __shared__ uint4 stage[STAGE ? WAVES : 1][STAGE ? kVisitParts : 1][64];
+    auto& stage = stage_area.v;
+    // the hierarchical depth op (-1: no hierarchical culling)
+    const int32_t hiz_op = OP >= 0 ? OP : a.hiz_op;
     uint32_t tried = 0;   // queues found empty, starting at home
 
     for (;;) {
@@ -284,7 +296,7 @@ raster_kernel(RasterArgs a)
             // culling: a primitive whose depth interval fails the test
             // against all of them cannot change any of them.
             float bmin = __builtin_inff(), bmax = -__builtin_inff();
-            if (a.hiz_op >= 0) {
+            if (hiz_op >= 0) {
 #pragma unroll
                 for (int k = 0; k < BPW; k++) {
                     if (inside[k]) {
@@ -346,7 +358,7 @@ raster_kernel(RasterArgs a)
                 const uint32_t nb = base + 64u + (uint32_t)lane;
                 e_next = nb < lend ? a.tile_prims[nb] : 0u;
             }
-            if (a.hiz_op >= 0) {
+            if (hiz_op >= 0) {
                 if (ov) {
                     float lo, hi;
                     if constexpr (GATHER) {
@@ -357,7 +369,7 @@ raster_kernel(RasterArgs a)
                         lo = dz.x;
                         hi = dz.y;
                     }
-                    ov = !hiz_rejects(a.hiz_op, lo, hi, bmin, bmax);
+                    ov = !hiz_rejects(hiz_op, lo, hi, bmin, bmax);
                 }
             }
             uint64_t mask = __ballot(ov);
@@ -414,7 +426,7 @@ raster_kernel(RasterArgs a)
                                 !fst->depth_bounds_test && !fst->stencil_test;
                         depth_kind = fst->ds.depth_kind;
                     }
-                    depth_op = fst->depth_compare;
+                    depth_op = OP >= 0 ? OP : fst->depth_compare;
                 }
 #pragma unroll
                 for (int k = 0; k < BPW; k++) {
@@ -433,9 +445,9 @@ raster_kernel(RasterArgs a)
                     }
                     const int x = pxx[k], y = pxy[k];
                     if (x < r.x0 || x >= r.x1 || y < r.y0 || y >= r.y1) continue;
-                    if (a.hiz_op >= 0) {
+                    if (hiz_op >= 0) {
                         const float cd = stored_depth(FIXED ? 32 : a.hiz_depth_kind, px[k].ds0);
-                        if (hiz_rejects(a.hiz_op, r.dt_lo, r.dt_hi, cd, cd)) continue;
+                        if (hiz_rejects(hiz_op, r.dt_lo, r.dt_hi, cd, cd)) continue;
                     }
                     float w0, w1, w2, depth;
                     if (ANYPRIM && (r.flags & kPrimPoint)) {
@@ -488,11 +500,11 @@ raster_kernel(RasterArgs a)
                                              depth, 1.0f};
                         float colour[4];
                         if (FS::run(*env, in, fc, front, colour)) continue;
-                        const DepthStencilResult dr = depth_stencil_call<FIXED>(fst, px[k], dt, front ? 1u : 0u);
+                        const DepthStencilResult dr = depth_stencil_call<FIXED, OP>(fst, px[k], dt, front ? 1u : 0u);
                         px[k] = dr.px;
                         if (dr.write && (FIXED || fst->has_color)) color_write_state<FIXED>(*fst, px[k], colour);
                     } else {
-                        const DepthStencilResult dr = depth_stencil_call<FIXED>(fst, px[k], dt, front ? 1u : 0u);
+                        const DepthStencilResult dr = depth_stencil_call<FIXED, OP>(fst, px[k], dt, front ? 1u : 0u);
                         px[k] = dr.px;
                         if (!(dr.write && (FIXED || fst->has_color))) continue;
                         const uint32_t tag = gprim | ((uint32_t)k << 28);
@@ -529,6 +541,25 @@ raster_kernel(RasterArgs a)
 
         if (!a.persistent) break;
     }
+}
+
+// ANYPRIM: the launch may hold point and line records besides triangles
+// (one 16x16 / four-wave instantiation per fragment shader carries it).
+// FIXED: every pipeline and target of the launch has the fixed formats
+// (fold_fixed_formats).  The fixed-format kernels carry a second copy of
+// the body for depth op LESS, the common one.
+template <class FS, int TILE, int WAVES, bool ANYPRIM = false, bool FIXED = false>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(raster_waves_per_eu<FS, TILE, WAVES>()))) void
+raster_kernel(RasterArgs a)
+{
+    __shared__ RasterStage<FS, TILE, WAVES> stage;
+    if constexpr (FIXED) {
+        if (a.hiz_op == 1) {
+            raster_body<FS, TILE, WAVES, ANYPRIM, FIXED, 1>(a, stage);
+            return;
+        }
+    }
+    raster_body<FS, TILE, WAVES, ANYPRIM, FIXED, -1>(a, stage);
 }
 
 }  // namespace cpv
