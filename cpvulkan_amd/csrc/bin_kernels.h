@@ -501,6 +501,26 @@ __device__ __forceinline__ float pixel_centre(int p, float extent, float half)
     return ((float)p / extent + half) * 2.0f - 1.0f;
 }
 
+// An interval [lo, hi] holding pixel_centre(p) for every p0 <= p <= p1 (0
+// <= p < 2^24), from multiplications by 2 * half = fl(1 / extent) instead
+// of divisions.  fl(p * fl(1/e)) is within 1.5 * 2^-23 |p / e| of
+// fl(p / e), and the rounded steps after it (+ half, * 2, - 1) keep the
+// difference within 2^-23 (3.5 |c| + 7 |half| + 3.5) of the exact centre
+// c, which the margin 2^-20 (|c'| + 4) exceeds, with room for its own and
+// the subtraction's rounding, when |half| <= 1, i.e. |extent| >= 0.5 (the
+// caller checks).  The centres are monotone in p, so every one lies
+// between the two ends'.
+__device__ __forceinline__ void centre_range(int p0, int p1, float half, float& lo, float& hi)
+{
+    const float inv = 2.0f * half;
+    const float a = ((float)p0 * inv + half) * 2.0f - 1.0f;
+    const float b = ((float)p1 * inv + half) * 2.0f - 1.0f;
+    const float ea = fmaf(fabsf(a), 0x1p-20f, 0x1p-18f);
+    const float eb = fmaf(fabsf(b), 0x1p-20f, 0x1p-18f);
+    lo = fminf(a - ea, b - eb);
+    hi = fmaxf(a + ea, b + eb);
+}
+
 // pair_blocks less, for a triangle, the quarters where it covers no pixel
 // (points and lines keep their box's quarters).  box is relative to the
 // primitive's first tile, whose pixel origin is (ox, oy); e holds the
@@ -518,7 +538,9 @@ __device__ __forceinline__ float pixel_centre(int p, float extent, float half)
 // that is negative every pixel's is (a pixel's NaN needs A = B = +-inf,
 // which makes the bound +inf or NaN) and the quarter is dropped.  NaN is
 // not < 0, and non-finite terms keep every quarter.  Each edge's bound is
-// computed once per quarter column and row.
+// computed once per quarter column and row, from an interval holding the
+// column's (row's) centres: centre_range's, or for viewport extents below
+// 0.5 in magnitude the exact centres of its two ends.
 __device__ __forceinline__ uint32_t pair_blocks_covered(int tile_shift, int4 box, uint32_t col, uint32_t row, bool triangle,
                                                         const float4 e[3], float4 vp, int ox, int oy)
 {
@@ -534,26 +556,29 @@ __device__ __forceinline__ uint32_t pair_blocks_covered(int tile_shift, int4 box
     for (int i = 0; i < 3; i++)
         finite = finite & __builtin_isfinite(ax[i]) & __builtin_isfinite(ay[i]) & __builtin_isfinite(dx[i]) &
                  __builtin_isfinite(dy[i]);
+    // NaN extents fail the test and take the exact centres
+    const bool approx = fabsf(vp.x) >= 0.5f && fabsf(vp.y) >= 0.5f;
+    auto range = [&](int p0, int p1, float extent, float hf, float& lo, float& hi) {
+        if (approx) {
+            centre_range(p0, p1, hf, lo, hi);
+        } else {
+            const float u = pixel_centre(p0, extent, hf), v = pixel_centre(p1, extent, hf);
+            lo = fminf(u, v);
+            hi = fmaxf(u, v);
+            finite = finite & __builtin_isfinite(u) & __builtin_isfinite(v);
+        }
+        finite = finite & __builtin_isfinite(lo) & __builtin_isfinite(hi);
+    };
     float xlo[2], xhi[2], ylo[2], yhi[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         xlo[h] = xhi[h] = ylo[h] = yhi[h] = 0.0f;
         // quarter column h is in the box when a quarter of it is (bits h, h + 2)
-        if (bits & (5u << h)) {
-            const float u = pixel_centre(ox + max(X + half * h, box.x), vp.x, vp.z);
-            const float v = pixel_centre(ox + min(X + half * h + half, box.z) - 1, vp.x, vp.z);
-            finite = finite & __builtin_isfinite(u) & __builtin_isfinite(v);
-            xlo[h] = fminf(u, v);
-            xhi[h] = fmaxf(u, v);
-        }
+        if (bits & (5u << h))
+            range(ox + max(X + half * h, box.x), ox + min(X + half * h + half, box.z) - 1, vp.x, vp.z, xlo[h], xhi[h]);
         // quarter row h (bits 2h, 2h + 1)
-        if (bits & (3u << (2 * h))) {
-            const float u = pixel_centre(oy + max(Y + half * h, box.y), vp.y, vp.w);
-            const float v = pixel_centre(oy + min(Y + half * h + half, box.w) - 1, vp.y, vp.w);
-            finite = finite & __builtin_isfinite(u) & __builtin_isfinite(v);
-            ylo[h] = fminf(u, v);
-            yhi[h] = fmaxf(u, v);
-        }
+        if (bits & (3u << (2 * h)))
+            range(oy + max(Y + half * h, box.y), oy + min(Y + half * h + half, box.w) - 1, vp.y, vp.w, ylo[h], yhi[h]);
     }
     if (!finite) return m;
     float amax[3][2], bmin[3][2];
@@ -641,7 +666,10 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a)
             const uint32_t k = j - s_off[lo];
             const uint32_t wl = s_w[lo];
             const uint32_t w = wl & ~kNoEdgeTest;
-            const uint32_t row = k / w;
+            // row = k / w: k < kEmitBig = 2048 and w <= 2048, so (k + 1/2) / w
+            // is at least 1/(2w) from an integer while the product with
+            // v_rcp_f32's reciprocal (1 ulp) errs by under 1.5 * 2^-23 * 2049 / w
+            const uint32_t row = (uint32_t)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
             const uint32_t out = s_dst[lo] + k;
             if (out < a.cap) {
                 const uint32_t col = k - row * w;

@@ -100,3 +100,40 @@ def test_non_finite_edges_never_drop():
     with np.errstate(all="ignore"):
         b = corner_bound(xs, ys, f32(0.0), f32(0.0), big, big * f32(4))
     assert not (b < 0)
+
+
+def centre_range(p0, p1, extent):
+    """The emit kernels' interval for the centres of pixels p0..p1
+    (bin_kernels.h, centre_range): products with fl(1 / extent) and a
+    margin of 2^-20 (|c'| + 4), fmaf(|c'|, 2^-20, 2^-18) rounded once."""
+    half = f32(f32(1.0) / f32(extent)) * f32(0.5)
+    inv = f32(2.0) * half
+
+    def approx(p):
+        return (f32(p) * inv + half) * f32(2.0) - f32(1.0)
+
+    def margin(c):
+        return f32(np.float64(abs(c)) * 2.0 ** -20 + 2.0 ** -18)
+
+    a, b = approx(p0), approx(p1)
+    return min(a - margin(a), b - margin(b)), max(a + margin(a), b + margin(b))
+
+
+def test_centre_range_holds_every_centre():
+    """Every exact centre (division) of the range lies inside the interval
+    computed with the reciprocal, for extents down to 0.5 in magnitude,
+    negative heights and pixels up to 32767; and the interval is tight
+    (a few 1e-6 beyond the exact ends)."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    checked = 0
+    for extent in (0.5, -0.5, 0.75, 1.0, 3.0, 37.0, 64.0, 1080.0, -1080.0, 1920.0, 4096.0, 16384.0, 33.3, -48.0):
+        for _ in range(60):
+            p0 = int(rng.integers(0, 32760))
+            p1 = p0 + int(rng.integers(0, 8))
+            lo, hi = centre_range(p0, p1, extent)
+            exact = np.array([centre(p, extent) for p in range(p0, p1 + 1)], f32)
+            assert (exact >= lo).all() and (exact <= hi).all(), (extent, p0, p1, lo, hi, exact)
+            slack = max(float(exact.min() - lo), float(hi - exact.max()))
+            assert slack <= 2.0 ** -17 * float(np.abs(exact).max()) + 2.0 ** -15
+            checked += 1
+    assert checked == 14 * 60
