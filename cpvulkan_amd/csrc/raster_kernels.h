@@ -231,17 +231,13 @@ __device__ __forceinline__ PixelState shade_fragment(const TriSetup* recs, const
                         : (ANYPRIM && (g.flags & kPrimLine)) ? g.z[0] * w1 + g.z[1] * (1.0f - w1)
                                                                : g.z[0] * w0 + g.z[1] * w1 + g.z[2] * w2;
     float colour[4];
-    if (uniform_fstate >= 0) {
-        const FragmentState& fs = fstates[uniform_fstate];
-        const float fc[4] = {fs.origin_upper_left ? (float)x : d.vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
-        FS::run(de, in, fc, (g.flags & 1u) != 0, colour);
-        color_write_state<FIXED>(fs, ps, colour);
-    } else {
-        const FragmentState& fs = fstates[d.fstate];
-        const float fc[4] = {fs.origin_upper_left ? (float)x : d.vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
-        FS::run(de, in, fc, (g.flags & 1u) != 0, colour);
-        color_write_state<FIXED>(fs, ps, colour);
-    }
+    // one copy of the shader body (it reads no fragment state); only the
+    // colour write is taken twice
+    const int32_t upper_left = uniform_fstate >= 0 ? fstates[uniform_fstate].origin_upper_left : fstates[d.fstate].origin_upper_left;
+    const float fc[4] = {upper_left ? (float)x : d.vp_w - (float)x - 1.0f, (float)y, depth, 1.0f};
+    FS::run(de, in, fc, (g.flags & 1u) != 0, colour);
+    if (uniform_fstate >= 0) color_write_state<FIXED>(fstates[uniform_fstate], ps, colour);
+    else color_write_state<FIXED>(fstates[d.fstate], ps, colour);
     return ps;
 }
 
