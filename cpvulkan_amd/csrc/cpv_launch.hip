@@ -252,6 +252,7 @@ struct Exec {
     hipEvent_t r0 = nullptr, r1 = nullptr;
     uint32_t* overflow = nullptr; // overflow word shared by several launches (nullptr: the launch's own)
     unsigned long long* fragments = nullptr;   // covered-fragment counter (nullptr: the launch's own)
+    const uint32_t* fixed_order = nullptr;     // learned tile order computed earlier (nullptr: computed here)
 };
 
 // Exclusive scan of n counts into out (out[n] = the total when write_total);
@@ -430,17 +431,21 @@ int enqueue_bin(Exec& x, const LaunchPlan& P, uint8_t* arena, uint32_t pair_cap,
                        (const uint32_t*)(stats + kStatPairs), P.tiles, b.tile_start.as<uint32_t>());
     HIP_TRY(hipGetLastError());
 
-    // scheduling
+    // scheduling (a recorded batch computes its learned order once, when it
+    // is recorded, and its submits reuse it)
     const int sched = effective_scheduler(x);
     const uint32_t* order = nullptr;
-    if (needs_features(x) && P.tiles) {
+    const bool reuse_order = sched == CPV_SCHED_PREDICTED && x.fixed_order;
+    if (needs_features(x) && P.tiles && !(reuse_order && !x.cfg.record_costs)) {
         FeatureArgs fa{b.recs.as<TriSetup>(), b.vals[src].as<uint32_t>(), b.tile_start.as<uint32_t>(), d_tp, d_dd, d_fs,
                        b.features.as<float>(), P.tiles, (uint32_t)P.targets.size(), (int32_t)x.cfg.tile_size,
                        (int32_t)x.cfg.waves_per_tile, P.hiz_op >= 0 ? 1 : 0, 0};
         hipLaunchKernelGGL(tile_features_kernel, dim3((P.tiles + 3) / 4), dim3(256), 0, x.s, fa);
         HIP_TRY(hipGetLastError());
     }
-    if (sched == CPV_SCHED_PREDICTED && P.tiles) {
+    if (reuse_order && P.tiles) {
+        order = x.fixed_order;
+    } else if (sched == CPV_SCHED_PREDICTED && P.tiles) {
         rc = mlp_enqueue_predict(x.predictor, b.features.as<float>(), P.tiles, b.pred.as<float>(), x.s);
         if (rc) return rc;
         fill_u32(x.s, stats + kStatPredMax, 0u, 1);
@@ -1364,6 +1369,7 @@ struct cpv_batch {
     cpvrt::DevBuf arena;
     std::vector<LaunchPlan> plans;
     std::vector<uint32_t> caps;
+    std::vector<cpvrt::DevBuf> order;   // per launch: the learned tile order (PREDICTED), computed once
     Set set[2];
     hipEvent_t e0 = nullptr, e1 = nullptr, pre = nullptr;
     uint64_t submits = 0;
@@ -1392,6 +1398,7 @@ static void free_batch(cpv_batch* B)
         if (S.raster_done) (void)hipEventDestroy(S.raster_done);
         for (cpvrt::Buffers& b : S.b) b.release_all();
     }
+    for (cpvrt::DevBuf& o : B->order) o.release();
     for (hipEvent_t e : {B->e0, B->e1, B->pre})
         if (e) (void)hipEventDestroy(e);
     B->arena.release();
@@ -1428,6 +1435,45 @@ static int capture(hipStream_t s, hipGraphExec_t* out, F&& body)
 // Sizes both sets' buffers for the batch's capacities, clears their control
 // words, and records the four graphs: no allocation, no host sync and no
 // memset node inside a capture.
+// A submit's front half of set S on the context's front stream (what the
+// set's front graph holds): every launch's vertex stage, setup and binning,
+// into the set's buffers, with the launches' shared overflow word cleared
+// first.  bins receives where each launch's binning leaves its results.
+static int batch_front(cpv_context* c, cpv_batch* B, cpv_batch::Set& S, std::vector<Binned>& bins)
+{
+    uint8_t* arena = B->arena.as<uint8_t>();
+    fill_u32(c->front_stream, S.overflow(), 0u, 1);
+    for (size_t g = 0; g < B->plans.size(); g++) {
+        Exec x = context_exec(c, &S.b[g]);
+        x.cfg = B->cfg;
+        x.s = c->front_stream;
+        x.frozen = true;
+        x.overflow = S.overflow();
+        if (!B->order.empty() && B->order[g].p) x.fixed_order = B->order[g].as<uint32_t>();
+        fill_u32(x.s, S.b[g].stats.p, 0u, 2);   // the fragment counter restarts
+        int r = enqueue_front(x, B->plans[g], arena);
+        if (r == CPV_OK) r = enqueue_bin(x, B->plans[g], arena, B->caps[g], &bins[g]);
+        if (r) return r;
+    }
+    return CPV_OK;
+}
+
+// The raster kernels of a submit of set S on the context's stream (what the
+// set's raster graph holds).
+static int batch_raster(cpv_context* c, cpv_batch* B, cpv_batch::Set& S, const std::vector<Binned>& bins)
+{
+    uint8_t* arena = B->arena.as<uint8_t>();
+    for (size_t g = 0; g < B->plans.size(); g++) {
+        Exec x = context_exec(c, &S.b[g]);
+        x.cfg = B->cfg;
+        x.frozen = true;
+        x.overflow = S.overflow();
+        const int r = enqueue_raster(x, B->plans[g], arena, bins[g]);
+        if (r) return r;
+    }
+    return CPV_OK;
+}
+
 static int batch_record(cpv_context* c, cpv_batch* B)
 {
     const size_t G = B->plans.size();
@@ -1444,35 +1490,36 @@ static int batch_record(cpv_context* c, cpv_batch* B)
         }
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
+    // The learned order: the front half and binning of every launch run
+    // once, eagerly, on the first set's buffers, and the order they give is
+    // kept for every submit.  It is a schedule only (tiles are disjoint), so
+    // vertex data that changes between submits leaves the frames exact.
+    Exec probe = context_exec(c, nullptr);
+    probe.cfg = B->cfg;
+    if (effective_scheduler(probe) == CPV_SCHED_PREDICTED && B->order.empty()) {
+        B->order.resize(G);
+        for (size_t g = 0; g < G; g++) {
+            Exec x = context_exec(c, &B->set[0].b[g]);
+            x.cfg = B->cfg;
+            Binned bn;
+            int rc = enqueue_front(x, B->plans[g], arena);
+            if (rc == CPV_OK) rc = enqueue_bin(x, B->plans[g], arena, B->caps[g], &bn);
+            if (rc) return rc;
+            if (bn.order) {
+                if (B->order[g].ensure((size_t)B->plans[g].tiles * 4)) return CPV_ERROR_OUT_OF_MEMORY;
+                HIP_TRY(hipMemcpyAsync(B->order[g].p, bn.order, (size_t)B->plans[g].tiles * 4, hipMemcpyDeviceToDevice, c->stream));
+            }
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (cpv_batch::Set& S : B->set)
+            for (size_t g = 0; g < G; g++) fill_u32(c->stream, S.b[g].stats.p, 0u, kStatAlloc);
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     for (cpv_batch::Set& S : B->set) {
         std::vector<Binned> bins(G);
-        int rc = capture(c->front_stream, &S.front, [&]() {
-            fill_u32(c->front_stream, S.overflow(), 0u, 1);
-            for (size_t g = 0; g < G; g++) {
-                Exec x = context_exec(c, &S.b[g]);
-                x.cfg = B->cfg;
-                x.s = c->front_stream;
-                x.frozen = true;
-                x.overflow = S.overflow();
-                fill_u32(x.s, S.b[g].stats.p, 0u, 2);   // the fragment counter restarts
-                int r = enqueue_front(x, B->plans[g], arena);
-                if (r == CPV_OK) r = enqueue_bin(x, B->plans[g], arena, B->caps[g], &bins[g]);
-                if (r) return r;
-            }
-            return (int)CPV_OK;
-        });
+        int rc = capture(c->front_stream, &S.front, [&]() { return batch_front(c, B, S, bins); });
         if (rc) return rc;
-        rc = capture(c->stream, &S.raster, [&]() {
-            for (size_t g = 0; g < G; g++) {
-                Exec x = context_exec(c, &S.b[g]);
-                x.cfg = B->cfg;
-                x.frozen = true;
-                x.overflow = S.overflow();
-                const int r = enqueue_raster(x, B->plans[g], arena, bins[g]);
-                if (r) return r;
-            }
-            return (int)CPV_OK;
-        });
+        rc = capture(c->stream, &S.raster, [&]() { return batch_raster(c, B, S, bins); });
         if (rc) return rc;
     }
     return CPV_OK;
