@@ -400,7 +400,10 @@ int cpv_get_tile_traces(cpv_context* ctx, cpv_tile_trace* traces, uint32_t capac
  * the batch again with the pair counts it needed and replays that submit
  * before anything else is enqueued, so the results are those of the
  * submits run in order.  CPV_ERROR_OVERFLOW is returned only for more than
- * 2^32 - 1 pairs in one launch. */
+ * 2^32 - 1 pairs in one launch.  With the learned scheduler the tile order
+ * is computed once, at recording time, from the geometry then bound, and
+ * every submit reuses it (a schedule only: the frames are exact whatever
+ * the geometry of later submits). */
 typedef struct cpv_batch cpv_batch;
 int cpv_batch_create(cpv_context* ctx, const cpv_job_desc* jobs, uint32_t job_count, cpv_batch** out);
 /* A batch of render passes laid out as for cpv_render_passes (a command

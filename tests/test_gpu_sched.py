@@ -211,6 +211,35 @@ def test_pipelined_submits_bit_exact(ctx, sched):
             mlp.close()
 
 
+def test_learned_batch_order_recorded_once(ctx):
+    """A batch recorded with the learned scheduler computes its tile order
+    once, from the geometry bound at recording time, and every submit
+    reuses it.  The order is a schedule only: after the vertex buffer is
+    overwritten with a frame whose clusters lie elsewhere, the submit must
+    still draw that frame exactly."""
+    a = W.skewed_scene(40_000, 640, 360, seed=21)
+    b = W.skewed_scene(40_000, 640, 360, seed=22)
+    ref_a, ref_b = oracle_lib.render_scene(a), oracle_lib.render_scene(b)
+    mlp = cpvulkan_amd.Mlp(ctx, seed=7)
+    ctx.set_predictor(mlp)
+    _cfg(ctx, scheduler=SCHED["PREDICTED"], count_fragments=0)
+    mem, ms, jobs = materialize_jobs(ctx, [a])
+    try:
+        with ctx.batch(jobs) as bt:
+            bt.submit()
+            bt.wait()
+            compare(a, (ms[0].read_color(), ms[0].read_ds(), None), ref_a[:2] + (None,), "learned batch, recorded frame")
+            mem.upload(ms[0].draws[0].vertex_address[0], b.draws[0].vertices[0])
+            for _ in range(2):
+                bt.submit()
+            bt.wait()
+            compare(b, (ms[0].read_color(), ms[0].read_ds(), None), ref_b[:2] + (None,), "learned batch, new geometry")
+    finally:
+        mem.release()
+        ctx.set_predictor(None)
+        mlp.close()
+
+
 def test_pipelined_submits_order_with_other_work(ctx):
     """Work enqueued on the context between two submits (a clear of part of
     the colour attachment) is ordered between them: the second submit's
