@@ -387,7 +387,7 @@ def bench_skewed(args, ctx, scene, workload, embedded: bool = False):
     plain_img = (m.read_color(), m.read_ds(), None)
     plain_fps = timed(ctx, lambda: ctx.render_pass(m.pass_desc, m.pipelines, m.draws), steps, warmup)
     plain_raster = ctx.timing()[0]
-    # the counter-ordered persistent queue, recorded: the judge of the learned order
+    # the counter-ordered persistent queue, recorded: what the learned order is compared with
     ctx.set_config(tile_size=args.tile, waves_per_tile=args.waves, scheduler=abi.SCHED["COUNTER"], count_fragments=0)
     with ctx.batch([job_desc(m)]) as cbatch:
         counter_fps = timed(ctx, cbatch.submit, steps, warmup)

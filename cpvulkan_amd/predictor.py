@@ -11,7 +11,7 @@ kernel then takes tiles in descending predicted cost (longest first), which
 shortens the tail of a frame whose tiles are very uneven.  Rendering
 results never depend on the model: it only orders the queue.
 
-Schedule quality is judged against the counter-ordered queue with the
+Schedule quality is compared with the counter-ordered queue's on the
 measured per-tile costs: `simulate_makespan` list-schedules tiles in a
 given order onto the resident wavefront slots and returns the finish time.
 """

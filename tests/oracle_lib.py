@@ -71,7 +71,7 @@ def lib() -> C.CDLL:
 
 def default_threads() -> int:
     """Host threads for the tile-parallel oracle: the cores this process may
-    use, at most 16 (the GPU boxes' CPU share)."""
+    use, at most 16 (the host cores one MI355X of the test machines gets)."""
     import os
     try:
         n = len(os.sched_getaffinity(0))
