@@ -201,13 +201,20 @@ __device__ __forceinline__ void fragment_inputs(const Rec& g, const float* vout,
         for (int c = 0; c < FS::kInputs; c++) in[c] = interpolate_line(FS::interp(c), w, pw, q0[c], q1[c], qp[c]);
         return;
     }
-    // x / 1.0f == x exactly, so clip w == 1 drops the divisions
+    // x / 1.0f == x exactly, so clip w == 1 drops the divisions.  One
+    // branch around all inputs: with a select per input the compiler
+    // computes the perspective denominator (three divisions) for every
+    // fragment.
     const bool unit_w = pw[0] == 1.0f && pw[1] == 1.0f && pw[2] == 1.0f;
+    if (unit_w) {
 #pragma unroll
-    for (int c = 0; c < FS::kInputs; c++) {
-        const int mode = FS::interp(c);
-        in[c] = (mode == 0 && unit_w) ? interpolate_unit_w(w, q0[c], q1[c], q2[c])
-                                      : interpolate(mode, w, pw, q0[c], q1[c], q2[c], qp[c]);
+        for (int c = 0; c < FS::kInputs; c++) {
+            const int mode = FS::interp(c);
+            in[c] = mode == 0 ? interpolate_unit_w(w, q0[c], q1[c], q2[c]) : interpolate(mode, w, pw, q0[c], q1[c], q2[c], qp[c]);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < FS::kInputs; c++) in[c] = interpolate(FS::interp(c), w, pw, q0[c], q1[c], q2[c], qp[c]);
     }
 }
 

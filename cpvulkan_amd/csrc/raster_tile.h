@@ -243,8 +243,12 @@ __device__ __forceinline__ void raster_body(const RasterArgs& a, RasterStage<FS,
         bool vp_identity = true;
         const FragmentState* fst = nullptr;
         const DrawEnv* env = nullptr;
-        bool early = false;   // conservative depth rejection allowed for this draw
-        int32_t depth_op = 7;
+        // Conservative depth rejection allowed for this draw, and its op.
+        // With the op a constant (OP >= 0: every draw has the depth test
+        // with op OP and no stencil or depth-bounds test) both are constants
+        // from the start, so the rejection test folds to one compare.
+        bool early = OP >= 0;
+        int32_t depth_op = OP >= 0 ? OP : 7;
         int32_t depth_kind = 0;
         unsigned long long nfrag = 0;
 
@@ -418,7 +422,9 @@ __device__ __forceinline__ void raster_body(const RasterArgs& a, RasterStage<FS,
                     // Rejecting on the depth test alone is exact when nothing
                     // else observes the fragment: no depth-bounds or stencil
                     // test (and so no stencil write).
-                    if constexpr (FIXED) {
+                    if constexpr (OP >= 0) {
+                        depth_kind = 32;
+                    } else if constexpr (FIXED) {
                         early = fst->depth_test && !fst->depth_bounds_test;
                         depth_kind = 32;
                     } else {
