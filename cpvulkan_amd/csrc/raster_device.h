@@ -551,6 +551,51 @@ CPV_DEV bool depth_surely_fails_unit(int32_t op, float cur, float e0, float e1, 
     return interval_fails(op, lo, hi, cur, cur);
 }
 
+// Bounds [lo, hi] of the depth the raster loop computes for a triangle at
+// every pixel centre in the rectangle [xlo, xhi] x [ylo, yhi] (edge values
+// e_i as in the raster loop, w_i = e_i / area correctly rounded,
+// depth = (z0 w0 + z1 w1) + z2 w2).  Returns false when it gives none (area
+// below 2^-40, a NaN or infinity anywhere, K beyond 2^100).
+//
+// In real arithmetic let E_i(x, y) = (x - ax_i) dy_i - (y - ay_i) dx_i,
+// M_i >= the largest |x - ax_i||dy_i| + |y - ay_i||dx_i| over the rectangle
+// and K = sum |z_i| M_i / area.  The five roundings of an edge value give
+// |e_i - E_i| <= 3.0001u M_i (u = 2^-24), plus 3 * 2^-149 where products
+// are subnormal, so the affine function D = sum z_i E_i / area is within
+// 3.0001u K + 2^-106 sum |z_i| of sum z_i e_i / area at every point of the
+// rectangle (area >= 2^-40).  The raster loop's depth, and the estimate da
+// of depth_surely_fails at a corner, lie within 12.2u S + 2^-120 of that
+// sum, with S <= K (1 + 8u).  D takes its extremes over the rectangle at
+// the corners, so every pixel's depth lies in [min da - B, max da + B] with
+// B = 2 (3.0001u + 12.2u (1 + 8u)) K + 2^-105 sum |z_i| + 2^-119, which
+// 2^-18 K + 2^-100 sum |z_i| + 2^-118 exceeds by more than the rounding of
+// K, B, lo and hi.
+CPV_DEV bool block_depth_range(const float ax[3], const float ay[3], const float dy[3], const float dx[3], float area,
+                               const float z[3], float xlo, float xhi, float ylo, float yhi, float& lo, float& hi)
+{
+    if (!(area >= 0x1p-40f)) return false;
+    const float r = __builtin_amdgcn_rcpf(area);
+    // corners (xlo, ylo), (xhi, ylo), (xlo, yhi), (xhi, yhi)
+    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, d3 = 0.0f, K = 0.0f, Z = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float pl = (xlo - ax[i]) * dy[i], ph = (xhi - ax[i]) * dy[i];
+        const float ql = (ylo - ay[i]) * dx[i], qh = (yhi - ay[i]) * dx[i];
+        d0 += z[i] * ((pl - ql) * r);
+        d1 += z[i] * ((ph - ql) * r);
+        d2 += z[i] * ((pl - qh) * r);
+        d3 += z[i] * ((ph - qh) * r);
+        K += fabsf(z[i]) * (fmaxf(fabsf(pl), fabsf(ph)) + fmaxf(fabsf(ql), fabsf(qh)));
+        Z += fabsf(z[i]);
+    }
+    K = K * r;
+    if (!(K < 0x1p100f) || !__builtin_isfinite((d0 + d1) + (d2 + d3))) return false;
+    const float B = (K * 0x1p-18f + Z * 0x1p-100f) + 0x1p-118f;
+    lo = fminf(fminf(d0, d1), fminf(d2, d3)) - B;
+    hi = fmaxf(fmaxf(d0, d1), fmaxf(d2, d3)) + B;
+    return __builtin_isfinite(lo) && __builtin_isfinite(hi);
+}
+
 // Colour write of CompileWriteFragmentAttachment/Blend (PipelineCompiler.cpp:1528-1727)
 // with blending per ApplyBlend.
 CPV_DEV void color_write(const FragmentState& st, PixelState& px, const float colour[4])

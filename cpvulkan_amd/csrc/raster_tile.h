@@ -149,6 +149,9 @@ __device__ __forceinline__ void raster_body(const RasterArgs& a, RasterStage<FS,
     // current one.
     constexpr bool GATHER = RasterStage<FS, TILE, WAVES>::kOn;
     constexpr bool STAGE = GATHER;
+    // block-local depth bounds of triangles, where a lane holds one pixel
+    // of an 8x8 block (lane = row * 8 + column)
+    constexpr bool BLOCK_DEPTH = !ANYPRIM && !GATHER && BPW == 1;
     auto& stage = stage_area.v;
     // the hierarchical depth op (-1: no hierarchical culling)
     const int32_t hiz_op = OP >= 0 ? OP : a.hiz_op;
@@ -345,7 +348,7 @@ __device__ __forceinline__ void raster_body(const RasterArgs& a, RasterStage<FS,
             }
             // the candidates' visit fields, all lanes at once, straight into
             // the wave's LDS region (global_load_lds: no registers)
-            float2 dz = make_float2(0.0f, 0.0f);
+            float4 dz = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // dt_lo, dt_hi, area, draw
             if constexpr (GATHER) {
                 wave_lds_fence();   // the previous chunk's reads come first
                 if (ov) {
@@ -354,7 +357,7 @@ __device__ __forceinline__ void raster_body(const RasterArgs& a, RasterStage<FS,
                     for (int p = 0; p < kVisitParts; p++)
                         __builtin_amdgcn_global_load_lds((const void*)(src + p),
                                                          (__attribute__((address_space(3))) void*)&stage[wave][p][0], 16, 0, 0);
-                    dz = *(const float2*)&a.recs[gp_l].dt_lo;
+                    dz = *(const float4*)&a.recs[gp_l].dt_lo;
                 }
             }
             // the next chunk's pairs, loaded while this chunk is worked on
@@ -363,17 +366,46 @@ __device__ __forceinline__ void raster_body(const RasterArgs& a, RasterStage<FS,
                 e_next = nb < lend ? a.tile_prims[nb] : 0u;
             }
             if (hiz_op >= 0) {
-                if (ov) {
-                    float lo, hi;
-                    if constexpr (GATHER) {
-                        lo = dz.x;
-                        hi = dz.y;
-                    } else {
-                        const float2 dz = *(const float2*)&a.recs[gp_l].dt_lo;
-                        lo = dz.x;
-                        hi = dz.y;
+                if constexpr (BLOCK_DEPTH) {
+                    // A triangle's depth bounded over the pixels it may cover
+                    // in this wave's block (block_depth_range): its box
+                    // clipped to the block, far tighter than its bounds over
+                    // all its pixels.  Taken for the candidates of the draw
+                    // last visited, whose pixel centres the lanes hold, when
+                    // that draw's depth range is [0, 1] (the depth then is
+                    // the tested value).  The record's visit fields are
+                    // loaded at once: one memory latency per chunk.
+                    uint4 p0 = make_uint4(0u, 0u, 0u, 0u), p1 = p0, p2 = p0, p3 = p0, p5 = p0;
+                    if (ov) {
+                        const uint4* src = (const uint4*)&a.recs[gp_l];
+                        p0 = src[0]; p1 = src[1]; p2 = src[2]; p3 = src[3];
+                        dz = *(const float4*)(src + 4);
+                        p5 = src[5];
+                        ov = !hiz_rejects(hiz_op, dz.x, dz.y, bmin, bmax);
                     }
-                    ov = !hiz_rejects(hiz_op, lo, hi, bmin, bmax);
+                    const bool near = cur_draw != 0xFFFFFFFFu && vp_identity;   // wave-uniform
+                    if (near) {
+                        // lane c holds column bx0 + c, lane 8 r row by0 + r
+                        const int bx0 = __builtin_amdgcn_readfirstlane(pxx[0]), by0 = __builtin_amdgcn_readfirstlane(pxy[0]);
+                        const int cl = min(max((int)p3.x - bx0, 0), 7), ch = min(max((int)p3.z - 1 - bx0, 0), 7);
+                        const int rl = min(max((int)p3.y - by0, 0), 7), rh = min(max((int)p3.w - 1 - by0, 0), 7);
+                        const float xa = __shfl(xf[0], cl, 64), xb = __shfl(xf[0], ch, 64);
+                        const float ya = __shfl(yf[0], rl * 8, 64), yb = __shfl(yf[0], rh * 8, 64);
+                        if (ov && __float_as_uint(dz.w) == cur_draw) {
+                            const float ax[3] = {__uint_as_float(p0.x), __uint_as_float(p0.y), __uint_as_float(p0.z)};
+                            const float ay[3] = {__uint_as_float(p0.w), __uint_as_float(p1.x), __uint_as_float(p1.y)};
+                            const float dy[3] = {__uint_as_float(p1.z), __uint_as_float(p1.w), __uint_as_float(p2.x)};
+                            const float dx[3] = {__uint_as_float(p2.y), __uint_as_float(p2.z), __uint_as_float(p2.w)};
+                            const float z[3] = {__uint_as_float(p5.x), __uint_as_float(p5.y), __uint_as_float(p5.z)};
+                            float lo, hi;
+                            if (block_depth_range(ax, ay, dy, dx, dz.z, z, fminf(xa, xb), fmaxf(xa, xb), fminf(ya, yb), fmaxf(ya, yb),
+                                                  lo, hi))
+                                ov = !hiz_rejects(hiz_op, lo, hi, bmin, bmax);
+                        }
+                    }
+                } else if (ov) {
+                    if constexpr (!GATHER) dz = *(const float4*)&a.recs[gp_l].dt_lo;
+                    ov = !hiz_rejects(hiz_op, dz.x, dz.y, bmin, bmax);
                 }
             }
             uint64_t mask = __ballot(ov);
