@@ -374,16 +374,20 @@ __device__ __forceinline__ void raster_body(const RasterArgs& a, RasterStage<FS,
                     // last visited, whose pixel centres the lanes hold, when
                     // that draw's depth range is [0, 1] (the depth then is
                     // the tested value).  The record's visit fields are
-                    // loaded at once: one memory latency per chunk.
+                    // loaded with the depth interval, at once: one memory
+                    // latency per chunk; before the tile's first visit
+                    // (all of a short list) only the interval is loaded.
+                    const bool near = cur_draw != 0xFFFFFFFFu && vp_identity;   // wave-uniform
                     uint4 p0 = make_uint4(0u, 0u, 0u, 0u), p1 = p0, p2 = p0, p3 = p0, p5 = p0;
                     if (ov) {
                         const uint4* src = (const uint4*)&a.recs[gp_l];
-                        p0 = src[0]; p1 = src[1]; p2 = src[2]; p3 = src[3];
+                        if (near) {
+                            p0 = src[0]; p1 = src[1]; p2 = src[2]; p3 = src[3];
+                            p5 = src[5];
+                        }
                         dz = *(const float4*)(src + 4);
-                        p5 = src[5];
                         ov = !hiz_rejects(hiz_op, dz.x, dz.y, bmin, bmax);
                     }
-                    const bool near = cur_draw != 0xFFFFFFFFu && vp_identity;   // wave-uniform
                     if (near) {
                         // lane c holds column bx0 + c, lane 8 r row by0 + r
                         const int bx0 = __builtin_amdgcn_readfirstlane(pxx[0]), by0 = __builtin_amdgcn_readfirstlane(pxy[0]);
